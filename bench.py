@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Benchmark of the MUSCL residual + explicit-RK step on one MI355X.
+
+Default workload: the multiblock explosion (2x4 blocks of 150x150 on [0,10]x[0,20], Roe,
+Venkatakrishnan, Green-Gauss, RK4 at CFL 0.8, reflecting walls, FP64).  A "step" is one full time
+step (CFL dt, four RK stages over all blocks).  ``value`` is FP64 cell-stage updates per second of
+the whole job with the state resident in HBM.  ``roofline`` prices the stage kernel's algorithmic
+bytes (compulsory FP64 state traffic per cell-stage) against the 8 TB/s HBM peak, with the kernel
+duration measured by HIP events on the launch stream.  ``cpu_baseline`` times the NumPy oracle on
+the host on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "FP64 cell-stage updates/s + % HBM peak; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+PROFILES = ROOT / "profiles"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def alg_bytes_per_cell_step(table) -> float:
+    """Compulsory FP64 state traffic per cell and step of the accumulator-form RK program: per stage
+    one read of every distinct input (stage state, U0, accumulators) and one write of every output."""
+    from pyhype_amd.backend import stage_programs
+
+    total = 0
+    for prog in stage_programs(table):
+        reads = {prog["in"]} | {o["src"] for o in prog["outs"] if o["a"] != 0.0 or o["src"] != o["dst"]}
+        writes = {o["dst"] for o in prog["outs"]}
+        total += 32 * (len(reads) + len(writes))
+    return float(total)
+
+
+def build_workload(name: str, args):
+    from pyhype_amd import Euler2D, Euler2DBatch
+    from pyhype_amd import workloads as W
+
+    tune = {"rows_per_item": args.rows, "grid_waves": args.grid}
+    if name == "explosion_batch":
+        cfgs, meshes, params = W.explosion_batch(seed=args.seed)
+        solver = Euler2DBatch(cfgs, meshes, **tune)
+        desc = {"workload": name, "cases": len(cfgs), "seed": args.seed,
+                "sides": [c["n"] for c in params["cases"]], "scheme": "Roe/Venkatakrishnan/GG, RK4, CFL 0.8"}
+        return solver, cfgs[0], meshes, desc
+    fn = getattr(W, name)
+    cfg, mesh = fn()
+    solver = Euler2D(cfg, mesh, **tune)
+    nb = len(solver.plan.blocks)
+    desc = {"workload": name, "blocks": nb, "nx": cfg.nx, "ny": cfg.ny,
+            "scheme": f"{cfg.fvm_flux_function_type}/Venkatakrishnan/GG, {cfg.time_integrator}, CFL {cfg.CFL}",
+            "geometry": "cartesian" if all(e.cartesian for e in solver.plan.blocks) else "general"}
+    return solver, cfg, mesh, desc
+
+
+def cpu_baseline(name: str, seconds: float):
+    """Oracle (NumPy restatement, single thread) on a bounded sample of the same workload."""
+    import numpy as np
+
+    from oracle import pyhype_oracle as O
+    from pyhype_amd import Euler2D
+    from pyhype_amd import workloads as W
+    from pyhype_amd.solvers import mesh_dict
+    from pyhype_amd.time_marching import butcher_table
+
+    if name == "explosion_batch":
+        cfgs, meshes, _ = W.explosion_batch(seed=0)
+        k = int(np.argmax([c.nx for c in cfgs]))
+        cfg, mesh = cfgs[k], meshes[k]
+        sample = f"case {k} ({cfg.nx}x{cfg.ny}) of the batch"
+    else:
+        cfg, mesh = getattr(W, name)()
+        sample = f"all {len(mesh_dict(mesh))} block(s) of {cfg.nx}x{cfg.ny}"
+    from pyhype_amd.mesh import BlockGeometry
+    from pyhype_amd.solvers import HostBlock
+
+    md = mesh_dict(mesh)
+    sim = O.Simulation(md, cfg.nx, cfg.ny, flux=cfg.fvm_flux_function_type,
+                       integrator=butcher_table(cfg.time_integrator, cfg), CFL=cfg.CFL, t_final=cfg.t_final,
+                       a_inf=cfg.fluid.far_field.a)
+
+    class _E:  # minimal entry for HostBlock
+        def __init__(self, n, g):
+            self.num, self.geom, self.nx, self.ny = n, g, g.nx, g.ny
+
+    states = {}
+    for n, d in md.items():
+        g = BlockGeometry({k2: d[k2] for k2 in ("SW", "SE", "NW", "NE")}, cfg.nx, cfg.ny)
+        hb = HostBlock(cfg, _E(n, g))
+        cfg.initial_condition.apply_to_block(hb)
+        states[n] = hb.state.data
+    sim.set_state(states)
+    cells = sum(cfg.nx * cfg.ny for _ in md)
+    stages = len(butcher_table(cfg.time_integrator, cfg))
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        sim.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or steps >= 50:
+            break
+    return {"value": cells * stages * steps / el, "unit": "cell-stage updates/s", "cores": 1, "kind": "port",
+            "sample": f"NumPy oracle, {sample}, {steps} step(s) from the initial state, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="explosion_multiblock")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0, help="rows per work item (0 = tuned/default)")
+    ap.add_argument("--grid", type=int, default=0, help="persistent grid in waves (0 = one per item)")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        log("bench.py runs on one GPU only: this build has no multi-GPU path (--gpus must be 1)")
+        sys.exit(2)
+
+    import torch
+
+    from pyhype_amd import workloads as W
+    from pyhype_amd.time_marching import butcher_table
+
+    default_steps = W.WORKLOADS[args.workload][1]
+    K = args.steps if args.steps is not None else default_steps
+    Wm = args.warmup
+    solver, cfg, mesh, desc = build_workload(args.workload, args)
+    plan = solver.plan
+    table = butcher_table(cfg.time_integrator, cfg)
+    S = len(table)
+    cells = plan.cells()
+    solver.apply_initial_condition()
+    log(f"workload {desc}; cells {cells}; launch {plan.launch_info()}")
+
+    use_graph = not args.no_graph
+    plan.run(Wm, use_graph=use_graph)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record()
+    plan.run(K, use_graph=use_graph, check=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ev0.elapsed_time(ev1)
+    plan.check_realizable()
+    value = cells * S * K / wall
+
+    # stage-kernel duration with HIP events on the launch stream
+    reps = max(3, min(50, int(2000 // max(1, K))))
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(S * reps)]
+    for r in range(reps):
+        for s in range(S):
+            e = evs[r * S + s]
+            e[0].record()
+            plan.stage(s)
+            e[1].record()
+    torch.cuda.synchronize()
+    stage_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / len(evs)
+    b_cell_step = alg_bytes_per_cell_step(table)
+    bytes_per_launch = cells * b_cell_step / S
+    achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = PROFILES / f"traffic_{args.workload}.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "k_stage (one RK stage over all blocks)", "stage_ms": round(stage_ms, 5),
+                "alg_bytes_per_cell_stage": b_cell_step / S}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(args.workload, args.cpu_seconds)
+        except Exception as exc:  # the baseline never blocks the GPU figure
+            log(f"cpu baseline failed: {exc!r}")
+    desc.update({"steps_timed": K, "graph": use_graph, "launch": plan.launch_info()})
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-stage updates/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic", "config": desc, "roofline": roofline, "cpu_baseline": cpu,
+        "hbm_frac_of_value": value * (b_cell_step / S) / (HBM_PEAK_GBS * 1e9), "event_ms": ev_ms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

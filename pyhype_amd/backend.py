@@ -1,0 +1,320 @@
+"""Device plan: block table, packed geometry and the C-ABI calls, with torch as the buffer carrier.
+
+``DevicePlan`` turns one or more independent cases (each a reference-style block dictionary with
+its own ``nx``/``ny``) into a ``phx_plan`` of the HIP library and owns the FP64 state tensor on the
+GPU.  All arithmetic runs in the library's kernels; this module only lays out memory and calls
+them.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _native as nat
+from .mesh import BlockGeometry
+from .states import RealizabilityError
+
+_WALLS = ("Reflection", "Slipwall")
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyhype_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch
+
+
+@dataclass
+class CaseSpec:
+    """One independent simulation: block dictionaries (mesh/base.py:201-223) and block size."""
+    mesh: Dict[int, dict]
+    nx: int
+    ny: int
+    t_final: float = float("inf")   # already scaled by a_inf
+
+
+@dataclass
+class BlockEntry:
+    case: int
+    num: int            # block number within its case (nBLK)
+    index: int          # global block index in the plan
+    geom: BlockGeometry
+    offset: int         # state offset in doubles
+    nx: int
+    ny: int
+    sides: List[int]
+    neighbors: List[int]
+    cartesian: bool
+
+
+def stage_programs(table) -> List[dict]:
+    """The accumulator-form stage programs the library derives from a Butcher table (same rule as
+    build_programs in csrc/phx_kernels.hip): buffer names "U" (state / U0), "S0", "S1" (stage
+    states) and "A<m>" (accumulators)."""
+    S = len(table)
+    P = ["U"] * S
+    free, nacc, progs = [], 0, []
+    for s in range(S):
+        inp = "U" if s == 0 else ("S0" if (s - 1) % 2 == 0 else "S1")
+        out = "U" if (s == S - 1 and S > 1) else ("S0" if s % 2 == 0 else "S1")
+        outs = [{"dst": out, "src": P[s], "a": float(table[s][s])}]
+        if P[s].startswith("A"):
+            free.append(P[s])
+        for t in range(s + 1, S):
+            c = float(table[t][s])
+            if c == 0.0:
+                continue
+            if P[t] == "U":
+                name = free.pop() if free else f"A{nacc}"
+                if name == f"A{nacc}":
+                    nacc += 1
+                outs.append({"dst": name, "src": "U", "a": c})
+                P[t] = name
+            else:
+                outs.append({"dst": P[t], "src": P[t], "a": c})
+        progs.append({"in": inp, "outs": outs})
+    return progs
+
+
+def side_kind(bc, neighbor) -> int:
+    """Ghost/outer-state rule of a side (blocks/ghost.py:128-143,187-216; fvm/base.py:305-325)."""
+    if bc is None:
+        return nat.SIDE_OPEN if neighbor is None else nat.SIDE_NEIGHBOR
+    if bc in _WALLS:
+        return nat.SIDE_WALL
+    if bc == "OutletDirichlet":
+        return nat.SIDE_OUTLET
+    raise ValueError("Boundary Condition type " + str(bc) + " has not been specialized.")
+
+
+class DevicePlan:
+    def __init__(self, cases: Sequence[CaseSpec], *, flux: str, table: List[List[float]], fluid, cfl: float,
+                 rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False):
+        torch = _torch()
+        self.torch = torch
+        self.lib = nat.load()
+        self.cases = list(cases)
+        flux_id = {"Roe": nat.FLUX_ROE, "HLLL": nat.FLUX_HLLL}.get(flux)
+        if flux_id is None:
+            raise ValueError(f"Flux function type {flux} is not available.")
+        self.flux = flux
+        self.table = table
+        S = len(table)
+        a = np.zeros((S, S))
+        for s, row in enumerate(table):
+            if len(row) != s + 1:
+                raise ValueError("Butcher table rows must have stage+1 entries")
+            a[s, : s + 1] = row
+        self._a = np.ascontiguousarray(a)
+        # blocks
+        self.blocks: List[BlockEntry] = []
+        self.case_blocks: List[Dict[int, BlockEntry]] = []
+        offset = 0
+        for c, spec in enumerate(self.cases):
+            index_of = {}
+            entries = {}
+            for n, d in spec.mesh.items():
+                g = BlockGeometry({k: d[k] for k in ("SW", "SE", "NW", "NE")}, spec.nx, spec.ny)
+                e = BlockEntry(c, n, len(self.blocks), g, offset, spec.nx, spec.ny, [], [], False)
+                offset += 4 * spec.nx * spec.ny
+                index_of[n] = e.index
+                self.blocks.append(e)
+                entries[n] = e
+            for n, d in spec.mesh.items():
+                e = entries[n]
+                for s in ("E", "W", "N", "S"):
+                    bc, nb = d["BCType" + s], d["Neighbor" + s]
+                    k = side_kind(bc, nb)
+                    if k == nat.SIDE_NEIGHBOR and nb not in index_of:
+                        raise ValueError(f"block {n}: neighbour {nb} does not exist")
+                    e.sides.append(k)
+                    e.neighbors.append(index_of[nb] if k == nat.SIDE_NEIGHBOR else -1)
+            self.case_blocks.append(entries)
+        self.state_len = offset
+        # packed geometry
+        self._geoms = []
+        descs = (nat.BlockDesc * len(self.blocks))()
+        for e, dsc in zip(self.blocks, descs):
+            packed = None if force_general else e.geom.pack_cartesian()
+            e.cartesian = packed is not None
+            if packed is None:
+                packed = e.geom.pack_general()
+            packed = np.ascontiguousarray(packed, dtype=np.float64)
+            self._geoms.append(packed)
+            dsc.nx, dsc.ny, dsc.case_id = e.nx, e.ny, e.case
+            dsc.geom_kind = nat.GEOM_CARTESIAN if e.cartesian else nat.GEOM_GENERAL
+            for s in range(4):
+                dsc.side_type[s] = e.sides[s]
+                dsc.neighbor[s] = e.neighbors[s]
+            dsc.state_offset = e.offset
+            dsc.geom = packed.ctypes.data_as(C.POINTER(C.c_double))
+            dsc.geom_len = packed.size
+        self._tfinal = np.array([c.t_final for c in self.cases], dtype=np.float64)
+        g = fluid.gamma()
+        pd = nat.PlanDesc()
+        pd.abi_version = nat.PHX_ABI_VERSION
+        pd.nblocks = len(self.blocks)
+        pd.blocks = descs
+        pd.ncases = len(self.cases)
+        pd.flux = flux_id
+        pd.limiter = nat.LIMITER_VENKATAKRISHNAN
+        pd.nstages = S
+        pd.butcher_a = self._a.ctypes.data_as(C.POINTER(C.c_double))
+        pd.gamma = g
+        pd.gm1 = fluid.gm1()
+        pd.one_over_gm1 = fluid.one_over_gm1()
+        pd.g_over_gm1 = fluid.g_over_gm1()
+        pd.cfl = cfl
+        pd.t_final = self._tfinal.ctypes.data_as(C.POINTER(C.c_double))
+        pd.rows_per_item = rows_per_item
+        pd.grid_waves = grid_waves
+        self._descs = descs
+        handle = C.c_void_p()
+        nat.check(self.lib.phx_plan_create(C.byref(pd), C.byref(handle)), "phx_plan_create")
+        self.handle = handle
+        self.U = torch.zeros(self.state_len, dtype=torch.float64, device="cuda")
+        self.nsteps_done = 0
+
+    # ------------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self.lib.phx_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ helpers
+    def stream(self):
+        return C.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+
+    @staticmethod
+    def ptr(t) -> C.c_void_p:
+        return C.c_void_p(t.data_ptr())
+
+    def block_view(self, e: BlockEntry, buf=None):
+        buf = self.U if buf is None else buf
+        return buf[e.offset:e.offset + 4 * e.nx * e.ny].view(e.ny, e.nx, 4)
+
+    def set_states(self, states: Dict[int, Dict[int, np.ndarray]]):
+        """states[case][block_num] = (ny, nx, 4) conservative array."""
+        host = np.zeros(self.state_len)
+        for e in self.blocks:
+            arr = np.asarray(states[e.case][e.num], dtype=np.float64)
+            if arr.shape != (e.ny, e.nx, 4):
+                raise ValueError(f"state of block {e.num} has shape {arr.shape}, expected {(e.ny, e.nx, 4)}")
+            host[e.offset:e.offset + arr.size] = arr.ravel()
+        self.U.copy_(self.torch.from_numpy(host))
+
+    def get_states(self, buf=None) -> List[Dict[int, np.ndarray]]:
+        host = (self.U if buf is None else buf).cpu().numpy()
+        out: List[Dict[int, np.ndarray]] = [dict() for _ in self.cases]
+        for e in self.blocks:
+            out[e.case][e.num] = host[e.offset:e.offset + 4 * e.nx * e.ny].reshape(e.ny, e.nx, 4).copy()
+        return out
+
+    # ------------------------------------------------------------------ operations
+    def reset(self, t0: Optional[Sequence[float]] = None, step0: int = 0):
+        t = np.zeros(len(self.cases)) if t0 is None else np.asarray(t0, dtype=np.float64)
+        nat.check(self.lib.phx_reset(self.handle, t.ctypes.data_as(C.POINTER(C.c_double)), step0, self.stream()),
+                  "phx_reset")
+
+    def run(self, nsteps: int, use_graph: bool = False, check: bool = True):
+        """Advance nsteps; with check, raise RealizabilityError if any case failed."""
+        nat.check(self.lib.phx_run(self.handle, self.ptr(self.U), int(nsteps), int(bool(use_graph)), self.stream()),
+                  "phx_run")
+        self.nsteps_done += nsteps
+        if check:
+            self.check_realizable()
+
+    def run_frozen(self, dt_sched, check: bool = True):
+        """dt_sched: device tensor (nsteps, ncases)."""
+        nsteps = int(dt_sched.shape[0])
+        nat.check(self.lib.phx_run_frozen(self.handle, self.ptr(self.U), nsteps, self.ptr(dt_sched), self.stream()),
+                  "phx_run_frozen")
+        self.nsteps_done += nsteps
+        if check:
+            self.check_realizable()
+
+    def stage(self, s: int):
+        """One Runge-Kutta stage of the plan's program on the current state (for timing)."""
+        nat.check(self.lib.phx_stage(self.handle, self.ptr(self.U), int(s), self.stream()), "phx_stage")
+
+    def status(self):
+        """(bad_step per case (-1 = fine), steps taken, time per case, last dt per case)."""
+        nc = len(self.cases)
+        bad = (C.c_int32 * nc)()
+        step = C.c_int32()
+        t = np.zeros(nc)
+        dt = np.zeros(nc)
+        nat.check(self.lib.phx_status(self.handle, bad, C.byref(step),
+                                      t.ctypes.data_as(C.POINTER(C.c_double)),
+                                      dt.ctypes.data_as(C.POINTER(C.c_double))), "phx_status")
+        return np.array(bad[:], dtype=np.int64), step.value, t, dt
+
+    def failed_cases(self):
+        bad = self.status()[0]
+        return {c: int(b) for c, b in enumerate(bad) if b >= 0}
+
+    def check_realizable(self):
+        failed = self.failed_cases()
+        if failed:
+            c, b = min(failed.items(), key=lambda kv: kv[1])
+            raise RealizabilityError(
+                f"ConservativeState has unrealizable values (rho <= 0 or E <= 0) in case {c} after step {b}"
+                + (f"; failed cases: {failed}" if len(failed) > 1 else ""))
+
+    def residual(self, U=None):
+        torch = self.torch
+        U = self.U if U is None else U
+        R = torch.zeros_like(U)
+        nat.check(self.lib.phx_residual(self.handle, self.ptr(U), self.ptr(R), self.stream()), "phx_residual")
+        return R
+
+    def dt(self, U=None):
+        U = self.U if U is None else U
+        out = self.torch.zeros(len(self.cases), dtype=self.torch.float64, device="cuda")
+        nat.check(self.lib.phx_dt(self.handle, self.ptr(U), self.ptr(out), self.stream()), "phx_dt")
+        return out
+
+    def ghosts(self, U=None):
+        """Ghost strips per block: dict (case, num) -> {E: (ny,1,4), W, N: (1,nx,4), S}."""
+        U = self.U if U is None else U
+        n = sum(4 * (2 * e.ny + 2 * e.nx) for e in self.blocks)
+        G = self.torch.zeros(n, dtype=self.torch.float64, device="cuda")
+        nat.check(self.lib.phx_ghosts(self.handle, self.ptr(U), self.ptr(G), self.stream()), "phx_ghosts")
+        g = G.cpu().numpy()
+        out, off = {}, 0
+        for e in self.blocks:
+            m = 2 * e.ny + 2 * e.nx
+            a = g[off:off + 4 * m].reshape(m, 4)
+            off += 4 * m
+            out[(e.case, e.num)] = {
+                "E": a[:e.ny].reshape(e.ny, 1, 4), "W": a[e.ny:2 * e.ny].reshape(e.ny, 1, 4),
+                "N": a[2 * e.ny:2 * e.ny + e.nx].reshape(1, e.nx, 4), "S": a[2 * e.ny + e.nx:].reshape(1, e.nx, 4),
+            }
+        return out
+
+    def residual_vjp(self, U, lamR):
+        lamU = self.torch.zeros_like(U)
+        nat.check(self.lib.phx_residual_vjp(self.handle, self.ptr(U), self.ptr(lamR), self.ptr(lamU), self.stream()),
+                  "phx_residual_vjp")
+        return lamU
+
+    def set_launch(self, rows_per_item: int, grid_waves: int = 0):
+        nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves)), "phx_set_launch")
+
+    def launch_info(self):
+        v = [C.c_int32() for _ in range(4)]
+        nat.check(self.lib.phx_launch_info(self.handle, *[C.byref(x) for x in v]), "phx_launch_info")
+        return {"items_cart": v[0].value, "items_gen": v[1].value, "grid_cart": v[2].value, "grid_gen": v[3].value}
+
+    def cells(self) -> int:
+        return sum(e.nx * e.ny for e in self.blocks)

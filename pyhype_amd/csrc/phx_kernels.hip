@@ -1,0 +1,1002 @@
+// phx_kernels.hip -- gfx950 kernels of the pyHype MUSCL/RK backend and the C ABI declared in
+// include/pyhype_hip.h.
+//
+// Stage kernel design ("rolling rows"): one wavefront owns a strip of 60 columns of one block
+// plus two halo lanes on each side and walks up a range of rows.  Per row it loads one row of the
+// stage state, reconstructs each cell once (Green-Gauss + Venkatakrishnan + limited face states),
+// evaluates each x-face flux once (lane j computes face j-1|j and hands it to lane j-1 through a
+// cross-lane shuffle), evaluates the y-face between this row and the previous one in registers
+// (same lane), finishes the residual of the previous row and applies every Butcher update of the
+// stage in the same pass.  Ghost cells (inter-block neighbours, reflected walls, outlets) are
+// produced on the fly from the stage state, so there is no separate ghost-fill pass.  The last
+// stage of a step also reduces the CFL time step of the next step and the realizability flag.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/pyhype_hip.h"
+#include "phx_numerics.hpp"
+
+using phx::Phys;
+using phx::Q4;
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int HALO = 2;
+constexpr int STRIP = WAVE - 2 * HALO;  // owned columns per wavefront
+constexpr int MAX_OUT = 8;
+constexpr int MAX_ACC = 8;
+constexpr unsigned long long DT_INF_BITS = 0x7FF0000000000000ull;  // +inf
+
+struct DevBlock {
+    int nx, ny, case_id, kind;
+    int side[4];
+    int nbr[4];
+    long long off;
+    const double* geom;
+    double wall[8];
+};
+
+struct WorkItem {
+    int block, j0, r0, r1;
+};
+
+struct StageArgs {
+    const DevBlock* blocks;
+    const WorkItem* items;
+    int nitems;
+    int nout;
+    int state_out;
+    const double* Uin;
+    const double* dt;
+    double* Rout;
+    double* dst[MAX_OUT];
+    const double* src[MAX_OUT];
+    double a[MAX_OUT];
+    unsigned long long* dtmin;
+    int* bad;          // per case: first step whose result was unrealizable (INT_MAX = none)
+    const int* step;   // index of the step being computed (1-based)
+    int skip_dead;     // skip the blocks of cases that failed at an earlier step
+    Phys ph;
+};
+
+// ---------------------------------------------------------------------------------------------
+// memory helpers
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ Q4 ld4(const double* p) {
+    const double2* v = reinterpret_cast<const double2*>(p);
+    const double2 a = v[0], b = v[1];
+    Q4 q;
+    q.v[0] = a.x; q.v[1] = a.y; q.v[2] = b.x; q.v[3] = b.y;
+    return q;
+}
+__device__ __forceinline__ void st4(double* p, const Q4& q) {
+    double2* v = reinterpret_cast<double2*>(p);
+    v[0] = make_double2(q.v[0], q.v[1]);
+    v[1] = make_double2(q.v[2], q.v[3]);
+}
+__device__ __forceinline__ Q4 shfl_up1(const Q4& q) {
+    Q4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.v[k] = __shfl_up(q.v[k], 1, WAVE);
+    return r;
+}
+__device__ __forceinline__ Q4 shfl_down1(const Q4& q) {
+    Q4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.v[k] = __shfl_down(q.v[k], 1, WAVE);
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// geometry accessors (layouts documented in include/pyhype_hip.h)
+// ---------------------------------------------------------------------------------------------
+struct CartG {
+    const double* g;
+    int nx, ny;
+    __device__ double col(int q, int j) const { return g[8 + q * nx + j]; }
+    __device__ double row(int q, int k) const { return g[8 + 11 * nx + q * ny + k]; }
+    __device__ double A(int k, int j) const { return g[8 + 11 * nx + 11 * ny + (long)k * nx + j]; }
+};
+enum { C_xlN, C_xlS, C_ylN, C_ylS, C_LN, C_LS, C_dxE, C_dxW, C_dxN, C_dxS, C_dx };
+enum { R_xlE, R_xlW, R_ylE, R_ylW, R_LE, R_LW, R_dyE, R_dyW, R_dyN, R_dyS, R_dy };
+
+struct GenG {
+    const double* g;
+    int nx, ny;
+    __device__ long nV() const { return (long)ny * (nx + 1); }
+    __device__ long nH() const { return (long)(ny + 1) * nx; }
+    __device__ double V(int q, int k, int jf) const { return g[q * nV() + (long)k * (nx + 1) + jf]; }
+    __device__ double H(int q, int kf, int j) const { return g[7 * nV() + q * nH() + (long)kf * nx + j]; }
+    __device__ double C(int q, int k, int j) const {
+        return g[7 * nV() + 7 * nH() + q * (long)ny * nx + (long)k * nx + j];
+    }
+};
+enum { F_L, F_xl, F_yl, F_xq, F_yq, F_c, F_s };
+enum { CL_A, CL_xc, CL_yc, CL_dx, CL_dy };
+
+template <bool CART>
+__device__ __forceinline__ void wall_cs(const DevBlock& B, int side, int k, int j, double& c, double& s) {
+    if (CART) {
+        c = B.wall[2 * side];
+        s = B.wall[2 * side + 1];
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        switch (side) {
+            case PHX_E: c = G.V(F_c, k, B.nx); s = G.V(F_s, k, B.nx); break;
+            case PHX_W: c = G.V(F_c, k, 0); s = G.V(F_s, k, 0); break;
+            case PHX_N: c = G.H(F_c, B.ny, j); s = G.H(F_s, B.ny, j); break;
+            default: c = G.H(F_c, 0, j); s = G.H(F_s, 0, j); break;
+        }
+    }
+}
+
+// Value of cell (k, j) of block B, where (k, j) may lie one cell outside the block on one side
+// (never a corner): Blocks.apply_boundary_condition / GhostBlock.send_boundary_data
+// (blocks/base.py:448-471, blocks/ghost.py:187-216) followed by the ghost's BC (ghost.py:243-278).
+template <bool CART>
+__device__ Q4 fetch(const DevBlock* __restrict__ blocks, const DevBlock& B, const double* __restrict__ U,
+                    int k, int j) {
+    const int nx = B.nx, ny = B.ny;
+    if (k >= 0 && k < ny && j >= 0 && j < nx) return ld4(U + B.off + 4 * ((long)k * nx + j));
+    int side, kk = k, jj = j;
+    if (j < 0) { side = PHX_W; jj = 0; }
+    else if (j >= nx) { side = PHX_E; jj = nx - 1; }
+    else if (k < 0) { side = PHX_S; kk = 0; }
+    else { side = PHX_N; kk = ny - 1; }
+    const int st = B.side[side];
+    if (st == PHX_SIDE_NEIGHBOR) {
+        const DevBlock& N = blocks[B.nbr[side]];
+        int nk = k, nj = j;
+        if (side == PHX_E) nj = 0;
+        else if (side == PHX_W) nj = N.nx - 1;
+        else if (side == PHX_N) nk = 0;
+        else nk = N.ny - 1;
+        return ld4(U + N.off + 4 * ((long)nk * N.nx + nj));
+    }
+    Q4 q = ld4(U + B.off + 4 * ((long)kk * nx + jj));
+    if (st == PHX_SIDE_WALL) {
+        double c, s;
+        wall_cs<CART>(B, side, k, j, c, s);
+        q = phx::reflect(q, c, s);
+    }
+    return q;
+}
+
+// Outer state of a boundary face: MUSCL._get_boundary_flux_states (fvm/base.py:305-325) and the
+// BC applied to it (fvm/base.py:352-362,421-431).  `own` is the block's limited face value,
+// `ghost` the ghost cell value (the ghost block's limited value with phi = 0, grad = 0).
+template <bool CART>
+__device__ __forceinline__ Q4 outer_state(const DevBlock& B, int side, const Q4& own, const Q4& ghost,
+                                          int k, int j) {
+    const int st = B.side[side];
+    if (st == PHX_SIDE_NEIGHBOR || st == PHX_SIDE_OPEN) return ghost;
+    if (st == PHX_SIDE_OUTLET) return own;
+    double c, s;
+    wall_cs<CART>(B, side, k, j, c, s);
+    return phx::reflect(own, c, s);
+}
+
+template <bool CART, int FLUX>
+__device__ __forceinline__ Q4 xflux(const DevBlock& B, const Q4& L, const Q4& R, int k, int jf, const Phys& ph) {
+    if (CART) return phx::face_flux<FLUX>(L, R, ph);
+    const GenG G{B.geom, B.nx, B.ny};
+    const double c = G.V(F_c, k, jf), s = G.V(F_s, k, jf);
+    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    return phx::unrotate(F, c, s);
+}
+
+template <bool CART, int FLUX>
+__device__ __forceinline__ Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R, int kf, int j, const Phys& ph) {
+    if (CART) {
+        const Q4 F = phx::face_flux<FLUX>(phx::rotate90(L), phx::rotate90(R), ph);
+        return phx::unrotate90(F);
+    }
+    const GenG G{B.geom, B.nx, B.ny};
+    const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
+    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    return phx::unrotate(F, c, s);
+}
+
+template <bool CART>
+__device__ __forceinline__ phx::CellGeo cell_geo(const DevBlock& B, int k, int j) {
+    phx::CellGeo g;
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        g.xlE = G.row(R_xlE, k); g.xlW = G.row(R_xlW, k);
+        g.ylE = G.row(R_ylE, k); g.ylW = G.row(R_ylW, k);
+        g.xlN = G.col(C_xlN, j); g.xlS = G.col(C_xlS, j);
+        g.ylN = G.col(C_ylN, j); g.ylS = G.col(C_ylS, j);
+        g.ainv = 1.0 / G.A(k, j);
+        g.dxE = G.col(C_dxE, j); g.dxW = G.col(C_dxW, j);
+        g.dxN = G.col(C_dxN, j); g.dxS = G.col(C_dxS, j);
+        g.dyE = G.row(R_dyE, k); g.dyW = G.row(R_dyW, k);
+        g.dyN = G.row(R_dyN, k); g.dyS = G.row(R_dyS, k);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        g.xlE = G.V(F_xl, k, j + 1); g.xlW = -G.V(F_xl, k, j);
+        g.ylE = G.V(F_yl, k, j + 1); g.ylW = -G.V(F_yl, k, j);
+        g.xlN = G.H(F_xl, k + 1, j); g.xlS = -G.H(F_xl, k, j);
+        g.ylN = G.H(F_yl, k + 1, j); g.ylS = -G.H(F_yl, k, j);
+        g.ainv = 1.0 / G.C(CL_A, k, j);
+        const double xc = G.C(CL_xc, k, j), yc = G.C(CL_yc, k, j);
+        g.dxE = G.V(F_xq, k, j + 1) - xc; g.dyE = G.V(F_yq, k, j + 1) - yc;
+        g.dxW = G.V(F_xq, k, j) - xc; g.dyW = G.V(F_yq, k, j) - yc;
+        g.dxN = G.H(F_xq, k + 1, j) - xc; g.dyN = G.H(F_yq, k + 1, j) - yc;
+        g.dxS = G.H(F_xq, k, j) - xc; g.dyS = G.H(F_yq, k, j) - yc;
+    }
+    return g;
+}
+
+// face lengths of cell (k, j): west, east, south, north, and its area
+template <bool CART>
+__device__ __forceinline__ void cell_L(const DevBlock& B, int k, int j, double& LW, double& LE, double& LS,
+                                       double& LN, double& A) {
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        LW = G.row(R_LW, k); LE = G.row(R_LE, k);
+        LS = G.col(C_LS, j); LN = G.col(C_LN, j);
+        A = G.A(k, j);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        LW = G.V(F_L, k, j); LE = G.V(F_L, k, j + 1);
+        LS = G.H(F_L, k, j); LN = G.H(F_L, k + 1, j);
+        A = G.C(CL_A, k, j);
+    }
+}
+
+template <bool CART>
+__device__ __forceinline__ void cell_dxdy(const DevBlock& B, int k, int j, double& dx, double& dy) {
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        dx = G.col(C_dx, j);
+        dy = G.row(R_dy, k);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        dx = G.C(CL_dx, k, j);
+        dy = G.C(CL_dy, k, j);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// one work item of one RK stage
+// ---------------------------------------------------------------------------------------------
+template <bool CART, int FLUX>
+__device__ void stage_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock B = A.blocks[it.block];
+    // a case that became unrealizable is frozen: the reference aborts that run (Euler2D.py:144-152)
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = B.nx, ny = B.ny;
+    const int j = it.j0 - HALO + lane;
+    const bool valid = (j >= -1) && (j <= nx);
+    const bool inner = (j >= 0) && (j < nx);
+    const bool gcol = (j == -1) || (j == nx);
+    const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP);
+    const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP, nx));
+    const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP);
+    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const Phys ph = A.ph;
+
+    const int kr0 = max(it.r0 - 1, 0);
+    const int kr1 = min(it.r1, ny);  // row ny is a virtual row closing the north boundary face
+
+    Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
+    if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, kr0 - 1, j);
+    if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, kr0, j);
+
+    Q4 QNprev = phx::q4zero(), part = phx::q4zero();
+    double prevLN = 0.0, prevA = 1.0;
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+
+    for (int k = kr0; k <= kr1; ++k) {
+        const bool real = (k < ny);
+        if (real) {
+            if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+            else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+        }
+        const Q4 Uw = shfl_up1(Uc);
+        const Q4 Ue = shfl_down1(Uc);
+        phx::Recon r;
+        r.E = r.W = r.N = r.S = phx::q4zero();
+        if (real && recon) {
+            const phx::CellGeo g = cell_geo<CART>(B, k, j);
+            r = phx::reconstruct(Uc, Uw, Ue, Um, Up, g);
+        }
+        const Q4 qsend = (j == -1) ? Uc : r.E;
+        const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 (ghost value for j == 0)
+        const bool xrow = (k >= it.r0) && (k < it.r1);
+        Q4 FW = phx::q4zero();
+        if (xrow && xface) {
+            const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
+            const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
+            FW = xflux<CART, FLUX>(B, L, R, k, j, ph);
+        }
+        const Q4 FE = shfl_down1(FW);  // flux of face j+1 (east face of cell j)
+
+        if (k >= it.r0 && owner) {
+            // face between rows k-1 and k (south face of row k, north face of row k-1)
+            const Q4 L = (k == 0) ? outer_state<CART>(B, PHX_S, r.S, Um, k, j) : QNprev;
+            const Q4 R = (k == ny) ? outer_state<CART>(B, PHX_N, QNprev, Uc, k - 1, j) : r.S;
+            const Q4 F = yflux<CART, FLUX>(B, L, R, k, j, ph);
+            if (k - 1 >= it.r0) {
+                // residual of row k-1: 0.5*(W - E + S - N)/A with I = L*(2*F) (fvm/base.py:141-190)
+                const int kk = k - 1;
+                Q4 Res;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * F.v[c]))) / prevA;
+                const long idx = B.off + 4 * ((long)kk * nx + j);
+                if (A.Rout) st4(A.Rout + idx, Res);
+                for (int m = 0; m < A.nout; ++m) {
+                    const Q4 s = ld4(A.src[m] + idx);
+                    Q4 o;
+                    const double am = A.a[m];
+                    if (am == 0.0) {
+                        o = s;
+                    } else {
+                        const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) o.v[c] = s.v[c] + cdt * Res.v[c];
+                    }
+                    st4(A.dst[m] + idx, o);
+                    if (m == A.state_out && A.dtmin != nullptr) {
+                        double dx, dy;
+                        cell_dxdy<CART>(B, kk, j, dx, dy);
+                        const double t = phx::cell_dt(o, dx, dy, ph);
+                        if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
+                        else lane_dt = fmin(lane_dt, t);
+                    }
+                }
+            }
+            if (xrow) {
+                double LW, LE, LS, LN, Ac;
+                cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    part.v[c] = (LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c])) + LS * (2.0 * F.v[c]);
+                prevLN = LN;
+                prevA = Ac;
+            }
+        }
+        QNprev = r.N;
+        Um = Uc;
+        Uc = Up;
+    }
+
+    if (A.dtmin != nullptr) {
+        double m = lane_dt;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, WAVE));
+        const bool anybad = __any(lane_bad);
+        if (lane == 0) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
+            if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
+        }
+    }
+}
+
+// Work distribution: 8 XCDs receive workgroups round-robin; each XCD gets a contiguous range of the
+// (spatially ordered) work list so that halo rows/columns shared by neighbouring items are reused
+// from the same L2.
+__device__ __forceinline__ void item_range(int n, int& lo, int& hi, int& first, int& stride) {
+    const int G = gridDim.x, w = blockIdx.x;
+    if (G >= 8) {
+        const int x = w & 7;
+        lo = (int)(((long)n * x) >> 3);
+        hi = (int)(((long)n * (x + 1)) >> 3);
+        first = lo + (w >> 3);
+        stride = (G - x + 7) >> 3;
+    } else {
+        lo = 0; hi = n; first = w; stride = G;
+    }
+}
+
+template <bool CART, int FLUX>
+__global__ void __launch_bounds__(WAVE) k_stage(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX>(A, A.items[i]);
+}
+
+// standalone CFL reduction over the owned cells of every work item
+struct DtArgs {
+    const DevBlock* blocks;
+    const WorkItem* items;
+    int nitems;
+    const double* U;
+    unsigned long long* dtmin;
+    int* bad;
+    const int* step;
+    Phys ph;
+};
+
+template <bool CART>
+__global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = A.items[i];
+        const DevBlock B = A.blocks[it.block];
+        const int j = it.j0 + lane;
+        const bool own = (lane < STRIP) && (j < B.nx);
+        double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+        bool lane_bad = false;
+        if (own) {
+            for (int k = it.r0; k < it.r1; ++k) {
+                const Q4 q = ld4(A.U + B.off + 4 * ((long)k * B.nx + j));
+                double dx, dy;
+                cell_dxdy<CART>(B, k, j, dx, dy);
+                const double t = phx::cell_dt(q, dx, dy, A.ph);
+                if (!(q.v[0] > 0.0) || !(q.v[3] > 0.0) || !(t == t)) lane_bad = true;
+                else lane_dt = fmin(lane_dt, t);
+            }
+        }
+        double m = lane_dt;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, WAVE));
+        const bool anybad = __any(lane_bad);
+        if (lane == 0) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
+            if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
+        }
+    }
+}
+
+// dt = CFL*min, clipped to t_final - t (solvers/base.py:126-136); advance t (Euler2D.py:209).
+// One workgroup; a case that failed at an earlier step gets dt = 0 and keeps its time.
+__global__ void k_finalize(int ncases, double cfl, unsigned long long* dtmin, const double* tfinal, double* t,
+                           double* dt, int* step, const int* bad, const double* frozen, int advance) {
+    const int s_next = step[0] + 1;
+    for (int c = threadIdx.x; c < ncases; c += blockDim.x) {
+        double d;
+        if (frozen != nullptr) {
+            d = frozen[c];
+        } else {
+            d = cfl * __longlong_as_double((long long)dtmin[c]);
+            const double rem = tfinal[c] - t[c];
+            if (rem < d) d = rem;
+        }
+        if (advance && bad[c] < s_next) d = 0.0;
+        dt[c] = d;
+        if (advance) {
+            t[c] = t[c] + d;
+            dtmin[c] = DT_INF_BITS;
+        }
+    }
+    __syncthreads();
+    if (advance && threadIdx.x == 0) step[0] = s_next;
+}
+
+__global__ void k_reset(int ncases, const double* t0, double* t, unsigned long long* dtmin, int* bad, int* step,
+                        int step0) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncases) {
+        t[c] = t0[c];
+        dtmin[c] = DT_INF_BITS;
+        bad[c] = 0x7fffffff;
+    }
+    if (c == 0) step[0] = step0;
+}
+
+__global__ void k_fill_inf(int ncases, unsigned long long* dtmin) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncases) dtmin[c] = DT_INF_BITS;
+}
+
+// ghost strips for tests: block b, cell index i over E(ny) W(ny) N(nx) S(nx)
+__global__ void k_ghosts(const DevBlock* blocks, int nblocks, const long long* goff, const double* U, double* G) {
+    const int b = blockIdx.x;
+    if (b >= nblocks) return;
+    const DevBlock B = blocks[b];
+    const int n = 2 * B.ny + 2 * B.nx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int k, j;
+        if (i < B.ny) { k = i; j = B.nx; }
+        else if (i < 2 * B.ny) { k = i - B.ny; j = -1; }
+        else if (i < 2 * B.ny + B.nx) { k = B.ny; j = i - 2 * B.ny; }
+        else { k = -1; j = i - 2 * B.ny - B.nx; }
+        const Q4 q = (B.kind == PHX_GEOM_CARTESIAN) ? fetch<true>(blocks, B, U, k, j) : fetch<false>(blocks, B, U, k, j);
+        st4(G + goff[b] + 4L * i, q);
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+// host side
+// =============================================================================================
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PHX_CK(call)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(PHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+enum BufId { BUF_USER = 0, BUF_S0 = 1, BUF_S1 = 2, BUF_ACC0 = 3 };
+
+struct StageOut {
+    int dst, src;
+    double a;
+};
+struct StageProg {
+    int in;
+    std::vector<StageOut> outs;
+    int state_out;
+};
+
+}  // namespace
+
+struct phx_plan {
+    int device = 0;
+    int ncu = 0;
+    int nblocks = 0, ncases = 0, flux = 0;
+    Phys ph{};
+    std::vector<phx_block_desc> desc;
+    std::vector<DevBlock> hblocks;
+    DevBlock* dblocks = nullptr;
+    double* dgeom = nullptr;
+    int64_t state_len = 0;
+    int rows_per_item = 32, grid_waves = 0;
+    std::vector<WorkItem> items[2];  // [0] general, [1] cartesian
+    WorkItem* ditems[2] = {nullptr, nullptr};
+    int grid[2] = {0, 0};
+    std::vector<StageProg> progs;
+    int nacc = 0;
+    double* sbuf[2] = {nullptr, nullptr};
+    double* acc[MAX_ACC] = {};
+    double *dt = nullptr, *t = nullptr, *tfinal = nullptr;
+    unsigned long long* dtmin = nullptr;
+    int *step = nullptr, *bad = nullptr;
+    long long* dghost_off = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* graph_U = nullptr;
+    bool copy_back = false;  // one-stage tables: result lands in sbuf[0] and is copied to U
+};
+
+namespace {
+
+double* buf_ptr(phx_plan* P, double* U, int id) {
+    if (id == BUF_USER) return U;
+    if (id == BUF_S0) return P->sbuf[0];
+    if (id == BUF_S1) return P->sbuf[1];
+    return P->acc[id - BUF_ACC0];
+}
+
+// Butcher table -> per-stage program in accumulator form: U^(s+1) = P_s + (dt a_ss) k_s and
+// P_t += (dt a_ts) k_s for later rows t, each P_t starting from U0; zero coefficients skipped,
+// which reproduces the summation order of ExplicitRungeKutta.integrate (explicit_runge_kutta.py:63-75).
+int build_programs(phx_plan* P, int S, const double* a) {
+    std::vector<int> Pt(S, BUF_USER);
+    std::vector<int> free_acc;
+    int nacc = 0;
+    P->progs.clear();
+    for (int s = 0; s < S; ++s) {
+        StageProg g;
+        g.in = (s == 0) ? BUF_USER : ((s - 1) % 2 == 0 ? BUF_S0 : BUF_S1);
+        // the last stage writes the caller's state in place unless it also reads it as stencil input
+        const int out = (s == S - 1 && S > 1) ? BUF_USER : (s % 2 == 0 ? BUF_S0 : BUF_S1);
+        g.outs.push_back({out, Pt[s], a[s * S + s]});
+        g.state_out = (s == S - 1) ? 0 : -1;
+        if (Pt[s] >= BUF_ACC0) free_acc.push_back(Pt[s]);
+        for (int t = s + 1; t < S; ++t) {
+            const double c = a[t * S + s];
+            if (c == 0.0) continue;
+            if (Pt[t] == BUF_USER) {
+                int id;
+                if (!free_acc.empty()) { id = free_acc.back(); free_acc.pop_back(); }
+                else { if (nacc >= MAX_ACC) return fail(PHX_E_INVALID, "Butcher table needs too many accumulators"); id = BUF_ACC0 + nacc++; }
+                g.outs.push_back({id, BUF_USER, c});
+                Pt[t] = id;
+            } else {
+                g.outs.push_back({Pt[t], Pt[t], c});
+            }
+        }
+        if ((int)g.outs.size() > MAX_OUT) return fail(PHX_E_INVALID, "Butcher stage has too many outputs");
+        P->progs.push_back(g);
+    }
+    P->nacc = nacc;
+    P->copy_back = (S == 1);
+    return PHX_OK;
+}
+
+int build_items(phx_plan* P) {
+    for (int kind = 0; kind < 2; ++kind) {
+        P->items[kind].clear();
+        if (P->ditems[kind]) { (void)hipFree(P->ditems[kind]); P->ditems[kind] = nullptr; }
+    }
+    const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
+    for (int b = 0; b < P->nblocks; ++b) {
+        const phx_block_desc& d = P->desc[b];
+        const int kind = d.geom_kind == PHX_GEOM_CARTESIAN ? 1 : 0;
+        for (int r0 = 0; r0 < d.ny; r0 += rows)
+            for (int j0 = 0; j0 < d.nx; j0 += STRIP)
+                P->items[kind].push_back({b, j0, r0, std::min(r0 + rows, d.ny)});
+    }
+    for (int kind = 0; kind < 2; ++kind) {
+        const size_t n = P->items[kind].size();
+        if (n) {
+            PHX_CK(hipMalloc(&P->ditems[kind], n * sizeof(WorkItem)));
+            PHX_CK(hipMemcpy(P->ditems[kind], P->items[kind].data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
+        }
+        const int gw = P->grid_waves;
+        P->grid[kind] = (int)((gw > 0 && (size_t)gw < n) ? gw : n);
+    }
+    if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    P->graph_U = nullptr;
+    return PHX_OK;
+}
+
+StageArgs base_args(phx_plan* P) {
+    StageArgs A;
+    memset(&A, 0, sizeof(A));
+    A.blocks = P->dblocks;
+    A.state_out = -1;
+    A.dt = P->dt;
+    A.bad = P->bad;
+    A.step = P->step;
+    A.ph = P->ph;
+    return A;
+}
+
+void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
+    for (int kind = 0; kind < 2; ++kind) {
+        const int n = (int)P->items[kind].size();
+        if (!n) continue;
+        A.items = P->ditems[kind];
+        A.nitems = n;
+        dim3 grid(P->grid[kind]), block(WAVE);
+        if (kind == 1) {
+            if (P->flux == PHX_FLUX_ROE) hipLaunchKernelGGL((k_stage<true, 0>), grid, block, 0, st, A);
+            else hipLaunchKernelGGL((k_stage<true, 1>), grid, block, 0, st, A);
+        } else {
+            if (P->flux == PHX_FLUX_ROE) hipLaunchKernelGGL((k_stage<false, 0>), grid, block, 0, st, A);
+            else hipLaunchKernelGGL((k_stage<false, 1>), grid, block, 0, st, A);
+        }
+    }
+}
+
+void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill_inf, dim3((P->ncases + 255) / 256), dim3(256), 0, st, P->ncases, P->dtmin);
+    for (int kind = 0; kind < 2; ++kind) {
+        const int n = (int)P->items[kind].size();
+        if (!n) continue;
+        DtArgs D{P->dblocks, P->ditems[kind], n, U, P->dtmin, P->bad, P->step, P->ph};
+        if (kind == 1) hipLaunchKernelGGL(k_dt<true>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
+        else hipLaunchKernelGGL(k_dt<false>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
+    }
+}
+
+void launch_prog_stage(phx_plan* P, double* U, size_t s, hipStream_t st) {
+        const StageProg& g = P->progs[s];
+        StageArgs A = base_args(P);
+        A.Uin = buf_ptr(P, U, g.in);
+        A.nout = (int)g.outs.size();
+        for (int m = 0; m < A.nout; ++m) {
+            A.dst[m] = buf_ptr(P, U, g.outs[m].dst);
+            A.src[m] = buf_ptr(P, U, g.outs[m].src);
+            A.a[m] = g.outs[m].a;
+        }
+        A.state_out = g.state_out;
+        A.skip_dead = 1;
+        if (g.state_out >= 0) A.dtmin = P->dtmin;
+        launch_stage(P, A, st);
+}
+
+void launch_step(phx_plan* P, double* U, hipStream_t st, const double* frozen) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+                       P->dt, P->step, P->bad, frozen, 1);
+    for (size_t s = 0; s < P->progs.size(); ++s) launch_prog_stage(P, U, s, st);
+    if (P->copy_back)
+        (void)hipMemcpyAsync(U, P->sbuf[0], (size_t)P->state_len * sizeof(double), hipMemcpyDeviceToDevice, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* phx_last_error(void) { return g_err.c_str(); }
+
+int64_t phx_state_len(const phx_plan* P) { return P ? P->state_len : -1; }
+
+int phx_plan_destroy(phx_plan* P) {
+    if (!P) return PHX_OK;
+    (void)hipDeviceSynchronize();
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
+    if (P->ev_a) (void)hipEventDestroy(P->ev_a);
+    if (P->ev_b) (void)hipEventDestroy(P->ev_b);
+    void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->sbuf[0], P->sbuf[1], P->dt, P->t,
+                    P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int m = 0; m < MAX_ACC; ++m)
+        if (P->acc[m]) (void)hipFree(P->acc[m]);
+    delete P;
+    return PHX_OK;
+}
+
+int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
+    if (!D || !out) return fail(PHX_E_INVALID, "null argument");
+    *out = nullptr;
+    if (D->abi_version != PHX_ABI_VERSION) return fail(PHX_E_INVALID, "ABI version mismatch");
+    if (D->nblocks < 1 || !D->blocks) return fail(PHX_E_INVALID, "no blocks");
+    if (D->ncases < 1) return fail(PHX_E_INVALID, "ncases must be >= 1");
+    if (D->flux != PHX_FLUX_ROE && D->flux != PHX_FLUX_HLLL)
+        return fail(PHX_E_INVALID, "Flux function type is not available.");
+    if (D->limiter != PHX_LIMITER_VENKATAKRISHNAN) return fail(PHX_E_INVALID, "MUSCL: Slope limiter type not specified.");
+    if (D->nstages < 1 || D->nstages > PHX_MAX_STAGES || !D->butcher_a)
+        return fail(PHX_E_INVALID, "invalid Butcher table");
+    if (!D->t_final) return fail(PHX_E_INVALID, "t_final is null");
+    int64_t total = 0;
+    for (int b = 0; b < D->nblocks; ++b) {
+        const phx_block_desc& d = D->blocks[b];
+        const std::string where = "block " + std::to_string(b) + ": ";
+        if (d.nx < 1 || d.ny < 1) return fail(PHX_E_INVALID, where + "nx, ny must be >= 1");
+        if (d.case_id < 0 || d.case_id >= D->ncases) return fail(PHX_E_INVALID, where + "case_id out of range");
+        if (d.state_offset < 0 || d.state_offset % 4) return fail(PHX_E_INVALID, where + "state_offset must be a non-negative multiple of 4");
+        if (!d.geom) return fail(PHX_E_INVALID, where + "geometry is null");
+        int64_t glen;
+        if (d.geom_kind == PHX_GEOM_CARTESIAN) glen = 8 + 11LL * d.nx + 11LL * d.ny + (int64_t)d.nx * d.ny;
+        else if (d.geom_kind == PHX_GEOM_GENERAL) glen = 7LL * d.ny * (d.nx + 1) + 7LL * (d.ny + 1) * d.nx + 5LL * d.ny * d.nx;
+        else return fail(PHX_E_INVALID, where + "unknown geometry kind");
+        if (d.geom_len != glen) return fail(PHX_E_INVALID, where + "geometry length mismatch");
+        for (int s = 0; s < 4; ++s) {
+            const int st = d.side_type[s];
+            if (st < PHX_SIDE_NEIGHBOR || st > PHX_SIDE_OUTLET) return fail(PHX_E_INVALID, where + "invalid side type");
+            if (st == PHX_SIDE_NEIGHBOR) {
+                const int n = d.neighbor[s];
+                if (n < 0 || n >= D->nblocks) return fail(PHX_E_INVALID, where + "neighbour index out of range");
+                const phx_block_desc& o = D->blocks[n];
+                if ((s == PHX_E || s == PHX_W) && o.ny != d.ny) return fail(PHX_E_INVALID, where + "E/W neighbour ny mismatch");
+                if ((s == PHX_N || s == PHX_S) && o.nx != d.nx) return fail(PHX_E_INVALID, where + "N/S neighbour nx mismatch");
+                if (o.case_id != d.case_id) return fail(PHX_E_INVALID, where + "neighbour belongs to another case");
+            }
+        }
+        total = std::max<int64_t>(total, d.state_offset + 4LL * d.nx * d.ny);
+    }
+    // overlap check
+    {
+        std::vector<std::pair<int64_t, int64_t>> iv;
+        for (int b = 0; b < D->nblocks; ++b)
+            iv.push_back({D->blocks[b].state_offset, D->blocks[b].state_offset + 4LL * D->blocks[b].nx * D->blocks[b].ny});
+        std::sort(iv.begin(), iv.end());
+        for (size_t i = 1; i < iv.size(); ++i)
+            if (iv[i].first < iv[i - 1].second) return fail(PHX_E_INVALID, "block state ranges overlap");
+    }
+
+    phx_plan* P = new phx_plan();
+    auto bail = [&](int rc) { phx_plan_destroy(P); return rc; };
+    int rc;
+#define PHX_CKP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return bail(fail(PHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    PHX_CKP(hipGetDevice(&P->device));
+    PHX_CKP(hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, P->device));
+    P->nblocks = D->nblocks;
+    P->ncases = D->ncases;
+    P->flux = D->flux;
+    P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl};
+    P->desc.assign(D->blocks, D->blocks + D->nblocks);
+    P->state_len = total;
+    P->rows_per_item = D->rows_per_item > 0 ? D->rows_per_item : 32;
+    P->grid_waves = D->grid_waves;
+
+    // geometry upload (each block aligned to 4 doubles)
+    std::vector<int64_t> goff(D->nblocks);
+    int64_t gtot = 0;
+    for (int b = 0; b < D->nblocks; ++b) {
+        goff[b] = gtot;
+        gtot += (D->blocks[b].geom_len + 3) / 4 * 4;
+    }
+    PHX_CKP(hipMalloc(&P->dgeom, std::max<int64_t>(gtot, 4) * sizeof(double)));
+    for (int b = 0; b < D->nblocks; ++b)
+        PHX_CKP(hipMemcpy(P->dgeom + goff[b], D->blocks[b].geom, D->blocks[b].geom_len * sizeof(double), hipMemcpyHostToDevice));
+    P->hblocks.resize(D->nblocks);
+    std::vector<long long> ghost_off(D->nblocks);
+    long long gacc = 0;
+    for (int b = 0; b < D->nblocks; ++b) {
+        const phx_block_desc& d = D->blocks[b];
+        DevBlock& h = P->hblocks[b];
+        memset(&h, 0, sizeof(h));
+        h.nx = d.nx; h.ny = d.ny; h.case_id = d.case_id; h.kind = d.geom_kind;
+        for (int s = 0; s < 4; ++s) { h.side[s] = d.side_type[s]; h.nbr[s] = d.neighbor[s]; }
+        h.off = d.state_offset;
+        h.geom = P->dgeom + goff[b];
+        if (d.geom_kind == PHX_GEOM_CARTESIAN)
+            for (int q = 0; q < 8; ++q) h.wall[q] = d.geom[q];
+        ghost_off[b] = gacc;
+        gacc += 4LL * (2 * d.ny + 2 * d.nx);
+    }
+    PHX_CKP(hipMalloc(&P->dblocks, D->nblocks * sizeof(DevBlock)));
+    PHX_CKP(hipMemcpy(P->dblocks, P->hblocks.data(), D->nblocks * sizeof(DevBlock), hipMemcpyHostToDevice));
+    PHX_CKP(hipMalloc(&P->dghost_off, D->nblocks * sizeof(long long)));
+    PHX_CKP(hipMemcpy(P->dghost_off, ghost_off.data(), D->nblocks * sizeof(long long), hipMemcpyHostToDevice));
+
+    rc = build_programs(P, D->nstages, D->butcher_a);
+    if (rc) return bail(rc);
+    rc = build_items(P);
+    if (rc) return bail(rc);
+
+    const size_t sbytes = (size_t)std::max<int64_t>(total, 4) * sizeof(double);
+    PHX_CKP(hipMalloc(&P->sbuf[0], sbytes));
+    if (D->nstages > 2) PHX_CKP(hipMalloc(&P->sbuf[1], sbytes));
+    for (int m = 0; m < P->nacc; ++m) PHX_CKP(hipMalloc(&P->acc[m], sbytes));
+    const size_t cb = (size_t)D->ncases * sizeof(double);
+    PHX_CKP(hipMalloc(&P->dt, cb));
+    PHX_CKP(hipMalloc(&P->t, cb));
+    PHX_CKP(hipMalloc(&P->tfinal, cb));
+    PHX_CKP(hipMalloc(&P->dtmin, D->ncases * sizeof(unsigned long long)));
+    PHX_CKP(hipMalloc(&P->step, sizeof(int)));
+    PHX_CKP(hipMalloc(&P->bad, D->ncases * sizeof(int)));
+    PHX_CKP(hipMemcpy(P->tfinal, D->t_final, cb, hipMemcpyHostToDevice));
+    PHX_CKP(hipMemset(P->dt, 0, cb));
+    std::vector<double> zeros(D->ncases, 0.0);
+    double* t0 = nullptr;
+    PHX_CKP(hipMalloc(&t0, cb));
+    PHX_CKP(hipMemcpy(t0, zeros.data(), cb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_reset, dim3((D->ncases + 255) / 256), dim3(256), 0, 0, D->ncases, t0, P->t, P->dtmin,
+                       P->bad, P->step, 0);
+    PHX_CKP(hipGetLastError());
+    PHX_CKP(hipDeviceSynchronize());
+    PHX_CKP(hipFree(t0));
+    PHX_CKP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
+    PHX_CKP(hipEventCreateWithFlags(&P->ev_a, hipEventDisableTiming));
+    PHX_CKP(hipEventCreateWithFlags(&P->ev_b, hipEventDisableTiming));
+#undef PHX_CKP
+    *out = P;
+    return PHX_OK;
+}
+
+int phx_reset(phx_plan* P, const double* t0_host, int32_t step0, void* stream) {
+    if (!P || !t0_host) return fail(PHX_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t cb = (size_t)P->ncases * sizeof(double);
+    double* t0 = nullptr;
+    PHX_CK(hipMalloc(&t0, cb));
+    PHX_CK(hipMemcpy(t0, t0_host, cb, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_reset, dim3((P->ncases + 255) / 256), dim3(256), 0, st, P->ncases, t0, P->t, P->dtmin,
+                       P->bad, P->step, step0);
+    PHX_CK(hipGetLastError());
+    PHX_CK(hipStreamSynchronize(st));
+    PHX_CK(hipFree(t0));
+    return PHX_OK;
+}
+
+int phx_run(phx_plan* P, double* U, int32_t nsteps, int32_t use_graph, void* stream) {
+    if (!P || !U) return fail(PHX_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(PHX_E_INVALID, "nsteps must be >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    launch_dt_reduce(P, U, st);
+    PHX_CK(hipGetLastError());
+    if (!use_graph) {
+        for (int s = 0; s < nsteps; ++s) launch_step(P, U, st, nullptr);
+        PHX_CK(hipGetLastError());
+        return PHX_OK;
+    }
+    if (!P->gexec || P->graph_U != U) {
+        if (P->gexec) { PHX_CK(hipGraphExecDestroy(P->gexec)); P->gexec = nullptr; }
+        if (P->graph) { PHX_CK(hipGraphDestroy(P->graph)); P->graph = nullptr; }
+        PHX_CK(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
+        launch_step(P, U, P->cap_stream, nullptr);
+        PHX_CK(hipStreamEndCapture(P->cap_stream, &P->graph));
+        PHX_CK(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
+        P->graph_U = U;
+    }
+    PHX_CK(hipEventRecord(P->ev_a, st));
+    PHX_CK(hipStreamWaitEvent(P->cap_stream, P->ev_a, 0));
+    for (int s = 0; s < nsteps; ++s) PHX_CK(hipGraphLaunch(P->gexec, P->cap_stream));
+    PHX_CK(hipEventRecord(P->ev_b, P->cap_stream));
+    PHX_CK(hipStreamWaitEvent(st, P->ev_b, 0));
+    return PHX_OK;
+}
+
+int phx_run_frozen(phx_plan* P, double* U, int32_t nsteps, const double* dt_sched, void* stream) {
+    if (!P || !U || (!dt_sched && nsteps > 0)) return fail(PHX_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    for (int s = 0; s < nsteps; ++s) launch_step(P, U, st, dt_sched + (size_t)s * P->ncases);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+int phx_stage(phx_plan* P, double* U, int32_t stage, void* stream) {
+    if (!P || !U) return fail(PHX_E_INVALID, "null argument");
+    if (stage < 0 || stage >= (int)P->progs.size()) return fail(PHX_E_INVALID, "stage index out of range");
+    launch_prog_stage(P, U, (size_t)stage, (hipStream_t)stream);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+int phx_residual(phx_plan* P, const double* U, double* R, void* stream) {
+    if (!P || !U || !R) return fail(PHX_E_INVALID, "null argument");
+    StageArgs A = base_args(P);
+    A.Uin = U;
+    A.Rout = R;
+    A.nout = 0;
+    launch_stage(P, A, (hipStream_t)stream);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+int phx_dt(phx_plan* P, const double* U, double* dt_dev, void* stream) {
+    if (!P || !U || !dt_dev) return fail(PHX_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    launch_dt_reduce(P, U, st);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+                       dt_dev, P->step, P->bad, (const double*)nullptr, 0);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+int phx_ghosts(phx_plan* P, const double* U, double* G, void* stream) {
+    if (!P || !U || !G) return fail(PHX_E_INVALID, "null argument");
+    hipLaunchKernelGGL(k_ghosts, dim3(P->nblocks), dim3(256), 0, (hipStream_t)stream, P->dblocks, P->nblocks,
+                       P->dghost_off, U, G);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+int phx_status(phx_plan* P, int32_t* bad_step, int32_t* step, double* t_host, double* dt_host) {
+    if (!P) return fail(PHX_E_INVALID, "null argument");
+    PHX_CK(hipDeviceSynchronize());
+    std::vector<int> b(P->ncases);
+    int s = 0;
+    PHX_CK(hipMemcpy(b.data(), P->bad, P->ncases * sizeof(int), hipMemcpyDeviceToHost));
+    PHX_CK(hipMemcpy(&s, P->step, sizeof(int), hipMemcpyDeviceToHost));
+    if (bad_step)
+        for (int c = 0; c < P->ncases; ++c) bad_step[c] = (b[c] == 0x7fffffff) ? -1 : b[c];
+    if (step) *step = s;
+    if (t_host) PHX_CK(hipMemcpy(t_host, P->t, P->ncases * sizeof(double), hipMemcpyDeviceToHost));
+    if (dt_host) PHX_CK(hipMemcpy(dt_host, P->dt, P->ncases * sizeof(double), hipMemcpyDeviceToHost));
+    return PHX_OK;
+}
+
+int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* lamU, void* stream) {
+    (void)U; (void)lamR; (void)lamU; (void)stream;
+    if (!P) return fail(PHX_E_INVALID, "null argument");
+    return fail(PHX_E_STATE, "phx_residual_vjp is not available in this build");
+}
+
+int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves) {
+    if (!P || rows_per_item < 1 || grid_waves < 0) return fail(PHX_E_INVALID, "invalid launch parameters");
+    PHX_CK(hipDeviceSynchronize());
+    P->rows_per_item = rows_per_item;
+    P->grid_waves = grid_waves;
+    return build_items(P);
+}
+
+int phx_launch_info(const phx_plan* P, int32_t* items_cart, int32_t* items_gen, int32_t* grid_cart, int32_t* grid_gen) {
+    if (!P) return fail(PHX_E_INVALID, "null argument");
+    if (items_cart) *items_cart = (int32_t)P->items[1].size();
+    if (items_gen) *items_gen = (int32_t)P->items[0].size();
+    if (grid_cart) *grid_cart = P->grid[1];
+    if (grid_gen) *grid_gen = P->grid[0];
+    return PHX_OK;
+}
+
+}  // extern "C"

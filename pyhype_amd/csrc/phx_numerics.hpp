@@ -1,0 +1,297 @@
+// phx_numerics.hpp -- per-cell and per-face arithmetic of pyHype's second-order MUSCL scheme.
+//
+// Every function reproduces the reference's floating-point evaluation order exactly (left-to-right
+// Python/Numba association, the COO storage order of the Roe eigen-system mat-vecs, no fused
+// multiply-add).  The translation unit must be compiled with -ffp-contract=off; division and
+// square root are the IEEE correctly-rounded operations.  The functions are __host__ __device__ so
+// that the same source is checked against the CPU oracle on the host.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace phx {
+
+struct Q4 {
+    double v[4];
+};
+
+// Fluid constants, evaluated on the host exactly as the reference evaluates them:
+// gm1 = gamma - 1 (states/conservative.py:110), one_over_gm1 = 1.0/(gamma - 1.0) and
+// g_over_gm1 = gamma/(gamma - 1.0) (fluids/base.py:58-64).
+struct Phys {
+    double gamma, gm1, one_over_gm1, g_over_gm1, cfl;
+};
+
+struct Prim {
+    double r, u, v, p;
+};
+
+__host__ __device__ inline Q4 q4zero() {
+    Q4 q;
+    q.v[0] = q.v[1] = q.v[2] = q.v[3] = 0.0;
+    return q;
+}
+
+// ConservativeConverter.to_primitive (states/converter/concrete_defs.py:85-100), with
+// u = rhou/rho, v = rhov/rho (conservative.py:85-99), ek = rho*Ek, Ek = 0.5*(u*u+v*v)
+// (conservative.py:118-124), p = (gamma-1)*(e - ek) (conservative.py:107-110).
+__host__ __device__ inline Prim cons_to_prim(const Q4& q, const Phys& ph) {
+    Prim w;
+    w.r = q.v[0];
+    w.u = q.v[1] / w.r;
+    w.v = q.v[2] / w.r;
+    const double ek = w.r * (0.5 * (w.u * w.u + w.v * w.v));
+    w.p = ph.gm1 * (q.v[3] - ek);
+    return w;
+}
+
+// rotate_JIT / unrotate_JIT (utils/utils.py:75-85,148-158) with c = cos(theta), s = sin(theta).
+__host__ __device__ inline Q4 rotate(Q4 q, double c, double s) {
+    const double u = q.v[1], v = q.v[2];
+    q.v[1] = u * c + v * s;
+    q.v[2] = v * c - u * s;
+    return q;
+}
+__host__ __device__ inline Q4 unrotate(Q4 q, double c, double s) {
+    const double u = q.v[1], v = q.v[2];
+    q.v[1] = u * c - v * s;
+    q.v[2] = v * c + u * s;
+    return q;
+}
+// rotate90 / unrotate90 (utils/utils.py:88-112,161-185).
+__host__ __device__ inline Q4 rotate90(Q4 q) {
+    const double u = q.v[2], v = q.v[1];
+    q.v[1] = u;
+    q.v[2] = -v;
+    return q;
+}
+__host__ __device__ inline Q4 unrotate90(Q4 q) {
+    const double u = q.v[2], v = q.v[1];
+    q.v[1] = -u;
+    q.v[2] = v;
+    return q;
+}
+// BoundaryConditionFunctions.reflection (boundary_conditions/funcs.py:27-43).
+__host__ __device__ inline Q4 reflect(Q4 q, double c, double s) {
+    q = rotate(q, c, s);
+    q.v[1] = -q.v[1];
+    return unrotate(q, c, s);
+}
+
+// Harten entropy fix on the Roe wave speeds (flux/base.py:119-146).  Inputs are the slow (u-a)
+// and fast (u+a) speeds of the right, left and Roe states; l1/l4 are corrected in place.
+__host__ __device__ inline void harten(double Rslow, double Rfast, double Lslow, double Lfast,
+                                       double& l1, double& l4) {
+    double tp = 2.0 * (Rslow - Lslow);
+    double tm = 2.0 * (Rfast - Lfast);
+    tp = (tp <= 0.0) ? 1e-8 : tp;
+    tm = (tm <= 0.0) ? 1e-8 : tm;
+    if (fabs(l1) < tp) l1 = 0.5 * ((l1 * l1) / tp + tp);
+    if (fabs(l4) < tm) l4 = 0.5 * ((l4 * l4) / tm + tm);
+}
+
+// FluxRoe.compute_flux (flux/Roe.py:262-304) for one face, primitive left/right states.
+//  * Roe state: RoePrimitiveState._roe_state_from_prim_JIT (states/primitive.py:301-320) --
+//    note the sqrt(rho)-weighted pressure.
+//  * F from primitives: PrimitiveState._F_from_prim_JIT (states/primitive.py:223-237).
+//  * Dissipation Rc |Lambda| Lp dW with the SciPy COO accumulation order of the reference's
+//    block layout (flux/eigen_system.py:115-158,205-242; values flux/Roe.py:175-260).  The
+//    leading "0 +" of each COO row sum only changes the sign of a zero and is omitted.
+__host__ __device__ inline Q4 roe_flux(const Prim& L, const Prim& R, const Phys& ph) {
+    const double ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
+    const double ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
+    const double ruL = L.r * L.u, ruR = R.r * R.u;
+    const double FL0 = ruL, FL1 = ruL * L.u + L.p, FL2 = ruL * L.v;
+    const double FL3 = L.u * ((ph.one_over_gm1 * L.p + ekL) + L.p);
+    const double FR0 = ruR, FR1 = ruR * R.u + R.p, FR2 = ruR * R.v;
+    const double FR3 = R.u * ((ph.one_over_gm1 * R.p + ekR) + R.p);
+
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double inv = 1.0 / (sL + sR);
+    const double rs = sqrt(L.r * R.r);
+    const double us = (L.u * sL + R.u * sR) * inv;
+    const double vs = (L.v * sL + R.v * sR) * inv;
+    const double ps = (L.p * sL + R.p * sR) * inv;
+
+    const double aL = sqrt(ph.gamma * L.p / L.r);
+    const double aR = sqrt(ph.gamma * R.p / R.r);
+    const double as = sqrt(ph.gamma * ps / rs);
+    double l1 = us - as, l4 = us + as;
+    harten(R.u - aR, R.u + aR, L.u - aL, L.u + aL, l1, l4);
+
+    const double Ek = 0.5 * (us * us + vs * vs);           // Ek_JIT
+    const double H = ph.g_over_gm1 * ps / rs + Ek;         // H_given_Ek_JIT
+    const double ua = us * as;
+    const double ia = 1.0 / as;                            // compute_Lp
+    const double ia2 = ia * ia;
+    const double h = 0.5 * ia2;
+    const double r2a = 0.5 * rs * ia;
+
+    const double d0 = R.r - L.r, d1 = R.u - L.u, d2 = R.v - L.v, d3 = R.p - L.p;
+    // Lp * dW: rows in COO order (3,1),(1,0),(2,2),(3,3),(0,1),(1,3),(0,3)
+    const double y0 = (-r2a) * d1 + h * d3;
+    const double y1 = 1.0 * d0 + (-ia2) * d3;
+    const double y2 = 1.0 * d2;
+    const double y3 = r2a * d1 + h * d3;
+    // |Lambda| (diagonal)
+    const double z0 = fabs(l1) * y0, z1 = fabs(us) * y1, z2 = fabs(us) * y2, z3 = fabs(l4) * y3;
+    // Rc * z: COO order (3,0),(2,0),(3,1),(1,0),(2,1),(3,2),(0,0),(1,1),(2,2),(3,3),(0,1),(2,3),(1,3),(0,3)
+    const double s0 = (1.0 * z0 + 1.0 * z1) + 1.0 * z3;
+    const double s1 = (l1 * z0 + us * z1) + l4 * z3;
+    const double s2 = ((vs * z0 + vs * z1) + 1.0 * z2) + vs * z3;
+    const double s3 = (((H - ua) * z0 + Ek * z1) + vs * z2) + (H + ua) * z3;
+
+    Q4 F;
+    F.v[0] = 0.5 * (FL0 + FR0) - 0.5 * s0;
+    F.v[1] = 0.5 * (FL1 + FR1) - 0.5 * s1;
+    F.v[2] = 0.5 * (FL2 + FR2) - 0.5 * s2;
+    F.v[3] = 0.5 * (FL3 + FR3) - 0.5 * s3;
+    return F;
+}
+
+// Euclidean norm of a 4-vector: sqrt(((x0^2 + x1^2) + x2^2) + x3^2).  Restates the BLAS dnrm2
+// reached through Numba's np.linalg.norm in flux/HLLL.py:86-93 (agrees to an ulp).
+__host__ __device__ inline double nrm4(double a, double b, double c, double d) {
+    return sqrt(((a * a + b * b) + c * c) + d * d);
+}
+
+// FluxHLLL.compute_flux + _HLLL_flux_JIT (flux/HLLL.py:25-49,70-103) for one face.
+__host__ __device__ inline Q4 hlll_flux(const Prim& L, const Prim& R, const Phys& ph) {
+    const double sL = sqrt(L.r), sR = sqrt(R.r);
+    const double inv = 1.0 / (sL + sR);
+    const double rs = sqrt(L.r * R.r);
+    const double us = (L.u * sL + R.u * sR) * inv;
+    const double ps = (L.p * sL + R.p * sR) * inv;
+    const double as = sqrt(ph.gamma * ps / rs);
+    const double aL = sqrt(ph.gamma * L.p / L.r);
+    const double aR = sqrt(ph.gamma * R.p / R.r);
+    const double Lslow = L.u - aL, Lfast = L.u + aL;
+    const double Rslow = R.u - aR, Rfast = R.u + aR;
+    double l1 = us - as, l4 = us + as;
+    harten(Rslow, Rfast, Lslow, Lfast, l1, l4);
+    const double Lplus = (Rfast >= l4) ? Rfast : l4;      // np.maximum(R_m, Lm)
+    const double Lminus = (Lslow <= l1) ? Lslow : l1;     // np.minimum(L_p, Lp)
+
+    // ConservativeState(state=W): PrimitiveConverter.to_conservative (concrete_defs.py:126-141)
+    const double ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
+    const double ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
+    const double UL0 = L.r, UL1 = L.r * L.u, UL2 = L.r * L.v, UL3 = L.p / ph.gm1 + ekL;
+    const double UR0 = R.r, UR1 = R.r * R.u, UR2 = R.r * R.v, UR3 = R.p / ph.gm1 + ekR;
+    // PrimitiveState.F(U=...) (states/primitive.py:203-209)
+    const double FL0 = UL1, FL1 = UL1 * L.u + L.p, FL2 = UL1 * L.v, FL3 = L.u * (UL3 + L.p);
+    const double FR0 = UR1, FR1 = UR1 * R.u + R.p, FR2 = UR1 * R.v, FR3 = R.u * (UR3 + R.p);
+    Q4 F;
+    if (Lminus >= 0.0) {
+        F.v[0] = FL0; F.v[1] = FL1; F.v[2] = FL2; F.v[3] = FL3;
+    } else if (Lplus <= 0.0) {
+        F.v[0] = FR0; F.v[1] = FR1; F.v[2] = FR2; F.v[3] = FR3;
+    } else {
+        const double dU0 = UR0 - UL0, dU1 = UR1 - UL1, dU2 = UR2 - UL2, dU3 = UR3 - UL3;
+        const double dF0 = FR0 - FL0, dF1 = FR1 - FL1, dF2 = FR2 - FL2, dF3 = FR3 - FL3;
+        const double k = as * nrm4(dU0, dU1, dU2, dU3);
+        const double n = nrm4(dF0 - us * dU0, dF1 - us * dU1, dF2 - us * dU2, dF3 - us * dU3);
+        const double den = (k < 1e-16) ? (k + 1e-14) : k;
+        double alpha = 1.0 - n / den;
+        alpha = (0.0 >= alpha) ? 0.0 : alpha;              // np.maximum(0, ...)
+        const double ra = us / Lminus, rb = us / Lplus;
+        const double mx = (ra >= rb) ? ra : rb;            // np.maximum(u/Lm, u/Lp)
+        const double coef = Lminus * Lplus * (1.0 - alpha * (1.0 - mx));
+        const double w = Lplus - Lminus;
+        F.v[0] = (Lplus * FL0 - Lminus * FR0 + coef * dU0) / w;
+        F.v[1] = (Lplus * FL1 - Lminus * FR1 + coef * dU1) / w;
+        F.v[2] = (Lplus * FL2 - Lminus * FR2 + coef * dU2) / w;
+        F.v[3] = (Lplus * FL3 - Lminus * FR3 + coef * dU3) / w;
+    }
+    return F;
+}
+
+template <int FLUX>
+__host__ __device__ inline Q4 face_flux(const Q4& qL, const Q4& qR, const Phys& ph) {
+    const Prim L = cons_to_prim(qL, ph);
+    const Prim R = cons_to_prim(qR, ph);
+    if (FLUX == 0) return roe_flux(L, R, ph);
+    return hlll_flux(L, R, ph);
+}
+
+// Geometry of one cell as seen by the reconstruction: GG face weights x/yl = L*n (per face),
+// 1/A, and the quadrature-point offsets from the centroid (dxq = xq - xc, dyq = yq - yc).
+struct CellGeo {
+    double xlE, xlW, xlN, xlS, ylE, ylW, ylN, ylS;
+    double ainv;
+    double dxE, dxW, dxN, dxS, dyE, dyW, dyN, dyS;
+};
+
+struct Recon {
+    Q4 E, W, N, S;  // limited face states (SecondOrderMUSCL.limited_solution_at_quadrature_point)
+};
+
+// np.minimum of two scalars: a NaN operand is returned (NaN-propagating, unlike fmin).
+__host__ __device__ inline double np_minimum(double a, double b) {
+    return (a != a) ? a : ((b != b) ? b : ((a <= b) ? a : b));
+}
+
+// Venkatakrishnan._venkata (limiters/limiters.py:38-49) on the slope of one face, with the slope
+// of SlopeLimiter._compute_slope (limiters/base.py:189-221).
+__host__ __device__ inline double venkat_face(double dmax, double dmin, double d) {
+    const double num = (d > 0.0) ? dmax : dmin;
+    const double s = (d > 0.0 || d < 0.0) ? num / d : 1.0;
+    const double s2 = s * s;
+    return (s2 + 2.0 * s) / (s2 + s + 2.0);
+}
+
+// One cell of the second-order reconstruction:
+//  * interface averages (blocks/quad_block.py:181-218),
+//  * Green-Gauss gradient (gradients/greengauss.py:134-155),
+//  * high-order term gx*dx + gy*dy (blocks/base.py:282-289),
+//  * Venkatakrishnan limiter with the 5-point min/max incl. ghosts (limiters/base.py:47-187),
+//  * limited face values U + phi*hot (fvm/SecondOrderMUSCL.py:106-126).
+// U is the cell, Uw/Ue/Us/Un its west/east/south/north neighbours (ghost values at block edges).
+__host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
+                                             const Q4& Un, const CellGeo& g) {
+    Recon r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double u = U.v[k];
+        const double e = 0.5 * (u + Ue.v[k]);
+        const double w = 0.5 * (Uw.v[k] + u);
+        const double n = 0.5 * (u + Un.v[k]);
+        const double s = 0.5 * (Us.v[k] + u);
+        const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+        const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+        const double hE = gx * g.dxE + gy * g.dyE;
+        const double hW = gx * g.dxW + gy * g.dyW;
+        const double hN = gx * g.dxN + gy * g.dyN;
+        const double hS = gx * g.dxS + gy * g.dyS;
+        // np.maximum.reduce((U, UW, UE, US, UN)) - U
+        double mx = fmax(fmax(fmax(fmax(u, Uw.v[k]), Ue.v[k]), Us.v[k]), Un.v[k]);
+        double mn = fmin(fmin(fmin(fmin(u, Uw.v[k]), Ue.v[k]), Us.v[k]), Un.v[k]);
+        const double dmax = mx - u;
+        const double dmin = mn - u;
+        const double pE = venkat_face(dmax, dmin, (u + hE) - u);
+        const double pW = venkat_face(dmax, dmin, (u + hW) - u);
+        const double pN = venkat_face(dmax, dmin, (u + hN) - u);
+        const double pS = venkat_face(dmax, dmin, (u + hS) - u);
+        double phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+        phi = (phi < 0.0) ? 0.0 : phi;                  // np.where(phi < 0, 0, phi)
+        r.E.v[k] = u + phi * hE;
+        r.W.v[k] = u + phi * hW;
+        r.N.v[k] = u + phi * hN;
+        r.S.v[k] = u + phi * hS;
+    }
+    return r;
+}
+
+// Ideal-gas CFL quantity min(dx/(|u|+a), dy/(|v|+a)) of one cell (blocks/quad_block.py:433-436;
+// ConservativeState.u/v/p/a, states/conservative.py:85-137).
+__host__ __device__ inline double cell_dt(const Q4& q, double dx, double dy, const Phys& ph) {
+    const double rho = q.v[0];
+    const double u = q.v[1] / rho;
+    const double v = q.v[2] / rho;
+    const double p = ph.gm1 * (q.v[3] - rho * (0.5 * (u * u + v * v)));
+    const double a = sqrt(ph.gamma * p / rho);
+    const double tx = dx / (fabs(u) + a);
+    const double ty = dy / (fabs(v) + a);
+    return (tx <= ty) ? tx : ty;
+}
+
+}  // namespace phx

@@ -1,0 +1,184 @@
+"""Euler2D: the reference's solver API on top of the gfx950 backend.
+
+``Euler2D(config, mesh_config)`` accepts the same ``SolverConfig`` and mesh dictionary / generator
+as the reference (solvers/Euler2D.py:46-100, solvers/base.py:44-100).  ``solve()`` follows the
+loop of ``Euler2D._solve`` (solvers/Euler2D.py:175-216): dt from the CFL condition clipped to the
+final time, one explicit Runge-Kutta step of all blocks, a realizability check, ``t += dt`` --
+with every step executed on the device.  ``run_steps(n)`` advances exactly ``n`` steps.
+``Euler2DBatch`` advances many independent cases (each with its own dt) in the same launches.
+"""
+from __future__ import annotations
+
+import logging
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from .backend import CaseSpec, DevicePlan
+from .mesh import MeshGenerator
+from .states import ConservativeState
+from .time_marching import butcher_table
+
+_log = logging.getLogger("pyhype_amd")
+
+SUPPORTED_FVM = ("MUSCL", "MUSCL-HIP")
+
+
+class _Mesh:
+    """The parts of QuadMesh an initial condition reads: centroids shaped (ny, nx, 1)."""
+
+    def __init__(self, geom):
+        self.x = geom.xc[:, :, None]
+        self.y = geom.yc[:, :, None]
+        self.nx, self.ny = geom.nx, geom.ny
+        self.A = geom.A[:, :, None]
+        self.geometry = geom
+
+
+class HostBlock:
+    """Host-side view of one block for the initial-condition protocol
+    (initial_conditions/base.py:28-31): ``mesh.x/y``, ``config``, ``state``."""
+
+    def __init__(self, config, entry):
+        self.config = config
+        self.global_block_num = entry.num
+        self.mesh = _Mesh(entry.geom)
+        self.state = ConservativeState(config.fluid, shape=(entry.ny, entry.nx, 4))
+        self.entry = entry
+
+
+def validate_config(config) -> List[List[float]]:
+    """The construction-time checks of the reference factories, for the configuration space the
+    backend implements (fvm/factory.py:33-65, fvm/SecondOrderMUSCL.py:49-52, flux/factory.py:30-47,
+    limiters/factory.py:28-39, gradients/factory.py:26-37, time_marching/factory.py:29-62)."""
+    if config.fvm_type not in SUPPORTED_FVM:
+        raise ValueError(f"Specified finite volume method type {config.fvm_type} has not been specialized.")
+    if config.fvm_spatial_order != 2:
+        raise ValueError(f"No MUSCL finite volume method has been specialized with order {config.fvm_spatial_order}")
+    if config.nghost != 1:
+        raise ValueError("Number of ghost cells must be equal to 1 for this method.")
+    if config.fvm_num_quadrature_points != 1:
+        raise ValueError("only the 1-point Gauss rule is implemented")
+    if config.fvm_gradient_type != "GreenGauss":
+        raise ValueError(f"Gradient type {config.fvm_gradient_type} is not available.")
+    if config.fvm_flux_function_type not in ("Roe", "HLLL"):
+        raise ValueError(f"Flux function type {config.fvm_flux_function_type} is not available.")
+    if config.fvm_slope_limiter_type != "Venkatakrishnan":
+        raise ValueError("MUSCL: Slope limiter type not specified.")
+    if config.reconstruction_type is not ConservativeState:
+        raise ValueError("only ConservativeState reconstruction is implemented")
+    if config.interface_interpolation != "arithmetic_average":
+        raise ValueError("Interface Interpolation method is not defined.")
+    return butcher_table(config.time_integrator, config)
+
+
+def mesh_dict(mesh_config) -> Dict[int, dict]:
+    return mesh_config.dict if isinstance(mesh_config, MeshGenerator) else mesh_config
+
+
+class Euler2DBatch:
+    """Independent cases advanced together.  All cases share flux, limiter, time integrator, CFL and
+    fluid; each keeps its own block size, dt (CFL of its own blocks) and time."""
+
+    def __init__(self, configs: Sequence, meshes: Sequence, *, rows_per_item: int = 0, grid_waves: int = 0,
+                 force_general: bool = False):
+        if len(configs) != len(meshes) or not configs:
+            raise ValueError("need one mesh per config")
+        c0 = configs[0]
+        self.table = validate_config(c0)
+        for c in configs[1:]:
+            validate_config(c)
+            for attr in ("fvm_flux_function_type", "time_integrator", "CFL"):
+                if getattr(c, attr) != getattr(c0, attr):
+                    raise ValueError(f"batched cases must share {attr}")
+            if type(c.fluid) is not type(c0.fluid):
+                raise ValueError("batched cases must share the fluid")
+        self.configs = list(configs)
+        self.fluid = c0.fluid
+        specs = [CaseSpec(mesh_dict(m), c.nx, c.ny, c.t_final * c.fluid.far_field.a) for c, m in zip(configs, meshes)]
+        self.plan = DevicePlan(specs, flux=c0.fvm_flux_function_type, table=self.table, fluid=c0.fluid,
+                               cfl=c0.CFL, rows_per_item=rows_per_item, grid_waves=grid_waves,
+                               force_general=force_general)
+        self.case_blocks: List[Dict[int, HostBlock]] = []
+        for c, entries in enumerate(self.plan.case_blocks):
+            self.case_blocks.append({n: HostBlock(self.configs[c], e) for n, e in entries.items()})
+        self.t = np.zeros(len(configs))
+        self.num_time_step = 0
+        self.failed: Dict[int, int] = {}
+
+    # ---------------------------------------------------------------- set-up
+    def apply_initial_condition(self):
+        """Euler2D.apply_initial_condition (solvers/Euler2D.py:90-92), then upload."""
+        for c, blocks in enumerate(self.case_blocks):
+            for blk in blocks.values():
+                self.configs[c].initial_condition.apply_to_block(blk)
+        self.upload()
+
+    def upload(self):
+        self.plan.set_states([{n: b.state.data for n, b in blocks.items()} for blocks in self.case_blocks])
+        self.plan.reset(self.t, self.num_time_step)
+
+    def set_state(self, states: Sequence[Dict[int, np.ndarray]]):
+        for c, blocks in enumerate(self.case_blocks):
+            for n, b in blocks.items():
+                b.state.data = np.array(states[c][n], dtype=np.float64)
+        self.upload()
+
+    # ---------------------------------------------------------------- stepping
+    strict = False  # raise on any unrealizable case (True for the single-case Euler2D)
+
+    def run_steps(self, n: int, use_graph: bool = True):
+        """Advance exactly n steps (dt per case from the CFL condition, clipped to t_final).
+
+        A case whose state becomes unrealizable (rho <= 0 or E <= 0) is frozen at that step, as
+        the reference aborts that run (solvers/Euler2D.py:144-152); ``failed`` maps case -> step.
+        With ``strict`` a RealizabilityError is raised instead."""
+        if n <= 0:
+            return
+        self.plan.run(n, use_graph=use_graph, check=self.strict)
+        bad, step, t, _ = self.plan.status()
+        self.t = t
+        self.num_time_step = step
+        self.failed = {c: int(b) for c, b in enumerate(bad) if b >= 0}
+
+    def states(self) -> List[Dict[int, np.ndarray]]:
+        return self.plan.get_states()
+
+    def sync_to_host(self):
+        for c, st in enumerate(self.states()):
+            for n, arr in st.items():
+                self.case_blocks[c][n].state.data = arr
+
+    def last_dt(self) -> np.ndarray:
+        return self.plan.status()[3]
+
+
+class Euler2D(Euler2DBatch):
+    """Single-case solver with the reference's constructor and ``solve()``."""
+
+    strict = True
+
+    def __init__(self, config, mesh_config, **kw):
+        super().__init__([config], [mesh_config], **kw)
+        self.config = config
+        self.t_final = config.t_final * config.fluid.far_field.a
+
+    @property
+    def blocks(self):
+        return self.case_blocks[0].values()
+
+    def state(self, block_num: int) -> np.ndarray:
+        return self.states()[0][block_num]
+
+    def solve(self, chunk: int = 50, max_steps: Optional[int] = None):
+        """Run to t_final (solvers/Euler2D.py:97-100,175-216)."""
+        self.apply_initial_condition()
+        steps = 0
+        while self.t[0] < self.t_final and (max_steps is None or steps < max_steps):
+            if self.num_time_step % 50 == 0:
+                _log.info("Simulation time: %s, Timestep number: %s", self.t[0] / self.fluid.far_field.a,
+                          self.num_time_step)
+            n = chunk if max_steps is None else min(chunk, max_steps - steps)
+            self.run_steps(n)
+            steps += n
+        self.sync_to_host()

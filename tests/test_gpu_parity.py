@@ -1,0 +1,163 @@
+"""Parity of the HIP backend with the oracle (gpu).  Every comparison is bit-for-bit unless noted;
+the north-star bar (max relative error <= 1e-9 per variable) is asserted as well."""
+import numpy as np
+import pytest
+
+from oracle import pyhype_oracle as O
+from pyhype_amd import Euler2D, Euler2DBatch, RealizabilityError
+from pyhype_amd import workloads as W
+from tests.helpers import bitwise_equal_blocks, max_rel_err_blocks, oracle_for
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def _solver(cfg, mesh, **kw):
+    s = Euler2D(cfg, mesh, **kw)
+    s.apply_initial_condition()
+    return s
+
+
+def _perturbed(states, seed=0, amp=1e-3):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for n, a in states.items():
+        b = a.copy()
+        b *= 1 + amp * rng.standard_normal(b.shape)
+        b[..., 1:3] += amp * rng.standard_normal(b[..., 1:3].shape) * np.abs(a[..., 0:1]).max()
+        out[n] = b
+    return out
+
+
+@pytest.mark.parametrize("case", ["single", "multiblock", "oblique"])
+def test_residual_matches_oracle(case):
+    if case == "single":
+        cfg, mesh = W.explosion_single(24)
+    elif case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(12, 2, 4)
+    else:
+        cfg, mesh = W.oblique_shock(16, 3)
+    s = _solver(cfg, mesh)
+    st = _perturbed(s.states()[0], seed=3)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        ref = O.residual(b, sim.flux_fn)
+        np.testing.assert_array_equal(R[n], ref, err_msg=f"block {n}")
+
+
+def test_ghost_fill_matches_oracle():
+    cfg, mesh = W.explosion_multiblock(10, 2, 3)
+    s = _solver(cfg, mesh)
+    st = _perturbed(s.states()[0], seed=4)
+    s.set_state([st])
+    G = s.plan.ghosts()
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        for d in "EWNS":
+            np.testing.assert_array_equal(G[(0, n)][d], b.ghost[d], err_msg=f"block {n} side {d}")
+
+
+@pytest.mark.parametrize("case", ["single", "multiblock"])
+def test_dt_matches_oracle(case):
+    cfg, mesh = W.explosion_single(30) if case == "single" else W.explosion_multiblock(10, 2, 4)
+    s = _solver(cfg, mesh)
+    st = _perturbed(s.states()[0], seed=5)
+    s.set_state([st])
+    dt = float(s.plan.dt().cpu().numpy()[0])
+    assert dt == oracle_for(cfg, mesh, st).get_dt()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_explosion_single_steps(use_graph):
+    cfg, mesh = W.explosion_single(40)
+    s = _solver(cfg, mesh)
+    sim = oracle_for(cfg, mesh, s.states()[0])
+    s.run_steps(12, use_graph=use_graph)
+    sim.run(12)
+    g, r = s.states()[0], sim.states()
+    assert max_rel_err_blocks(g, r) <= TOL
+    assert bitwise_equal_blocks(g, r)
+    assert s.t[0] == sim.t
+
+
+def test_explosion_multiblock_steps():
+    cfg, mesh = W.explosion_multiblock(16, 2, 4)
+    s = _solver(cfg, mesh)
+    sim = oracle_for(cfg, mesh, s.states()[0])
+    s.run_steps(8)
+    sim.run(8)
+    g, r = s.states()[0], sim.states()
+    assert max_rel_err_blocks(g, r) <= TOL
+    assert bitwise_equal_blocks(g, r)
+
+
+def test_oblique_shock_hlll_ssprk2_steps():
+    cfg, mesh = W.oblique_shock(20, 3)
+    s = _solver(cfg, mesh)
+    sim = oracle_for(cfg, mesh, s.states()[0])
+    s.run_steps(10)
+    sim.run(10)
+    g, r = s.states()[0], sim.states()
+    assert max_rel_err_blocks(g, r) <= TOL
+    assert bitwise_equal_blocks(g, r)
+
+
+def test_batched_uneven_cases_match_independent_oracles():
+    """Uneven independent cases advanced in the same launches, each with its own dt; case 1 is one
+    the reference scheme cannot advance (pressure ratio 7.4 at CFL 0.8): it fails at step 1 in
+    both, is frozen on the device, and the other cases are unaffected."""
+    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52])[0]
+    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28])[0]
+    cfgs += W.explosion_batch(seed=3, ncases=1, sizes=[44])[0]
+    meshes = [W.single_block_mesh() for _ in cfgs]
+    b = Euler2DBatch(cfgs, meshes)
+    b.apply_initial_condition()
+    st0 = b.states()
+    b.run_steps(6)
+    got = b.states()
+    assert b.failed == {2: 1}
+    for c in (0, 1, 3):
+        sim = oracle_for(cfgs[c], meshes[c], st0[c])
+        sim.run(6)
+        assert bitwise_equal_blocks(got[c], sim.states()), f"case {c}"
+        assert b.t[c] == sim.t
+    sim = oracle_for(cfgs[2], meshes[2], st0[2])
+    with pytest.raises(FloatingPointError, match="after step 1"):
+        sim.run(1)
+
+
+def test_unstable_case_fails_like_the_oracle():
+    """A drawn case the reference scheme cannot advance (24x24, pressure ratio 7.4, CFL 0.8): the
+    oracle and the device both report an unrealizable state after step 1."""
+    cfgs, meshes, _ = W.explosion_batch(seed=0, ncases=1, sizes=[24])
+    s = Euler2D(cfgs[0], meshes[0])
+    s.apply_initial_condition()
+    sim = oracle_for(cfgs[0], meshes[0], s.states()[0])
+    with pytest.raises(FloatingPointError, match="after step 1"):
+        sim.run(1)
+    with pytest.raises(RealizabilityError, match="after step 1"):
+        s.run_steps(1)
+
+
+def test_explosion_single_full_size():
+    """The single-block workload at its full size and step count (100x100, 50 RK4 steps)."""
+    cfg, mesh = W.explosion_single(100)
+    s = _solver(cfg, mesh)
+    sim = oracle_for(cfg, mesh, s.states()[0])
+    s.run_steps(50)
+    sim.run(50)
+    g, r = s.states()[0], sim.states()
+    assert max_rel_err_blocks(g, r) <= TOL
+    assert bitwise_equal_blocks(g, r)
+
+
+def test_unrealizable_state_raises():
+    cfg, mesh = W.explosion_single(16)
+    s = _solver(cfg, mesh)
+    st = s.states()[0]
+    st[0][3, 3, 0] = -1.0
+    s.set_state([st])
+    with pytest.raises(RealizabilityError):
+        s.run_steps(1)

@@ -1,0 +1,113 @@
+"""The kernels' per-face / per-cell arithmetic (pyhype_amd/csrc/phx_numerics.hpp, built for the host)
+against the oracle, bit for bit, on seeded random inputs including the branches the reference
+takes (Harten fix active, HLLL supersonic branches, limiter d == 0, walls)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyhype_oracle as O
+
+PHYS = np.array([O.GAMMA, O.GM1, O.ONE_OVER_GM1, O.G_OVER_GM1, 0.8])
+
+
+def dp(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double))
+
+
+def rand_prim(rng, n, umax=1.0):
+    rho = rng.uniform(0.2, 5.0, n)
+    u = rng.uniform(-umax, umax, n)
+    v = rng.uniform(-umax, umax, n)
+    p = rng.uniform(0.1, 5.0, n)
+    return np.stack([rho, u, v, p], axis=1)
+
+
+def face_pairs(rng, n):
+    WL = rand_prim(rng, n, 2.0)
+    WR = rand_prim(rng, n, 2.0)
+    # nearly equal pairs (Harten correction active near sonic points) and identical pairs
+    k = n // 4
+    WR[:k] = WL[:k] * (1 + 1e-3 * rng.standard_normal((k, 4)))
+    WL[k:2 * k, 1] = np.sqrt(O.GAMMA * WL[k:2 * k, 3] / WL[k:2 * k, 0]) * rng.uniform(0.95, 1.05, k)
+    WR[2 * k:2 * k + 10] = WL[2 * k:2 * k + 10]
+    # supersonic to the right and to the left (HLLL upwind branches)
+    WL[-20:-10, 1] = 5.0
+    WR[-20:-10, 1] = 5.0
+    WL[-10:, 1] = -5.0
+    WR[-10:, 1] = -5.0
+    return WL, WR
+
+
+@pytest.mark.parametrize("flux", ["Roe", "HLLL"])
+def test_face_flux_bitwise(numerics_shim, flux):
+    rng = np.random.default_rng(1234)
+    n = 4000
+    WL, WR = face_pairs(rng, n)
+    F = np.zeros((n, 4))
+    numerics_shim.shim_prim_flux(0 if flux == "Roe" else 1, n, dp(WL), dp(WR), dp(F), dp(PHYS))
+    fn = O.flux_roe if flux == "Roe" else O.flux_hlll
+    ref = fn(WL.reshape(1, n, 4), WR.reshape(1, n, 4)).reshape(n, 4)
+    np.testing.assert_array_equal(F, ref)
+
+
+def test_cons_to_prim_and_reflect(numerics_shim):
+    rng = np.random.default_rng(7)
+    n = 1000
+    U = O.prim_to_cons(rand_prim(rng, n).reshape(1, n, 4)).reshape(n, 4)
+    W = np.zeros((n, 4))
+    numerics_shim.shim_cons_to_prim(n, dp(U), dp(W), dp(PHYS))
+    np.testing.assert_array_equal(W, O.cons_to_prim(U.reshape(1, n, 4)).reshape(n, 4))
+    th = rng.uniform(-np.pi, np.pi, n)
+    th[:10] = np.pi / 2
+    th[10:20] = 0.0
+    c = np.array([np.cos(t) for t in th])
+    import math
+    c = np.array([math.cos(t) for t in th])
+    s = np.array([math.sin(t) for t in th])
+    out = np.zeros((n, 4))
+    numerics_shim.shim_reflect(n, dp(U), dp(c), dp(s), dp(out))
+    ref = U.reshape(n, 1, 4).copy()
+    O.reflect(ref, th.reshape(n, 1))
+    np.testing.assert_array_equal(out, ref.reshape(n, 4))
+
+
+def test_reconstruct_matches_oracle_on_block(numerics_shim):
+    """Cell reconstruction inside a random block with walls, using the oracle's geometry."""
+    rng = np.random.default_rng(99)
+    nx, ny = 13, 9
+    for vert in ({"SW": [0, 0], "SE": [10, 0], "NW": [0, 7], "NE": [10, 7]},
+                 {"SW": [0, 0], "SE": [4.999999999999999, 0.1], "NW": [0.2, 5.000000000000001], "NE": [5, 5]}):
+        g = O.block_geometry(vert, nx, ny)
+        W = rand_prim(rng, nx * ny).reshape(ny, nx, 4)
+        W[2:5, 3:6] = W[2, 3]          # a uniform patch (limiter d == 0 branch)
+        U = O.prim_to_cons(W)
+        b = O.Block(0, g, {d: None for d in O.DIRS}, {d: "Reflection" for d in O.DIRS}, U=U)
+        O.apply_boundary_condition({0: b})
+        gx, gy = O.green_gauss(U, b.ghost, g)
+        hot = {d: O.high_order_term(gx, gy, g, d) for d in O.DIRS}
+        phi = O.limiter_phi(U, b.ghost, hot)
+        ref = np.concatenate([U + phi * hot[d] for d in ("E", "W", "N", "S")], axis=2)
+        EW = np.concatenate((b.ghost["W"], U, b.ghost["E"]), axis=1)
+        NS = np.concatenate((b.ghost["S"], U, b.ghost["N"]), axis=0)
+        Uw, Ue, Us, Un = EW[:, :-2], EW[:, 2:], NS[:-2], NS[2:]
+        f = g.face
+        cols = [f["E"].L * f["E"].nx, f["W"].L * f["W"].nx, f["N"].L * f["N"].nx, f["S"].L * f["S"].nx,
+                f["E"].L * f["E"].ny, f["W"].L * f["W"].ny, f["N"].L * f["N"].ny, f["S"].L * f["S"].ny,
+                1.0 / g.A,
+                f["E"].qx - g.xc, f["W"].qx - g.xc, f["N"].qx - g.xc, f["S"].qx - g.xc,
+                f["E"].qy - g.yc, f["W"].qy - g.yc, f["N"].qy - g.yc, f["S"].qy - g.yc]
+        G = np.stack(cols, axis=-1).reshape(-1, 17)
+        out = np.zeros((nx * ny, 16))
+        numerics_shim.shim_reconstruct(nx * ny, dp(U), dp(Uw), dp(Ue), dp(Us), dp(Un), dp(G), dp(out))
+        np.testing.assert_array_equal(out.reshape(ny, nx, 16), ref)
+
+
+def test_cell_dt(numerics_shim):
+    rng = np.random.default_rng(5)
+    g = O.block_geometry({"SW": [0, 0], "SE": [3, 0], "NW": [0, 2], "NE": [3, 2]}, 11, 7)
+    U = O.prim_to_cons(rand_prim(rng, 77).reshape(7, 11, 4))
+    b = O.Block(0, g, {}, {}, U=U)
+    out = np.zeros(77)
+    numerics_shim.shim_cell_dt(77, dp(U), dp(g.dx), dp(g.dy), dp(out), dp(PHYS))
+    assert 0.8 * out.min() == O.block_dt(b, 0.8)
