@@ -159,9 +159,11 @@ int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host,
  * fvm/base.py:108-190 with the active limiter/flux branches).  No counterpart in the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
 
-/* Tunables of the stage launch (rows per work item, persistent grid size in waves);
- * rebuilds the work list.  Used by the autotuner in pyhype_amd/autotune.py. */
-int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves);
+/* Tunables of the stage launch: rows per work item, persistent grid size in waves (0 = one
+ * workgroup per item) and the register budget variant (waves per SIMD, 2 or 3; 0 = 2); rebuilds
+ * the work list.  Used by the autotuner in pyhype_amd/autotune.py.  No counterpart in the
+ * reference. */
+int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd);
 
 /* Number of work items and the persistent grid used for the Cartesian and general kernels. */
 int phx_launch_info(const phx_plan* plan, int32_t* items_cart, int32_t* items_gen,

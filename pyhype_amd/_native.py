@@ -71,7 +71,7 @@ def load(path: Path = LIB_PATH):
         "phx_ghosts": ([vp, vp, vp, vp], C.c_int),
         "phx_status": ([vp, C.POINTER(i32), C.POINTER(i32), dp, dp], C.c_int),
         "phx_residual_vjp": ([vp, vp, vp, vp, vp], C.c_int),
-        "phx_set_launch": ([vp, i32, i32], C.c_int),
+        "phx_set_launch": ([vp, i32, i32, i32], C.c_int),
         "phx_launch_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "phx_last_error": ([], C.c_char_p),
     }

@@ -1,0 +1,95 @@
+"""Launch-shape autotuner for the stage kernel.
+
+The stage kernel's work item is a 60-column strip of one block over ``rows_per_item`` rows; each
+item recomputes one reconstruction row above and below its range, so short items waste work while
+long items leave SIMDs idle on small meshes.  ``autotune`` times whole steps (hipGraph replay)
+for a set of candidate shapes and keeps the fastest; results are cached in a JSON file keyed by
+a signature of the workload (geometry kind, flux, stages, block shapes, device).
+The state, time and step counter of the plan are restored after tuning.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from collections import Counter
+from pathlib import Path
+from typing import Dict, Optional, Sequence
+
+CACHE = Path(os.environ.get("PYHYPE_AMD_TUNING", Path(__file__).resolve().parent / "tuning.json"))
+ROWS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 64)
+
+
+def signature(plan) -> str:
+    import torch
+
+    kinds = "cart" if all(e.cartesian for e in plan.blocks) else (
+        "gen" if not any(e.cartesian for e in plan.blocks) else "mixed")
+    shapes = Counter((e.nx, e.ny) for e in plan.blocks)
+    shp = ",".join(f"{n}x{a}x{b}" for (a, b), n in sorted(shapes.items()))
+    dev = getattr(torch.cuda.get_device_properties(0), "gcnArchName", "gpu").split(":")[0] \
+        if torch.cuda.is_available() else "none"
+    return f"{kinds}|{plan.flux}|{len(plan.table)}|{shp}|{dev}"
+
+
+def load_cache(path: Path = CACHE) -> Dict[str, dict]:
+    try:
+        return json.loads(Path(path).read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def lookup(plan, path: Path = CACHE) -> Optional[dict]:
+    return load_cache(path).get(signature(plan))
+
+
+def _time_steps(plan, steps: int) -> float:
+    import torch
+
+    plan.run(1, use_graph=True, check=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.run(steps, use_graph=True, check=False)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def autotune(plan, steps: int = 10, rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,),
+             waves: Sequence[int] = (2, 3), path: Optional[Path] = CACHE, verbose: bool = False) -> dict:
+    """Time candidate (rows_per_item, grid_waves, waves_per_simd) shapes on the plan's current
+    state; apply and cache the best.  Returns the best shape, its ms/step and all trials."""
+    import numpy as np
+
+    U_saved = plan.U.clone()
+    _, step0, t0, _ = plan.status()
+    trials = []
+    for w in waves:
+        for g in grids:
+            for r in rows:
+                plan.set_launch(r, g, w)
+                plan.U.copy_(U_saved)
+                plan.reset(t0, step0)
+                ms = _time_steps(plan, steps) * 1e3
+                trials.append({"rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "ms_per_step": ms})
+                if verbose:
+                    print(f"autotune rows={r} grid={g} waves={w}: {ms:.4f} ms/step", flush=True)
+    best = min(trials, key=lambda t: t["ms_per_step"])
+    plan.set_launch(best["rows_per_item"], best["grid_waves"], best["waves_per_simd"])
+    plan.U.copy_(U_saved)
+    plan.reset(np.asarray(t0), step0)
+    result = dict(best, trials=trials)
+    if path is not None:
+        cache = load_cache(path)
+        cache[signature(plan)] = {k: best[k] for k in ("rows_per_item", "grid_waves", "waves_per_simd",
+                                                          "ms_per_step")}
+        Path(path).write_text(json.dumps(cache, indent=1, sort_keys=True))
+    return result
+
+
+def apply_cached(plan, path: Path = CACHE) -> bool:
+    """Apply the cached shape for this workload if there is one."""
+    hit = lookup(plan, path)
+    if hit is None:
+        return False
+    plan.set_launch(int(hit["rows_per_item"]), int(hit["grid_waves"]), int(hit.get("waves_per_simd", 2)))
+    return True

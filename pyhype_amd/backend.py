@@ -178,6 +178,7 @@ class DevicePlan:
         self.handle = handle
         self.U = torch.zeros(self.state_len, dtype=torch.float64, device="cuda")
         self.nsteps_done = 0
+        self.launch = {"rows_per_item": rows_per_item or 32, "grid_waves": grid_waves, "waves_per_simd": 2}
 
     # ------------------------------------------------------------------ lifetime
     def close(self):
@@ -308,13 +309,18 @@ class DevicePlan:
                   "phx_residual_vjp")
         return lamU
 
-    def set_launch(self, rows_per_item: int, grid_waves: int = 0):
-        nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves)), "phx_set_launch")
+    def set_launch(self, rows_per_item: int, grid_waves: int = 0, waves_per_simd: int = 2):
+        nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves), int(waves_per_simd)),
+                  "phx_set_launch")
+        self.launch = {"rows_per_item": int(rows_per_item), "grid_waves": int(grid_waves),
+                       "waves_per_simd": int(waves_per_simd)}
 
     def launch_info(self):
         v = [C.c_int32() for _ in range(4)]
         nat.check(self.lib.phx_launch_info(self.handle, *[C.byref(x) for x in v]), "phx_launch_info")
-        return {"items_cart": v[0].value, "items_gen": v[1].value, "grid_cart": v[2].value, "grid_gen": v[3].value}
+        out = {"items_cart": v[0].value, "items_gen": v[1].value, "grid_cart": v[2].value, "grid_gen": v[3].value}
+        out.update(getattr(self, "launch", {}))
+        return out
 
     def cells(self) -> int:
         return sum(e.nx * e.ny for e in self.blocks)

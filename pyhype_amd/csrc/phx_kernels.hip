@@ -269,7 +269,7 @@ __device__ __forceinline__ void cell_dxdy(const DevBlock& B, int k, int j, doubl
 // ---------------------------------------------------------------------------------------------
 template <bool CART, int FLUX>
 __device__ void stage_item(const StageArgs& A, const WorkItem it) {
-    const DevBlock B = A.blocks[it.block];
+    const DevBlock& B = A.blocks[it.block];
     // a case that became unrealizable is frozen: the reference aborts that run (Euler2D.py:144-152)
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -334,7 +334,9 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
                 for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * F.v[c]))) / prevA;
                 const long idx = B.off + 4 * ((long)kk * nx + j);
                 if (A.Rout) st4(A.Rout + idx, Res);
-                for (int m = 0; m < A.nout; ++m) {
+#pragma unroll
+                for (int m = 0; m < MAX_OUT; ++m) {
+                    if (m >= A.nout) break;
                     const Q4 s = ld4(A.src[m] + idx);
                     Q4 o;
                     const double am = A.a[m];
@@ -399,8 +401,9 @@ __device__ __forceinline__ void item_range(int n, int& lo, int& hi, int& first, 
     }
 }
 
-template <bool CART, int FLUX>
-__global__ void __launch_bounds__(WAVE) k_stage(StageArgs A) {
+// MINW: minimum waves per SIMD the register allocation must allow (2 or 3; tuned per workload)
+template <bool CART, int FLUX, int MINW>
+__global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) stage_item<CART, FLUX>(A, A.items[i]);
@@ -425,7 +428,7 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
     const int lane = threadIdx.x & (WAVE - 1);
     for (int i = first; i < hi; i += stride) {
         const WorkItem it = A.items[i];
-        const DevBlock B = A.blocks[it.block];
+        const DevBlock& B = A.blocks[it.block];
         const int j = it.j0 + lane;
         const bool own = (lane < STRIP) && (j < B.nx);
         double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
@@ -497,7 +500,7 @@ __global__ void k_fill_inf(int ncases, unsigned long long* dtmin) {
 __global__ void k_ghosts(const DevBlock* blocks, int nblocks, const long long* goff, const double* U, double* G) {
     const int b = blockIdx.x;
     if (b >= nblocks) return;
-    const DevBlock B = blocks[b];
+    const DevBlock& B = blocks[b];
     const int n = 2 * B.ny + 2 * B.nx;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
         int k, j;
@@ -555,7 +558,7 @@ struct phx_plan {
     DevBlock* dblocks = nullptr;
     double* dgeom = nullptr;
     int64_t state_len = 0;
-    int rows_per_item = 32, grid_waves = 0;
+    int rows_per_item = 32, grid_waves = 0, waves_per_simd = 2;
     std::vector<WorkItem> items[2];  // [0] general, [1] cartesian
     WorkItem* ditems[2] = {nullptr, nullptr};
     int grid[2] = {0, 0};
@@ -668,12 +671,24 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
         A.items = P->ditems[kind];
         A.nitems = n;
         dim3 grid(P->grid[kind]), block(WAVE);
+        const bool w3 = P->waves_per_simd == 3;
+        const bool roe = P->flux == PHX_FLUX_ROE;
         if (kind == 1) {
-            if (P->flux == PHX_FLUX_ROE) hipLaunchKernelGGL((k_stage<true, 0>), grid, block, 0, st, A);
-            else hipLaunchKernelGGL((k_stage<true, 1>), grid, block, 0, st, A);
+            if (roe) {
+                if (w3) hipLaunchKernelGGL((k_stage<true, 0, 3>), grid, block, 0, st, A);
+                else hipLaunchKernelGGL((k_stage<true, 0, 2>), grid, block, 0, st, A);
+            } else {
+                if (w3) hipLaunchKernelGGL((k_stage<true, 1, 3>), grid, block, 0, st, A);
+                else hipLaunchKernelGGL((k_stage<true, 1, 2>), grid, block, 0, st, A);
+            }
         } else {
-            if (P->flux == PHX_FLUX_ROE) hipLaunchKernelGGL((k_stage<false, 0>), grid, block, 0, st, A);
-            else hipLaunchKernelGGL((k_stage<false, 1>), grid, block, 0, st, A);
+            if (roe) {
+                if (w3) hipLaunchKernelGGL((k_stage<false, 0, 3>), grid, block, 0, st, A);
+                else hipLaunchKernelGGL((k_stage<false, 0, 2>), grid, block, 0, st, A);
+            } else {
+                if (w3) hipLaunchKernelGGL((k_stage<false, 1, 3>), grid, block, 0, st, A);
+                else hipLaunchKernelGGL((k_stage<false, 1, 2>), grid, block, 0, st, A);
+            }
         }
     }
 }
@@ -982,11 +997,13 @@ int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* l
     return fail(PHX_E_STATE, "phx_residual_vjp is not available in this build");
 }
 
-int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves) {
-    if (!P || rows_per_item < 1 || grid_waves < 0) return fail(PHX_E_INVALID, "invalid launch parameters");
+int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd) {
+    if (!P || rows_per_item < 1 || grid_waves < 0 || waves_per_simd < 0 || waves_per_simd > 3 || waves_per_simd == 1)
+        return fail(PHX_E_INVALID, "invalid launch parameters");
     PHX_CK(hipDeviceSynchronize());
     P->rows_per_item = rows_per_item;
     P->grid_waves = grid_waves;
+    P->waves_per_simd = waves_per_simd == 0 ? 2 : waves_per_simd;
     return build_items(P);
 }
 

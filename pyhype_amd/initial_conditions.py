@@ -38,17 +38,19 @@ class ExplosionInitialCondition(InitialCondition):
 
 
 class SquareExplosionIC(InitialCondition):
-    """Square [x0,x1]x[y0,y1] at (ratio*rho_low, ratio*p_low), elsewhere (rho_low, p_low)."""
+    """Square [x0,x1]x[y0,y1] at pressure ratio*p_low and density density_ratio*rho_low, still air
+    (rho_low, p_low) elsewhere."""
 
-    def __init__(self, x0, x1, y0, y1, ratio, rho_low=1.1742, p_low=101100.0):
+    def __init__(self, x0, x1, y0, y1, ratio, rho_low=1.1742, p_low=101100.0, density_ratio=None):
         self.box = (x0, x1, y0, y1)
         self.ratio = ratio
+        self.density_ratio = ratio if density_ratio is None else density_ratio
         self.low = (rho_low, p_low)
 
     def apply_to_block(self, block):
         f = block.config.fluid
         rl, pl = self.low
-        inside = _cons(f, self.ratio * rl, 0.0, 0.0, self.ratio * pl)
+        inside = _cons(f, self.density_ratio * rl, 0.0, 0.0, self.ratio * pl)
         outside = _cons(f, rl, 0.0, 0.0, pl)
         x0, x1, y0, y1 = self.box
         xc = np.logical_and(block.mesh.x >= x0, block.mesh.x <= x1)

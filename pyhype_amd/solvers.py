@@ -72,6 +72,11 @@ def validate_config(config) -> List[List[float]]:
     return butcher_table(config.time_integrator, config)
 
 
+def default_rows(cells: int) -> int:
+    """Rows per work item when no tuned value is cached: about 4096 items of 60 columns."""
+    return int(max(2, min(64, round(cells / (60 * 4096)))))
+
+
 def mesh_dict(mesh_config) -> Dict[int, dict]:
     return mesh_config.dict if isinstance(mesh_config, MeshGenerator) else mesh_config
 
@@ -81,7 +86,7 @@ class Euler2DBatch:
     fluid; each keeps its own block size, dt (CFL of its own blocks) and time."""
 
     def __init__(self, configs: Sequence, meshes: Sequence, *, rows_per_item: int = 0, grid_waves: int = 0,
-                 force_general: bool = False):
+                 force_general: bool = False, use_tuning_cache: bool = True):
         if len(configs) != len(meshes) or not configs:
             raise ValueError("need one mesh per config")
         c0 = configs[0]
@@ -99,6 +104,13 @@ class Euler2DBatch:
         self.plan = DevicePlan(specs, flux=c0.fvm_flux_function_type, table=self.table, fluid=c0.fluid,
                                cfl=c0.CFL, rows_per_item=rows_per_item, grid_waves=grid_waves,
                                force_general=force_general)
+        self.tuned = False
+        if rows_per_item == 0:
+            from . import autotune
+
+            self.tuned = use_tuning_cache and autotune.apply_cached(self.plan)
+            if not self.tuned:
+                self.plan.set_launch(default_rows(self.plan.cells()), grid_waves)
         self.case_blocks: List[Dict[int, HostBlock]] = []
         for c, entries in enumerate(self.plan.case_blocks):
             self.case_blocks.append({n: HostBlock(self.configs[c], e) for n, e in entries.items()})
@@ -140,6 +152,14 @@ class Euler2DBatch:
         self.t = t
         self.num_time_step = step
         self.failed = {c: int(b) for c, b in enumerate(bad) if b >= 0}
+
+    def autotune(self, steps: int = 10, **kw):
+        """Tune the launch shape on the current state (state, time and step are restored)."""
+        from . import autotune
+
+        res = autotune.autotune(self.plan, steps=steps, **kw)
+        self.tuned = True
+        return res
 
     def states(self) -> List[Dict[int, np.ndarray]]:
         return self.plan.get_states()

@@ -108,8 +108,8 @@ def test_batched_uneven_cases_match_independent_oracles():
     """Uneven independent cases advanced in the same launches, each with its own dt; case 1 is one
     the reference scheme cannot advance (pressure ratio 7.4 at CFL 0.8): it fails at step 1 in
     both, is frozen on the device, and the other cases are unaffected."""
-    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52])[0]
-    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28])[0]
+    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52], density_scaled=True)[0]
+    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28], density_scaled=True)[0]
     cfgs += W.explosion_batch(seed=3, ncases=1, sizes=[44])[0]
     meshes = [W.single_block_mesh() for _ in cfgs]
     b = Euler2DBatch(cfgs, meshes)
@@ -131,7 +131,7 @@ def test_batched_uneven_cases_match_independent_oracles():
 def test_unstable_case_fails_like_the_oracle():
     """A drawn case the reference scheme cannot advance (24x24, pressure ratio 7.4, CFL 0.8): the
     oracle and the device both report an unrealizable state after step 1."""
-    cfgs, meshes, _ = W.explosion_batch(seed=0, ncases=1, sizes=[24])
+    cfgs, meshes, _ = W.explosion_batch(seed=0, ncases=1, sizes=[24], density_scaled=True)
     s = Euler2D(cfgs[0], meshes[0])
     s.apply_initial_condition()
     sim = oracle_for(cfgs[0], meshes[0], s.states()[0])

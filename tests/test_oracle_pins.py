@@ -1,0 +1,296 @@
+"""Pins of the CPU oracle by mathematics (no reference import is possible here: numba and mpi4py
+are absent).  Each check is chosen so that a plausible misreading -- a dropped term, a wrong sign
+or index, a transposed operand, a wrong COO entry -- fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyhype_oracle as O
+
+RECT = {"SW": [0.0, 0.0], "SE": [10.0, 0.0], "NW": [0.0, 5.0], "NE": [10.0, 5.0]}
+
+
+def mesh1(bc="Reflection", vert=RECT):
+    d = {"nBLK": 0, **vert}
+    for s in O.DIRS:
+        d["Neighbor" + s] = None
+        d["BCType" + s] = bc
+    return {0: d}
+
+
+# ------------------------------------------------------------------ geometry
+def test_rectangle_geometry_closed_forms():
+    g = O.block_geometry(RECT, 20, 10)
+    assert g.is_cartesian
+    np.testing.assert_allclose(g.A, 0.5 * 0.5, rtol=1e-13)
+    np.testing.assert_allclose(g.face["E"].L, 0.5, rtol=1e-14)
+    np.testing.assert_allclose(g.face["N"].L, 0.5, rtol=1e-14)
+    # normals: E (1, 0), W (-1, -0), N (cos(pi/2), 1), S (-cos(pi/2), -1)
+    assert np.all(g.face["E"].nx == 1.0) and np.all(g.face["E"].ny == 0.0)
+    assert np.all(g.face["W"].nx == -1.0)
+    assert np.all(g.face["N"].nx == math.cos(math.pi / 2)) and np.all(g.face["N"].ny == 1.0)
+    assert np.all(g.face["S"].ny == -1.0)
+    np.testing.assert_allclose(g.xc[0], np.linspace(0.25, 9.75, 20), rtol=1e-14)
+    np.testing.assert_allclose(g.dx, 0.5, rtol=1e-14)
+    # each face's quadrature point is its midpoint
+    for d in O.DIRS:
+        np.testing.assert_allclose(g.face[d].qx, g.face[d].mx, rtol=1e-15)
+
+
+def test_skewed_geometry_area_and_normals():
+    vert = {"SW": [0.0, 0.0], "SE": [4.0, 1.0], "NW": [1.0, 3.0], "NE": [5.0, 4.0]}   # parallelogram
+    g = O.block_geometry(vert, 8, 6)
+    assert not g.is_cartesian
+    total = (4.0 * 3.0 - 1.0 * 1.0)
+    np.testing.assert_allclose(g.A.sum(), total, rtol=1e-12)
+    for d in O.DIRS:   # unit normals
+        np.testing.assert_allclose(g.face[d].nx ** 2 + g.face[d].ny ** 2, 1.0, rtol=1e-14)
+    # closed cell: sum of L*n over the four faces vanishes
+    sx = sum(g.face[d].L * g.face[d].nx for d in O.DIRS)
+    sy = sum(g.face[d].L * g.face[d].ny for d in O.DIRS)
+    assert np.max(np.abs(sx)) < 1e-13 and np.max(np.abs(sy)) < 1e-13
+
+
+def test_multiblock_corner_interpolation():
+    """The rectangular generator's transfinite interior corners are off by ulps -- the reason the
+    2x4 multiblock mesh runs the reference's rotated-frame path."""
+    m = O.rectangular_mesh_dict(["Reflection"] * 1, ["Reflection"], ["Reflection"], ["Reflection"],
+                                10.0, 0.0, 20.0, 0.0, 2, 4)
+    ne = m[0]["NE"]
+    assert abs(ne[0] - 5.0) < 1e-14 and abs(ne[1] - 5.0) < 1e-14
+    assert not (ne[0] == 5.0 and ne[1] == 5.0)
+    assert m[0]["NeighborE"] == 1 and m[0]["NeighborN"] == 2 and m[7]["NeighborS"] == 5
+    assert m[0]["BCTypeE"] is None and m[1]["BCTypeE"] == "Reflection"
+
+
+# ------------------------------------------------------------------ gradient and limiter
+def _block_with(U, vert=RECT, bc="OutletDirichlet"):
+    ny, nx, _ = U.shape
+    g = O.block_geometry(vert, nx, ny)
+    b = O.Block(0, g, {d: None for d in O.DIRS}, {d: bc for d in O.DIRS}, U=U)
+    O.apply_boundary_condition({0: b})
+    return b
+
+
+def test_green_gauss_exact_on_linear_field():
+    nx, ny = 12, 8
+    g = O.block_geometry(RECT, nx, ny)
+    U = np.stack([2.0 + 0.3 * g.xc - 0.7 * g.yc, 1 + 0 * g.xc, 0.5 * g.yc, 3 + g.xc], axis=-1)
+    b = _block_with(U)
+    gx, gy = O.green_gauss(U, b.ghost, g)
+    inner = np.s_[1:-1, 1:-1]
+    np.testing.assert_allclose(gx[inner][..., 0], 0.3, rtol=1e-12)
+    np.testing.assert_allclose(gy[inner][..., 0], -0.7, rtol=1e-12)
+    np.testing.assert_allclose(gy[inner][..., 2], 0.5, rtol=1e-12)
+    np.testing.assert_allclose(gx[inner][..., 3], 1.0, rtol=1e-12)
+    assert np.max(np.abs(gx[inner][..., 1])) < 1e-14
+
+
+def test_venkatakrishnan_closed_forms():
+    s = np.array([0.0, 1.0, 2.0, 1e12])
+    np.testing.assert_allclose(O.venkatakrishnan(s), [0.0, 0.75, 1.0, 1.0], rtol=1e-11)
+    # zero slope denominators default to s = 1, i.e. phi = 0.75; no upper clamp
+    assert O.compute_slope(np.array([1.0]), np.array([-1.0]), np.array([0.0]))[0] == 1.0
+    smax = 2 + 2 * math.sqrt(2)          # d(phi)/ds = 0: phi_max = 1.0938...
+    assert O.venkatakrishnan(np.array([smax]))[0] > 1.09
+    grid = np.linspace(0, 50, 20001)
+    assert np.max(O.venkatakrishnan(grid)) <= O.venkatakrishnan(np.array([smax]))[0] + 1e-15
+
+
+def test_limiter_on_one_dimensional_linear_field_is_three_quarters():
+    """x-linear field: E/W faces give s = 2 (phi = 1), N/S faces have zero reconstruction delta
+    (s = 1, phi = 0.75): the reference limits every interior cell of a 1-D ramp to 0.75."""
+    nx, ny = 10, 6
+    g = O.block_geometry(RECT, nx, ny)
+    U = np.stack([1.0 + 0.1 * g.xc, 1 + 0 * g.xc, 1 + 0 * g.xc, 5 + 0.2 * g.xc], axis=-1)
+    b = _block_with(U)
+    gx, gy = O.green_gauss(U, b.ghost, g)
+    hot = {d: O.high_order_term(gx, gy, g, d) for d in O.DIRS}
+    phi = O.limiter_phi(U, b.ghost, hot)
+    inner = np.s_[1:-1, 1:-1]
+    np.testing.assert_array_equal(phi[inner][..., 0], 0.75)
+    np.testing.assert_array_equal(phi[inner][..., 3], 0.75)
+
+
+# ------------------------------------------------------------------ fluxes
+def _prim(rng, n, umax=1.0):
+    return np.stack([rng.uniform(0.5, 3, n), rng.uniform(-umax, umax, n), rng.uniform(-umax, umax, n),
+                     rng.uniform(0.5, 3, n)], axis=1).reshape(1, n, 4)
+
+
+@pytest.mark.parametrize("fn", [O.flux_roe, O.flux_hlll])
+def test_flux_consistency(fn):
+    W = _prim(np.random.default_rng(0), 200, 2.0)
+    F = fn(W, W.copy())
+    np.testing.assert_allclose(F, O.prim_F(W), rtol=1e-13, atol=1e-13)
+
+
+def test_roe_eigen_layout_inverts():
+    """The COO blocks of Rc (with uncorrected eigenvalues) and Lp satisfy Lp * dW/dU * Rc =
+    diag(1, 1, 1/rho*, 1) at the Roe state: the reference never sets the (2,2) entry of Lp (it keeps
+    the 1.0 of np.ones, flux/Roe.py:222-251, flux/eigen_system.py:52-57), so its shear-wave strength
+    is dv instead of rho* dv.  Any other misplaced entry or sign breaks the identity."""
+    rng = np.random.default_rng(1)
+    F = 5
+    WL, WR = _prim(rng, F), _prim(rng, F)
+    Wr = O.roe_state(WL, WR)
+    a = O.prim_a(Wr)
+    u, v = Wr[0, :, 1], Wr[0, :, 2]
+    Ek = 0.5 * (u * u + v * v)
+    H = O.G_OVER_GM1 * Wr[0, :, 3] / Wr[0, :, 0] + Ek
+    Rc_i, Rc_j, Lp_i, Lp_j, _ = O._roe_coo(F)
+    ua = u * a[0]
+    Rc_data = np.ones(14 * F)
+    for k, val in ((0, H - ua), (1, v), (2, Ek), (3, u - a[0]), (4, v), (5, v), (7, u), (9, H + ua), (11, v),
+                   (12, u + a[0])):
+        Rc_data[k * F:(k + 1) * F] = val
+    ia = 1.0 / a[0]
+    r2a = 0.5 * Wr[0, :, 0] * ia
+    Lp_data = np.ones(7 * F)
+    for k, val in ((0, r2a), (3, 0.5 * ia * ia), (4, -r2a), (5, -ia * ia), (6, 0.5 * ia * ia)):
+        Lp_data[k * F:(k + 1) * F] = val
+    Rc = np.zeros((4 * F, 4 * F))
+    Lp = np.zeros((4 * F, 4 * F))
+    np.add.at(Rc, (Rc_i, Rc_j), Rc_data)
+    np.add.at(Lp, (Lp_i, Lp_j), Lp_data)
+    for f in range(F):
+        sl = slice(4 * f, 4 * f + 4)
+        rho = Wr[0, f, 0]
+        uu, vv = u[f], v[f]
+        # dW/dU for W = (rho, u, v, p), U = (rho, rho u, rho v, E)
+        M = np.array([[1, 0, 0, 0],
+                      [-uu / rho, 1 / rho, 0, 0],
+                      [-vv / rho, 0, 1 / rho, 0],
+                      [O.GM1 * Ek[f], -O.GM1 * uu, -O.GM1 * vv, O.GM1]])
+        np.testing.assert_allclose(Lp[sl, sl] @ M @ Rc[sl, sl], np.diag([1.0, 1.0, 1.0 / rho, 1.0]), atol=1e-12)
+
+
+def test_hlll_supersonic_branches_are_upwind():
+    W = _prim(np.random.default_rng(2), 50)
+    WL, WR = W.copy(), W[:, ::-1].copy()
+    WL[..., 1] = WR[..., 1] = 6.0
+    np.testing.assert_array_equal(O.flux_hlll(WL, WR), O.flux_hlll(WL, WL))
+    WL[..., 1] = WR[..., 1] = -6.0
+    np.testing.assert_array_equal(O.flux_hlll(WL, WR), O.flux_hlll(WR, WR))
+
+
+def test_wall_faces_carry_no_mass_or_energy():
+    """A state against its own reflection: zero mass and energy flux for Roe always, and for HLLL
+    while the wall-normal speed is below ~0.85 a.  Above that (flow leaving the wall), the Harten
+    fix applied to HLLL's signal speeds (flux/HLLL.py:33-37) makes S+ != -S- and the wall face
+    carries mass -- a property of the reference scheme, asserted here too."""
+    rng = np.random.default_rng(3)
+    W = _prim(rng, 40)
+    a = np.sqrt(O.GAMMA * W[..., 3] / W[..., 0])
+    W[..., 1] = a * rng.uniform(-0.5, 0.5, a.shape)
+    U = O.prim_to_cons(W)
+    Ur = U.copy()
+    O.reflect(Ur, np.zeros((1, 40)))
+    assert np.array_equal(Ur[..., 1], -U[..., 1]) and np.array_equal(Ur[..., 2], U[..., 2])
+    for fn in (O.flux_roe, O.flux_hlll):
+        F = fn(O.cons_to_prim(U), O.cons_to_prim(Ur))
+        assert np.max(np.abs(F[..., 0])) < 1e-14 and np.max(np.abs(F[..., 3])) < 1e-13
+    W[..., 1] = -0.95 * a                      # strong outflow from the wall
+    U = O.prim_to_cons(W)
+    Ur = U.copy()
+    O.reflect(Ur, np.zeros((1, 40)))
+    assert np.max(np.abs(O.flux_roe(O.cons_to_prim(U), O.cons_to_prim(Ur))[..., 0])) < 1e-14
+    assert np.min(np.abs(O.flux_hlll(O.cons_to_prim(U), O.cons_to_prim(Ur))[..., 0])) > 1e-6
+
+
+# ------------------------------------------------------------------ residual and time stepping
+@pytest.mark.parametrize("bc", ["Reflection", "OutletDirichlet"])
+def test_rest_state_is_steady(bc):
+    sim = O.Simulation(mesh1(bc), 9, 7, CFL=0.8)
+    U = O.prim_to_cons(np.tile(np.array([1.3, 0.0, 0.0, 2.0]), (7, 9, 1)))
+    sim.set_state({0: U})
+    assert np.max(np.abs(O.residual(sim.blocks[0], sim.flux_fn))) == 0.0
+
+
+def test_uniform_flow_with_outlets_is_steady():
+    sim = O.Simulation(mesh1("OutletDirichlet"), 9, 7, CFL=0.8)
+    U = O.prim_to_cons(np.tile(np.array([1.3, 0.4, -0.2, 2.0]), (7, 9, 1)))
+    sim.set_state({0: U})
+    assert np.max(np.abs(O.residual(sim.blocks[0], sim.flux_fn))) == 0.0
+
+
+@pytest.mark.parametrize("flux", ["Roe", "HLLL"])
+def test_single_block_walls_conserve_mass_and_energy(flux):
+    sim = O.Simulation(mesh1("Reflection"), 16, 12, flux=flux, CFL=0.5, a_inf=343.0)
+    U = O.explosion_state(sim.blocks[0].geom, box=(2.0, 6.0, 1.0, 3.0))
+    sim.set_state({0: U})
+    A = sim.blocks[0].geom.A[..., None]
+    m0 = (U * A).sum(axis=(0, 1))
+    sim.run(6)
+    m1 = (sim.states()[0] * A).sum(axis=(0, 1))
+    assert abs(m1[0] - m0[0]) / m0[0] < 1e-13
+    assert abs(m1[3] - m0[3]) / m0[3] < 1e-13
+
+
+def test_residual_mirror_equivariance_on_smooth_state():
+    """R(mirror U) = mirror R(U) for a smooth generic state (x- and y-mirrors and the transpose).
+    On piecewise-constant data the reference arithmetic is not mirror-exact: the cos(pi/2) leakage
+    of the N/S normals rounds mirrored cells differently by an ulp, and the limiter's d > 0 / d == 0
+    branches turn that into O(1) changes of phi -- so the check uses data without exact ties."""
+    n = 16
+    vert = {"SW": [0.0, 0.0], "SE": [10.0, 0.0], "NW": [0.0, 10.0], "NE": [10.0, 10.0]}
+    sim = O.Simulation(mesh1("Reflection", vert), n, n, CFL=0.8)
+    g = sim.blocks[0].geom
+    x, y = g.xc, g.yc
+    bump = lambda cx, cy: np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / 4.0)
+    rho = 1.0 + 0.3 * (bump(5, 5) + 0.2 * bump(2.5, 5) + 0.2 * bump(7.5, 5) + 0.1 * bump(5, 2.5) + 0.1 * bump(5, 7.5))
+    p = 1.0 + 0.5 * rho ** 2
+    W = np.stack([rho, 0 * x, 0 * x, p], axis=-1)
+    U = O.prim_to_cons(W)
+    sim.set_state({0: U})
+    R = O.residual(sim.blocks[0], sim.flux_fn)
+    sc = np.abs(R).max(axis=(0, 1)) + 1e-300
+    Mx = lambda A: A[:, ::-1] * np.array([1, -1, 1, 1])
+    sim.set_state({0: Mx(U)})
+    Rm = O.residual(sim.blocks[0], sim.flux_fn)
+    assert np.max(np.abs(Rm - Mx(R)) / sc) < 1e-10
+    T = lambda A: A.transpose(1, 0, 2)[..., [0, 2, 1, 3]]
+    sim.set_state({0: T(U)})
+    Rt = O.residual(sim.blocks[0], sim.flux_fn)
+    assert np.max(np.abs(Rt - T(R)) / sc) < 1e-10
+
+
+@pytest.mark.parametrize("name,poly", [
+    ("RK4", lambda z: 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24),
+    ("SSPRK2", lambda z: 1 + z + z ** 2 / 2),
+    ("RK2", lambda z: 1 + z + z ** 2 / 2),
+    ("RK3SSP", lambda z: 1 + z + z ** 2 / 2 + z ** 3 / 6),
+])
+def test_rk_stability_polynomial(monkeypatch, name, poly):
+    lam = -0.37
+    monkeypatch.setattr(O, "residual", lambda b, f: lam * b.U)
+    monkeypatch.setattr(O, "apply_boundary_condition", lambda blocks: None)
+    g = O.block_geometry(RECT, 3, 2)
+    b = O.Block(0, g, {}, {}, U=np.full((2, 3, 4), 2.0))
+    dt = 0.9
+    O.integrate({0: b}, dt, O.BUTCHER[name], None)
+    np.testing.assert_allclose(b.U, 2.0 * poly(lam * dt), rtol=1e-14)
+
+
+def test_dt_closed_form_on_uniform_state():
+    sim = O.Simulation(mesh1(), 20, 10, CFL=0.8)
+    U = O.prim_to_cons(np.tile(np.array([1.4, 0.3, -0.1, 1.0]), (10, 20, 1)))
+    sim.set_state({0: U})
+    a = math.sqrt(1.4 * 1.0 / 1.4)
+    np.testing.assert_allclose(sim.get_dt(), 0.8 * min(0.5 / (0.3 + a), 0.5 / (0.1 + a)), rtol=1e-14)
+
+
+def test_ghost_fill_neighbours_and_walls():
+    m = O.rectangular_mesh_dict(["Reflection"], ["OutletDirichlet"], ["Reflection"], ["Reflection"],
+                                4.0, 0.0, 2.0, 0.0, 2, 1)
+    sim = O.Simulation(m, 5, 4)
+    rng = np.random.default_rng(9)
+    st = {n: O.prim_to_cons(_prim(rng, 20).reshape(4, 5, 4)) for n in m}
+    sim.set_state(st)
+    b0, b1 = sim.blocks[0], sim.blocks[1]
+    np.testing.assert_array_equal(b0.ghost["E"], st[1][:, :1])     # neighbour's first column
+    np.testing.assert_array_equal(b1.ghost["W"], st[0][:, -1:])    # neighbour's last column
+    np.testing.assert_array_equal(b0.ghost["W"], st[0][:, :1])     # outlet: own column
+    g = b1.ghost["E"]
+    assert np.array_equal(g[..., 1], -st[1][:, -1:, 1]) and np.array_equal(g[..., 3], st[1][:, -1:, 3])
