@@ -117,6 +117,47 @@ def cpu_baseline(name: str, seconds: float):
             "sample": f"NumPy oracle, {sample}, {steps} step(s) from the initial state, {el:.1f} s"}
 
 
+def adjoint_ratio(n: int = 256, steps: int = 100, every: int = 10, reps: int = 3):
+    """Adjoint/forward time ratio on the discrete-adjoint workload (256x256 smoothed explosion,
+    Roe, SSP-RK2, CFL 0.4, 100 frozen-dt steps, checkpoints every 10): T(full reverse sweep,
+    checkpoint recompute included) / T(forward of the same 100 steps), HIP events on the launch
+    stream, best of ``reps``."""
+    import torch
+
+    from pyhype_amd import Euler2D
+    from pyhype_amd import workloads as W
+    from pyhype_amd.adjoint import AdjointProblem
+
+    cfg, mesh, target, params = W.adjoint_problem(n, seed=0)
+    prob = AdjointProblem(Euler2D(cfg, mesh), target, nsteps=steps, checkpoint_every=every)
+    P = prob.plan
+    J, g = prob.gradient()  # warm-up (allocates the adjoint work arrays)
+    lam = torch.zeros_like(P.U)
+    lam.view(-1, 4)[:, 0] = 1.0
+    best_f = best_b = best_fc = float("inf")
+    for _ in range(reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        P.U.copy_(prob.U0)
+        torch.cuda.synchronize()
+        ev[0].record()
+        P.run_frozen(prob.dt, check=False)
+        ev[1].record()
+        ev[2].record()
+        prob.traj.forward(prob.U0)
+        ev[3].record()
+        ev[4].record()
+        prob.traj.backward(lam)
+        ev[5].record()
+        torch.cuda.synchronize()
+        best_f = min(best_f, ev[0].elapsed_time(ev[1]))
+        best_fc = min(best_fc, ev[2].elapsed_time(ev[3]))
+        best_b = min(best_b, ev[4].elapsed_time(ev[5]))
+    P.close()
+    return {"workload": f"adjoint_problem({n}), {steps} SSP-RK2 steps, checkpoints every {every}",
+            "forward_ms": round(best_f, 4), "forward_with_checkpoints_ms": round(best_fc, 4),
+            "adjoint_ms": round(best_b, 4), "ratio": round(best_b / best_f, 4), "J": J}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +171,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the launch shape before timing")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward ratio measurement")
     ap.add_argument("--pmc-calib", action="store_true",
                     help="also run a 256 MiB device copy three times (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -219,6 +261,13 @@ def main():
         except Exception as exc:  # the baseline never blocks the GPU figure
             log(f"cpu baseline failed: {exc!r}")
     desc.update({"steps_timed": K, "graph": use_graph, "launch": plan.launch_info(), "launch_source": tuned})
+    adjoint = None
+    if not args.no_adjoint:
+        try:
+            adjoint = adjoint_ratio()
+        except Exception as exc:  # reported, never blocks the forward figure
+            log(f"adjoint measurement failed: {exc!r}")
+            adjoint = {"error": repr(exc)}
     if args.pmc_calib:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -231,6 +280,7 @@ def main():
         "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic", "config": desc, "roofline": roofline, "cpu_baseline": cpu,
         "hbm_frac_of_value": value * (b_cell_step / S) / (HBM_PEAK_GBS * 1e9), "event_ms": ev_ms,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

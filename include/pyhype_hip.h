@@ -130,6 +130,13 @@ int phx_run(phx_plan* plan, double* U, int32_t nsteps, int32_t use_graph, void* 
  * nsteps x ncases); used by the discrete adjoint.  Time is advanced as in phx_run. */
 int phx_run_frozen(phx_plan* plan, double* U, int32_t nsteps, const double* dt_sched, void* stream);
 
+/* phx_run_frozen that also records the input state of every stage of every step: stage s of step n
+ * goes to stages + (n*nstages + s)*state_len (device, nsteps x nstages x state_len doubles).  The
+ * recorded states are the linearisation points of phx_step_vjp.  No counterpart in the reference
+ * (the discrete adjoint is new). */
+int phx_run_record(phx_plan* plan, double* U, int32_t nsteps, const double* dt_sched, double* stages,
+                   void* stream);
+
 /* One stage (0-based) of the plan's Runge-Kutta program on state U and the plan's stage buffers,
  * using the dt of the current step: the body of the stage loop of ExplicitRungeKutta.integrate
  * (time_marching/explicit_runge_kutta.py:63-80) for all blocks at once.  Used to time the stage
@@ -156,8 +163,17 @@ int phx_ghosts(phx_plan* plan, const double* U, double* G, void* stream);
 int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host, double* dt_host);
 
 /* Reverse-mode product of the residual: lamU = (dR/dU)^T lamR at state U (discrete adjoint of
- * fvm/base.py:108-190 with the active limiter/flux branches).  No counterpart in the reference. */
+ * fvm/base.py:108-190 with the active limiter/flux branches; deterministic, no atomics).  Requires
+ * mutual neighbour links.  The first adjoint call allocates the plan's adjoint work arrays
+ * (about 60 doubles per cell).  No counterpart in the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
+
+/* Reverse-mode product of one Runge-Kutta step with a frozen time step: on entry lam holds the
+ * adjoint of the state after the step, on return the adjoint of the state before it.  stages:
+ * the nstages stage-input states of that step as written by phx_run_record (device,
+ * nstages x state_len); dt_dev: that step's dt per case (device, ncases).  No counterpart in the
+ * reference. */
+int phx_step_vjp(phx_plan* plan, const double* stages, const double* dt_dev, double* lam, void* stream);
 
 /* Tunables of the stage launch: rows per work item, persistent grid size in waves (0 = one
  * workgroup per item) and the register budget variant (waves per SIMD, 2 or 3; 0 = 2); rebuilds

@@ -19,8 +19,8 @@ GEOM_GENERAL, GEOM_CARTESIAN = 0, 1
 
 EXPORTS = (
     "phx_plan_create", "phx_plan_destroy", "phx_state_len", "phx_reset", "phx_run", "phx_run_frozen",
-    "phx_stage", "phx_residual", "phx_dt", "phx_ghosts", "phx_status", "phx_residual_vjp", "phx_set_launch",
-    "phx_launch_info", "phx_last_error",
+    "phx_run_record", "phx_stage", "phx_residual", "phx_dt", "phx_ghosts", "phx_status", "phx_residual_vjp",
+    "phx_step_vjp", "phx_set_launch", "phx_launch_info", "phx_last_error",
 )
 
 
@@ -65,12 +65,14 @@ def load(path: Path = LIB_PATH):
         "phx_reset": ([vp, dp, i32, vp], C.c_int),
         "phx_run": ([vp, vp, i32, i32, vp], C.c_int),
         "phx_run_frozen": ([vp, vp, i32, vp, vp], C.c_int),
+        "phx_run_record": ([vp, vp, i32, vp, vp, vp], C.c_int),
         "phx_stage": ([vp, vp, i32, vp], C.c_int),
         "phx_residual": ([vp, vp, vp, vp], C.c_int),
         "phx_dt": ([vp, vp, vp, vp], C.c_int),
         "phx_ghosts": ([vp, vp, vp, vp], C.c_int),
         "phx_status": ([vp, C.POINTER(i32), C.POINTER(i32), dp, dp], C.c_int),
         "phx_residual_vjp": ([vp, vp, vp, vp, vp], C.c_int),
+        "phx_step_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_set_launch": ([vp, i32, i32, i32], C.c_int),
         "phx_launch_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "phx_last_error": ([], C.c_char_p),

@@ -1,0 +1,147 @@
+"""Discrete adjoint of the explicit Runge-Kutta trajectory, behind ``torch.autograd.Function``.
+
+The reference has no adjoint.  This module differentiates the exact discrete map of the backend:
+``n`` Runge-Kutta steps with a frozen time-step schedule (the schedule of a plain forward run, so
+that dt does not depend on the state being differentiated), U_0 -> U_n.  The reverse sweep uses
+binomial-free uniform checkpointing: the forward pass keeps U every ``checkpoint_every`` steps; the
+reverse pass re-runs each segment from its checkpoint while recording the stage inputs
+(``phx_run_record``), then applies the step VJP of every step of the segment in reverse order
+(``phx_step_vjp``).  All arithmetic is in the HIP library; torch holds the buffers and chains the
+result into user objectives.
+
+    traj = Trajectory(plan, dt_sched, checkpoint_every=10)
+    U_T = trajectory(U0, traj)                # differentiable in U0
+    J = ((rho(U_T) - rho_target) ** 2).sum(); J.backward()
+"""
+from __future__ import annotations
+
+from typing import Optional
+
+import numpy as np
+
+
+def frozen_schedule(plan, nsteps: int):
+    """Advance ``plan`` by ``nsteps`` CFL steps from its current state and return the dt of every
+    step as a device tensor (nsteps, ncases) (one host synchronisation per step)."""
+    torch = plan.torch
+    dts = np.zeros((nsteps, len(plan.cases)))
+    for s in range(nsteps):
+        plan.run(1, use_graph=False, check=True)
+        dts[s] = plan.status()[3]
+    return torch.as_tensor(dts, dtype=torch.float64, device="cuda")
+
+
+class Trajectory:
+    """``len(dt_sched)`` frozen-dt steps of ``plan`` as a map of the state, with checkpoints."""
+
+    def __init__(self, plan, dt_sched, checkpoint_every: int = 10):
+        if checkpoint_every < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        self.plan = plan
+        self.dt = dt_sched.contiguous()
+        self.nsteps = int(self.dt.shape[0])
+        self.every = int(checkpoint_every)
+        self.checkpoints = []
+        torch = plan.torch
+        m = min(self.every, max(1, self.nsteps))
+        self.stages = torch.empty((m, plan.nstages, plan.state_len), dtype=torch.float64, device="cuda")
+
+    def forward(self, U0):
+        """U_n from U_0 (device tensor of the state layout); keeps the checkpoints."""
+        P = self.plan
+        P.U.copy_(U0)
+        self.checkpoints = []
+        for s0 in range(0, self.nsteps, self.every):
+            self.checkpoints.append(P.U.clone())
+            P.run_frozen(self.dt[s0:s0 + self.every], check=False)
+        P.check_realizable()
+        return P.U.clone()
+
+    def backward(self, lam_T):
+        """Adjoint of U_0 from the adjoint of U_n (the checkpoints of the last forward)."""
+        if len(self.checkpoints) != (self.nsteps + self.every - 1) // self.every:
+            raise RuntimeError("Trajectory.backward needs a preceding forward")
+        P = self.plan
+        lam = lam_T.detach().to(dtype=P.torch.float64).contiguous().clone()
+        for seg in reversed(range(len(self.checkpoints))):
+            s0 = seg * self.every
+            m = min(self.every, self.nsteps - s0)
+            P.U.copy_(self.checkpoints[seg])
+            P.run_record(self.dt[s0:s0 + m], self.stages[:m])
+            for i in reversed(range(m)):
+                P.step_vjp(self.stages[i], self.dt[s0 + i], lam)
+        return lam
+
+
+_FN = None
+
+
+def _function():
+    global _FN
+    if _FN is None:
+        import torch
+
+        class TrajectoryFunction(torch.autograd.Function):
+            """U_0 -> U_n of a ``Trajectory``; backward is the discrete adjoint."""
+
+            @staticmethod
+            def forward(ctx, U0, traj):
+                ctx.traj = traj
+                return traj.forward(U0.detach())
+
+            @staticmethod
+            def backward(ctx, grad_out):
+                return ctx.traj.backward(grad_out), None
+
+        _FN = TrajectoryFunction
+    return _FN
+
+
+def trajectory(U0, traj: Trajectory):
+    """Differentiable U_n = traj(U_0)."""
+    return _function().apply(U0, traj)
+
+
+def density(plan, U):
+    """Density of every cell of every block, a view of U (torch) in state order."""
+    return U.view(-1, 4)[:, 0]
+
+
+class AdjointProblem:
+    """J(U_0) = sum over cells of (rho(U_n) - rho_target(U_n))^2 on one solver's plan, with the
+    target trajectory computed once from a target initial state."""
+
+    def __init__(self, solver, target_ic, nsteps: int = 100, checkpoint_every: int = 10):
+        self.solver = solver
+        self.plan = solver.plan
+        P = self.plan
+        torch = P.torch
+        solver.apply_initial_condition()
+        self.U0 = P.U.clone()
+        self.dt = frozen_schedule(P, nsteps)
+        self.traj = Trajectory(P, self.dt, checkpoint_every)
+        # target: the same frozen schedule from the target initial state
+        solver.apply_initial_condition(target_ic)
+        self.U0_target = P.U.clone()
+        self.traj.forward(self.U0_target)
+        self.rho_target = density(P, P.U).clone()
+        P.U.copy_(self.U0)
+        self.torch = torch
+
+    def objective(self, U0):
+        UT = trajectory(U0, self.traj)
+        r = density(self.plan, UT) - self.rho_target
+        return (r * r).sum()
+
+    def value(self, U0) -> float:
+        """J without recording a graph (forward only)."""
+        UT = self.traj.forward(U0)
+        r = density(self.plan, UT) - self.rho_target
+        return float((r * r).sum())
+
+    def gradient(self, U0: Optional[object] = None):
+        """(J, dJ/dU_0) at U0 (default: the base initial state)."""
+        x = (self.U0 if U0 is None else U0).clone().requires_grad_(True)
+        J = self.objective(x)
+        J.backward()
+        return float(J.detach()), x.grad.detach()

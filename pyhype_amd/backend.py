@@ -91,6 +91,61 @@ def side_kind(bc, neighbor) -> int:
     raise ValueError("Boundary Condition type " + str(bc) + " has not been specialized.")
 
 
+class Layout:
+    """Blocks of every case in state order, with their packed geometry and C descriptors."""
+
+    def __init__(self, blocks, case_blocks, state_len, geoms, descs):
+        self.blocks, self.case_blocks, self.state_len = blocks, case_blocks, state_len
+        self.geoms, self.descs = geoms, descs
+
+
+def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layout:
+    """State layout and ``phx_block_desc`` array of a list of cases (host only, no GPU needed).
+    Blocks of a case are numbered in mesh-dictionary order; each block's state is its reference
+    ``(ny, nx, 4)`` array, blocks concatenated."""
+    blocks: List[BlockEntry] = []
+    case_blocks: List[Dict[int, BlockEntry]] = []
+    offset = 0
+    for c, spec in enumerate(cases):
+        index_of = {}
+        entries = {}
+        for n, d in spec.mesh.items():
+            g = BlockGeometry({k: d[k] for k in ("SW", "SE", "NW", "NE")}, spec.nx, spec.ny)
+            e = BlockEntry(c, n, len(blocks), g, offset, spec.nx, spec.ny, [], [], False)
+            offset += 4 * spec.nx * spec.ny
+            index_of[n] = e.index
+            blocks.append(e)
+            entries[n] = e
+        for n, d in spec.mesh.items():
+            e = entries[n]
+            for s in ("E", "W", "N", "S"):
+                bc, nb = d["BCType" + s], d["Neighbor" + s]
+                k = side_kind(bc, nb)
+                if k == nat.SIDE_NEIGHBOR and nb not in index_of:
+                    raise ValueError(f"block {n}: neighbour {nb} does not exist")
+                e.sides.append(k)
+                e.neighbors.append(index_of[nb] if k == nat.SIDE_NEIGHBOR else -1)
+        case_blocks.append(entries)
+    geoms = []
+    descs = (nat.BlockDesc * len(blocks))()
+    for e, dsc in zip(blocks, descs):
+        packed = None if force_general else e.geom.pack_cartesian()
+        e.cartesian = packed is not None
+        if packed is None:
+            packed = e.geom.pack_general()
+        packed = np.ascontiguousarray(packed, dtype=np.float64)
+        geoms.append(packed)
+        dsc.nx, dsc.ny, dsc.case_id = e.nx, e.ny, e.case
+        dsc.geom_kind = nat.GEOM_CARTESIAN if e.cartesian else nat.GEOM_GENERAL
+        for s in range(4):
+            dsc.side_type[s] = e.sides[s]
+            dsc.neighbor[s] = e.neighbors[s]
+        dsc.state_offset = e.offset
+        dsc.geom = packed.ctypes.data_as(C.POINTER(C.c_double))
+        dsc.geom_len = packed.size
+    return Layout(blocks, case_blocks, offset, geoms, descs)
+
+
 class DevicePlan:
     def __init__(self, cases: Sequence[CaseSpec], *, flux: str, table: List[List[float]], fluid, cfl: float,
                  rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False):
@@ -110,49 +165,10 @@ class DevicePlan:
                 raise ValueError("Butcher table rows must have stage+1 entries")
             a[s, : s + 1] = row
         self._a = np.ascontiguousarray(a)
-        # blocks
-        self.blocks: List[BlockEntry] = []
-        self.case_blocks: List[Dict[int, BlockEntry]] = []
-        offset = 0
-        for c, spec in enumerate(self.cases):
-            index_of = {}
-            entries = {}
-            for n, d in spec.mesh.items():
-                g = BlockGeometry({k: d[k] for k in ("SW", "SE", "NW", "NE")}, spec.nx, spec.ny)
-                e = BlockEntry(c, n, len(self.blocks), g, offset, spec.nx, spec.ny, [], [], False)
-                offset += 4 * spec.nx * spec.ny
-                index_of[n] = e.index
-                self.blocks.append(e)
-                entries[n] = e
-            for n, d in spec.mesh.items():
-                e = entries[n]
-                for s in ("E", "W", "N", "S"):
-                    bc, nb = d["BCType" + s], d["Neighbor" + s]
-                    k = side_kind(bc, nb)
-                    if k == nat.SIDE_NEIGHBOR and nb not in index_of:
-                        raise ValueError(f"block {n}: neighbour {nb} does not exist")
-                    e.sides.append(k)
-                    e.neighbors.append(index_of[nb] if k == nat.SIDE_NEIGHBOR else -1)
-            self.case_blocks.append(entries)
-        self.state_len = offset
-        # packed geometry
-        self._geoms = []
-        descs = (nat.BlockDesc * len(self.blocks))()
-        for e, dsc in zip(self.blocks, descs):
-            packed = None if force_general else e.geom.pack_cartesian()
-            e.cartesian = packed is not None
-            if packed is None:
-                packed = e.geom.pack_general()
-            packed = np.ascontiguousarray(packed, dtype=np.float64)
-            self._geoms.append(packed)
-            dsc.nx, dsc.ny, dsc.case_id = e.nx, e.ny, e.case
-            dsc.geom_kind = nat.GEOM_CARTESIAN if e.cartesian else nat.GEOM_GENERAL
-            for s in range(4):
-                dsc.side_type[s] = e.sides[s]
-                dsc.neighbor[s] = e.neighbors[s]
-            dsc.state_offset = e.offset
-            dsc.geom = packed.ctypes.data_as(C.POINTER(C.c_double))
-            dsc.geom_len = packed.size
+        self.nstages = S
+        lay = layout_cases(self.cases, force_general=force_general)
+        self.blocks, self.case_blocks, self.state_len = lay.blocks, lay.case_blocks, lay.state_len
+        self._geoms, descs = lay.geoms, lay.descs
         self._tfinal = np.array([c.t_final for c in self.cases], dtype=np.float64)
         g = fluid.gamma()
         pd = nat.PlanDesc()
@@ -304,10 +320,27 @@ class DevicePlan:
         return out
 
     def residual_vjp(self, U, lamR):
+        """(dR/dU)^T lamR at state U (device tensors of the state layout)."""
         lamU = self.torch.zeros_like(U)
         nat.check(self.lib.phx_residual_vjp(self.handle, self.ptr(U), self.ptr(lamR), self.ptr(lamU), self.stream()),
                   "phx_residual_vjp")
         return lamU
+
+    def run_record(self, dt_sched, stages):
+        """Frozen-dt steps on the current state that record every stage input into ``stages``
+        (device tensor (nsteps, nstages, state_len))."""
+        nsteps = int(dt_sched.shape[0])
+        if tuple(stages.shape) != (nsteps, self.nstages, self.state_len):
+            raise ValueError(f"stages must have shape {(nsteps, self.nstages, self.state_len)}")
+        nat.check(self.lib.phx_run_record(self.handle, self.ptr(self.U), nsteps, self.ptr(dt_sched),
+                                          self.ptr(stages), self.stream()), "phx_run_record")
+        self.nsteps_done += nsteps
+
+    def step_vjp(self, stages, dt, lam):
+        """In place: lam (adjoint of the state after one step) -> adjoint of the state before it.
+        stages: (nstages, state_len) stage inputs of that step; dt: (ncases,) its time steps."""
+        nat.check(self.lib.phx_step_vjp(self.handle, self.ptr(stages), self.ptr(dt), self.ptr(lam), self.stream()),
+                  "phx_step_vjp")
 
     def set_launch(self, rows_per_item: int, grid_waves: int = 0, waves_per_simd: int = 2):
         nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves), int(waves_per_simd)),

@@ -1,0 +1,264 @@
+// phx_adjoint.hip -- discrete adjoint of the MUSCL residual and of one explicit RK step
+// (vector-Jacobian products), gfx950 kernels and their C ABI (include/pyhype_hip.h).
+//
+// The reverse pass is deterministic: every adjoint value is produced by exactly one lane from a
+// fixed set of terms (no atomics).  Four element-wise passes per residual VJP, each one lane per
+// cell or face, over launch tiles of at most TILE elements of one block:
+//   k_adj_recon   limited face states of every cell at the linearisation point
+//   k_adj_face    seed of every face from kappa, flux VJP -> left/right state adjoints
+//   k_adj_cell    reconstruction VJP of every cell -> cell + 4 stencil-neighbour adjoints
+//   k_adj_gather  lambda_U of every cell from its own and its neighbours' slots, ghosts, neighbours
+// The forward trajectory is not stored by these kernels: the caller passes the stage states,
+// recorded by phx_run_record (frozen time steps) from a checkpoint.
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pyhype_hip.h"
+#include "phx_adjoint_ops.hpp"
+#include "phx_plan.hpp"
+
+using phx::Phys;
+using phx::Q4;
+using phx::dev::DevBlock;
+using phx::host::AdjBlock;
+using phx::host::AdjTile;
+using phx::host::fail;
+
+namespace {
+
+constexpr int TILE = 256;
+constexpr int MAX_TERMS = PHX_MAX_STAGES + 2;
+
+template <int FLUX>
+__global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                   const double* U, const double* rb, const double* kappa,
+                                                   double* fadj, Phys ph) {
+    const AdjTile t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const AdjBlock O = offs[t.block];
+    const long long f = (long long)t.start + threadIdx.x;
+    double* out = fadj + phx::adj::FA * (O.face + f);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::face_adj<true, FLUX>(blocks, B, U, rb, O, kappa, f, out, ph);
+    else phx::adj::face_adj<false, FLUX>(blocks, B, U, rb, O, kappa, f, out, ph);
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                    const double* U, double* rb) {
+    const AdjTile t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const AdjBlock O = offs[t.block];
+    const int e = t.start + threadIdx.x;
+    const int k = e / B.nx, j = e - k * B.nx;
+    double* out = rb + phx::adj::RB * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out);
+    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out);
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                   const double* U, const double* fadj, double* slot) {
+    const AdjTile t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const AdjBlock O = offs[t.block];
+    const int e = t.start + threadIdx.x;
+    const int k = e / B.nx, j = e - k * B.nx;
+    double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out);
+    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out);
+}
+
+__global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                     const double* slot, const double* fadj, double* lamU) {
+    const AdjTile t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const int e = t.start + threadIdx.x;
+    const int k = e / B.nx, j = e - k * B.nx;
+    const Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
+                       ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj)
+                       : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj);
+    phx::adj::st(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
+}
+
+// out = sum_m c_m * (dt of the cell's case if scaled_m) * in_m, over the cells of every block
+struct CombArgs {
+    const double* in[MAX_TERMS];
+    double c[MAX_TERMS];
+    int scaled[MAX_TERMS];
+    int n;
+    const double* dt;
+    double* out;
+};
+
+__global__ void __launch_bounds__(TILE) k_adj_comb(const DevBlock* blocks, const AdjTile* tiles, CombArgs A) {
+    const AdjTile t = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const double dt = A.dt ? A.dt[B.case_id] : 1.0;
+    const long long base = B.off + 4LL * (t.start + threadIdx.x);
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < MAX_TERMS; ++m) {
+        if (m >= A.n) break;
+        const double c = A.scaled[m] ? A.c[m] * dt : A.c[m];
+        const double* p = A.in[m] + base;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] += c * p[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) A.out[base + q] = r[q];
+}
+
+template <class T>
+int upload(const std::vector<T>& h, T** d) {
+    if (*d) { (void)hipFree(*d); *d = nullptr; }
+    if (h.empty()) return PHX_OK;
+    PHX_CK(hipMalloc(d, h.size() * sizeof(T)));
+    PHX_CK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PHX_OK;
+}
+
+const int OPP[4] = {PHX_W, PHX_E, PHX_S, PHX_N};
+
+// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual.
+int adj_setup(phx_plan* P) {
+    if (P->adj_ready) return PHX_OK;
+    for (int b = 0; b < P->nblocks; ++b) {
+        const phx_block_desc& d = P->desc[b];
+        for (int s = 0; s < 4; ++s) {
+            if (d.side_type[s] != PHX_SIDE_NEIGHBOR) continue;
+            const phx_block_desc& o = P->desc[d.neighbor[s]];
+            if (o.side_type[OPP[s]] != PHX_SIDE_NEIGHBOR || o.neighbor[OPP[s]] != b)
+                return fail(PHX_E_INVALID, "adjoint: block " + std::to_string(b) +
+                                               " has a neighbour link that is not mutual");
+        }
+    }
+    std::vector<AdjBlock> offs(P->nblocks);
+    long long faces = 0;
+    P->adj_cell_tiles.clear();
+    P->adj_face_tiles.clear();
+    for (int b = 0; b < P->nblocks; ++b) {
+        const phx_block_desc& d = P->desc[b];
+        const long long nc = (long long)d.nx * d.ny;
+        const long long nf = (long long)d.ny * (d.nx + 1) + (long long)(d.ny + 1) * d.nx;
+        offs[b].cell = d.state_offset / 4;
+        offs[b].face = faces;
+        faces += nf;
+        for (long long e = 0; e < nc; e += TILE)
+            P->adj_cell_tiles.push_back({b, (int)e, (int)std::min<long long>(TILE, nc - e)});
+        for (long long e = 0; e < nf; e += TILE)
+            P->adj_face_tiles.push_back({b, (int)e, (int)std::min<long long>(TILE, nf - e)});
+    }
+    P->adj_cells = P->state_len / 4;
+    P->adj_faces = faces;
+    int rc;
+    if ((rc = upload(offs, &P->dadj))) return rc;
+    if ((rc = upload(P->adj_cell_tiles, &P->dadj_cell_tiles))) return rc;
+    if ((rc = upload(P->adj_face_tiles, &P->dadj_face_tiles))) return rc;
+    const size_t L = (size_t)P->state_len;
+    PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_face, (size_t)faces * phx::adj::FA * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_slot, (size_t)P->adj_cells * phx::adj::SLOT * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_kappa, L * sizeof(double)));
+    for (int s = 0; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->adj_lam[s], L * sizeof(double)));
+    P->adj_ready = true;
+    return PHX_OK;
+}
+
+void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, double* lamU, hipStream_t st) {
+    const dim3 blk(TILE);
+    const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
+    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb);
+    if (P->flux == PHX_FLUX_ROE)
+        hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
+                           P->adj_face, P->ph);
+    else
+        hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
+                           P->adj_face, P->ph);
+    hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
+                       P->adj_slot);
+    hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
+                       lamU);
+}
+
+void launch_comb(phx_plan* P, const CombArgs& A, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_comb, dim3((unsigned)P->adj_cell_tiles.size()), dim3(TILE), 0, st, P->dblocks,
+                       P->dadj_cell_tiles, A);
+}
+
+}  // namespace
+
+void phx::host::adjoint_release(phx_plan* P) {
+    void* ptrs[] = {P->dadj, P->dadj_cell_tiles, P->dadj_face_tiles, P->adj_rb, P->adj_face, P->adj_slot, P->adj_kappa};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (int s = 0; s < PHX_MAX_STAGES; ++s)
+        if (P->adj_lam[s]) (void)hipFree(P->adj_lam[s]);
+    P->dadj = nullptr;
+    P->dadj_cell_tiles = P->dadj_face_tiles = nullptr;
+    P->adj_rb = P->adj_face = P->adj_slot = P->adj_kappa = nullptr;
+    for (int s = 0; s < PHX_MAX_STAGES; ++s) P->adj_lam[s] = nullptr;
+    P->adj_ready = false;
+}
+
+extern "C" {
+
+int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* lamU, void* stream) {
+    if (!P || !U || !lamR || !lamU) return fail(PHX_E_INVALID, "null argument");
+    int rc = adj_setup(P);
+    if (rc) return rc;
+    launch_residual_vjp(P, U, lamR, lamU, (hipStream_t)stream);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+// One explicit RK step U_S = U_0 + dt sum_m a[S-1][m] K_m, U_i = U_0 + dt sum_m a[i-1][m] K_m,
+// K_m = R(U_m), with dt frozen.  Reverse: kappa_i = dt sum_{l>i} a[l-1][i] lambda_l,
+// lambda_i = (dR/dU(U_i))^T kappa_i for i >= 1, and lambda_0 = lambda_S + sum_{l=1}^{S-1} lambda_l
+// + (dR/dU(U_0))^T kappa_0.
+int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double* lam, void* stream) {
+    if (!P || !stages || !dt_dev || !lam) return fail(PHX_E_INVALID, "null argument");
+    int rc = adj_setup(P);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int S = P->nstages;
+    const size_t L = (size_t)P->state_len;
+    const double* a = P->butcher.data();
+    auto lam_of = [&](int l) -> const double* { return l == S ? lam : P->adj_lam[l]; };
+    for (int i = S - 1; i >= 0; --i) {
+        CombArgs C;
+        memset(&C, 0, sizeof(C));
+        for (int l = i + 1; l <= S; ++l) {
+            const double c = a[(size_t)(l - 1) * S + i];
+            if (c == 0.0) continue;
+            C.in[C.n] = lam_of(l);
+            C.c[C.n] = c;
+            C.scaled[C.n] = 1;
+            ++C.n;
+        }
+        C.dt = dt_dev;
+        C.out = P->adj_kappa;
+        launch_comb(P, C, st);
+        launch_residual_vjp(P, stages + (size_t)i * L, P->adj_kappa, P->adj_lam[i], st);
+    }
+    CombArgs C;
+    memset(&C, 0, sizeof(C));
+    C.in[C.n] = lam;
+    C.c[C.n++] = 1.0;
+    for (int l = 0; l < S; ++l) {
+        C.in[C.n] = P->adj_lam[l];
+        C.c[C.n++] = 1.0;
+    }
+    C.out = lam;
+    launch_comb(P, C, st);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+}  // extern "C"

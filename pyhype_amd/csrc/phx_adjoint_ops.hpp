@@ -1,0 +1,351 @@
+// phx_adjoint_ops.hpp -- element operations of the residual and of its vector-Jacobian product.
+//
+// The residual of a block is evaluated element-wise in three passes: (1) the limited face states of
+// every cell, (2) the flux of every face from the two states next to it (block-edge faces take
+// the outer state of their boundary), (3) the flux balance of every cell.  The same face-state
+// selection drives the reverse pass, so that the adjoint is the transpose of exactly this
+// evaluation:
+//   recon_cell   U          -> face states E, W, N, S of a cell          (stored, 16 doubles)
+//   face_adj     kappa      -> adjoints of the left/right state of a face (8 doubles per face)
+//   cell_adj     face adj.  -> adjoints of the cell and of its 4 stencil neighbours (20 doubles)
+//   gather_adj   20-slots   -> lambda_U of a cell (own slot, neighbours' slots, ghost and
+//                              outer-state paths of block edges, neighbour blocks)
+// Every function is __host__ __device__; the device kernels run one element per lane and the
+// host build runs the same code in the CPU test harness.
+#pragma once
+#include "phx_adjoint_numerics.hpp"
+#include "phx_device.hpp"
+#include "phx_plan.hpp"
+
+namespace phx {
+namespace adj {
+
+using dev::DevBlock;
+using host::AdjBlock;
+
+constexpr int RB = 16;    // doubles per cell of the face-state array: E, W, N, S
+constexpr int FA = 8;     // doubles per face of the face-adjoint array: left, right
+constexpr int SLOT = 20;  // doubles per cell of the slot array: cell, W, E, S, N
+
+enum { RB_E = 0, RB_W = 4, RB_N = 8, RB_S = 12 };
+enum { SL_C = 0, SL_W = 4, SL_E = 8, SL_S = 12, SL_N = 16 };
+
+__host__ __device__ inline long long cell_index(const AdjBlock& O, const DevBlock& B, int k, int j) {
+    return O.cell + (long long)k * B.nx + j;
+}
+__host__ __device__ inline long long xface_index(const AdjBlock& O, const DevBlock& B, int k, int jf) {
+    return O.face + (long long)k * (B.nx + 1) + jf;
+}
+__host__ __device__ inline long long yface_index(const AdjBlock& O, const DevBlock& B, int kf, int j) {
+    return O.face + (long long)B.ny * (B.nx + 1) + (long long)kf * B.nx + j;
+}
+
+__host__ __device__ inline Q4 ld(const double* p) {
+    Q4 q;
+    q.v[0] = p[0]; q.v[1] = p[1]; q.v[2] = p[2]; q.v[3] = p[3];
+    return q;
+}
+__host__ __device__ inline void st(double* p, const Q4& q) {
+    p[0] = q.v[0]; p[1] = q.v[1]; p[2] = q.v[2]; p[3] = q.v[3];
+}
+__host__ __device__ inline void acc(Q4& a, const Q4& b) {
+    a.v[0] += b.v[0]; a.v[1] += b.v[1]; a.v[2] += b.v[2]; a.v[3] += b.v[3];
+}
+__host__ __device__ inline Q4 scale(const Q4& a, double c) {
+    Q4 r;
+    r.v[0] = a.v[0] * c; r.v[1] = a.v[1] * c; r.v[2] = a.v[2] * c; r.v[3] = a.v[3] * c;
+    return r;
+}
+__host__ __device__ inline bool is_zero(const Q4& a) {
+    return a.v[0] == 0.0 && a.v[1] == 0.0 && a.v[2] == 0.0 && a.v[3] == 0.0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass 1: limited face states of cell (k, j) (the reconstruction of the stage kernel)
+// ---------------------------------------------------------------------------------------------
+template <bool CART>
+__host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
+                                           double* rb) {
+    const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
+    const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
+    const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
+    const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
+    const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
+    const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    const Recon r = reconstruct(Uc, Uw, Ue, Us, Un, g);
+    st(rb + RB_E, r.E);
+    st(rb + RB_W, r.W);
+    st(rb + RB_N, r.N);
+    st(rb + RB_S, r.S);
+}
+
+// ---------------------------------------------------------------------------------------------
+// pass 2: the two states of a face.  Face f of block B: x-faces (k, jf), jf = 0..nx, first, then
+// y-faces (kf, j), kf = 0..ny.  Interior faces take the limited states of the cells on either
+// side; block-edge faces take the outer state of the boundary (dev::outer_state).
+// ---------------------------------------------------------------------------------------------
+struct FaceRef {
+    bool xdir;
+    int k, j;  // x-face: row k, face column j;  y-face: face row k, column j
+};
+
+__host__ __device__ inline FaceRef face_ref(const DevBlock& B, long long f) {
+    FaceRef r;
+    const long long nxf = (long long)B.ny * (B.nx + 1);
+    if (f < nxf) {
+        r.xdir = true;
+        r.k = (int)(f / (B.nx + 1));
+        r.j = (int)(f % (B.nx + 1));
+    } else {
+        f -= nxf;
+        r.xdir = false;
+        r.k = (int)(f / B.nx);
+        r.j = (int)(f % B.nx);
+    }
+    return r;
+}
+
+template <bool CART>
+__host__ __device__ inline void face_states(const DevBlock* blocks, const DevBlock& B, const double* U,
+                                            const double* rb, const AdjBlock& O, const FaceRef& F, Q4& L, Q4& R) {
+    const int nx = B.nx, ny = B.ny;
+    if (F.xdir) {
+        const int k = F.k, jf = F.j;
+        if (jf == 0) {
+            const Q4 own = ld(rb + RB * cell_index(O, B, k, 0) + RB_W);
+            L = dev::outer_state<CART>(B, PHX_W, own, dev::fetch<CART>(blocks, B, U, k, -1), k, 0);
+        } else {
+            L = ld(rb + RB * cell_index(O, B, k, jf - 1) + RB_E);
+        }
+        if (jf == nx) {
+            const Q4 own = ld(rb + RB * cell_index(O, B, k, nx - 1) + RB_E);
+            R = dev::outer_state<CART>(B, PHX_E, own, dev::fetch<CART>(blocks, B, U, k, nx), k, nx);
+        } else {
+            R = ld(rb + RB * cell_index(O, B, k, jf) + RB_W);
+        }
+    } else {
+        const int kf = F.k, j = F.j;
+        if (kf == 0) {
+            const Q4 own = ld(rb + RB * cell_index(O, B, 0, j) + RB_S);
+            L = dev::outer_state<CART>(B, PHX_S, own, dev::fetch<CART>(blocks, B, U, -1, j), 0, j);
+        } else {
+            L = ld(rb + RB * cell_index(O, B, kf - 1, j) + RB_N);
+        }
+        if (kf == ny) {
+            const Q4 own = ld(rb + RB * cell_index(O, B, ny - 1, j) + RB_N);
+            R = dev::outer_state<CART>(B, PHX_N, own, dev::fetch<CART>(blocks, B, U, ny, j), ny - 1, j);
+        } else {
+            R = ld(rb + RB * cell_index(O, B, kf, j) + RB_S);
+        }
+    }
+}
+
+// Face flux in the global frame (the forward of pass 2; used by the host residual of the tests).
+template <bool CART, int FLUX>
+__host__ __device__ inline Q4 face_flux_global(const DevBlock* blocks, const DevBlock& B, const double* U,
+                                               const double* rb, const AdjBlock& O, long long f, const Phys& ph) {
+    const FaceRef F = face_ref(B, f);
+    Q4 L, R;
+    face_states<CART>(blocks, B, U, rb, O, F, L, R);
+    return F.xdir ? dev::xflux<CART, FLUX>(B, L, R, F.k, F.j, ph) : dev::yflux<CART, FLUX>(B, L, R, F.k, F.j, ph);
+}
+
+// Residual of cell (k, j) from the global-frame face fluxes (fvm/base.py:141-190; same association
+// as the stage kernel).
+template <bool CART>
+__host__ __device__ inline Q4 cell_residual(const DevBlock& B, const AdjBlock& O, const double* flux, int k, int j) {
+    double LW, LE, LS, LN, A;
+    dev::cell_L<CART>(B, k, j, LW, LE, LS, LN, A);
+    const Q4 FW = ld(flux + 4 * xface_index(O, B, k, j));
+    const Q4 FE = ld(flux + 4 * xface_index(O, B, k, j + 1));
+    const Q4 FS = ld(flux + 4 * yface_index(O, B, k, j));
+    const Q4 FN = ld(flux + 4 * yface_index(O, B, k + 1, j));
+    Q4 R;
+    for (int c = 0; c < 4; ++c) {
+        const double part = (LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c])) + LS * (2.0 * FS.v[c]);
+        R.v[c] = (0.5 * (part - LN * (2.0 * FN.v[c]))) / A;
+    }
+    return R;
+}
+
+// Adjoint of face f: seed lambda_F = sum over the (up to two) cells of the face of
+// kappa_cell * dR_cell/dF, then the flux VJP in the face frame.  Writes left/right adjoints.
+template <bool CART, int FLUX>
+__host__ __device__ inline void face_adj(const DevBlock* blocks, const DevBlock& B, const double* U, const double* rb,
+                                         const AdjBlock& O, const double* kappa, long long f, double* out,
+                                         const Phys& ph) {
+    const FaceRef F = face_ref(B, f);
+    Q4 lam = q4zero();
+    double LW, LE, LS, LN, A;
+    if (F.xdir) {
+        if (F.j > 0) {  // east face of cell (k, jf-1): -LE/A
+            dev::cell_L<CART>(B, F.k, F.j - 1, LW, LE, LS, LN, A);
+            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j - 1)), -LE / A));
+        }
+        if (F.j < B.nx) {  // west face of cell (k, jf): +LW/A
+            dev::cell_L<CART>(B, F.k, F.j, LW, LE, LS, LN, A);
+            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j)), LW / A));
+        }
+    } else {
+        if (F.k > 0) {  // north face of cell (kf-1, j): -LN/A
+            dev::cell_L<CART>(B, F.k - 1, F.j, LW, LE, LS, LN, A);
+            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k - 1, F.j)), -LN / A));
+        }
+        if (F.k < B.ny) {  // south face of cell (kf, j): +LS/A
+            dev::cell_L<CART>(B, F.k, F.j, LW, LE, LS, LN, A);
+            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j)), LS / A));
+        }
+    }
+    Q4 bL = q4zero(), bR = q4zero();
+    if (!is_zero(lam)) {
+        Q4 L, R;
+        face_states<CART>(blocks, B, U, rb, O, F, L, R);
+        if (CART && F.xdir) {
+            face_flux_vjp<FLUX>(L, R, lam, ph, bL, bR);
+        } else if (CART) {
+            // F = unrotate90(flux(rotate90 L, rotate90 R)); transposes: rotate90^T = unrotate90
+            Q4 bl, br;
+            face_flux_vjp<FLUX>(rotate90(L), rotate90(R), rotate90(lam), ph, bl, br);
+            bL = unrotate90(bl);
+            bR = unrotate90(br);
+        } else {
+            const dev::GenG G{B.geom, B.nx, B.ny};
+            double c, s;
+            if (F.xdir) { c = G.V(dev::F_c, F.k, F.j); s = G.V(dev::F_s, F.k, F.j); }
+            else { c = G.H(dev::F_c, F.k, F.j); s = G.H(dev::F_s, F.k, F.j); }
+            // F = unrotate(flux(rotate L, rotate R)); rotate^T = unrotate
+            Q4 bl, br;
+            face_flux_vjp<FLUX>(rotate(L, c, s), rotate(R, c, s), rotate(lam, c, s), ph, bl, br);
+            bL = unrotate(bl, c, s);
+            bR = unrotate(br, c, s);
+        }
+    }
+    st(out, bL);
+    st(out + 4, bR);
+}
+
+// Adjoint of the boundary outer state that is a function of the block's own limited face state
+// (outlet: identity; wall: reflection, which is symmetric); zero for neighbour and open sides,
+// whose outer state is the ghost cell value.
+template <bool CART>
+__host__ __device__ inline Q4 outer_own_adj(const DevBlock& B, int side, const Q4& b, int k, int j) {
+    const int st_ = B.side[side];
+    if (st_ == PHX_SIDE_OUTLET) return b;
+    if (st_ == PHX_SIDE_WALL) {
+        double c, s;
+        dev::wall_cs<CART>(B, side, k, j, c, s);
+        return reflect(b, c, s);
+    }
+    return q4zero();
+}
+
+// pass 3 reverse: adjoints of the four face states of cell (k, j) -> reconstruction VJP.
+template <bool CART>
+__host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock& B, const double* U,
+                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot) {
+    const int nx = B.nx, ny = B.ny;
+    Recon bQ;
+    bQ.E = ld(fadj + FA * xface_index(O, B, k, j + 1));          // left state of the east face
+    bQ.W = ld(fadj + FA * xface_index(O, B, k, j) + 4);          // right state of the west face
+    bQ.N = ld(fadj + FA * yface_index(O, B, k + 1, j));          // left state of the north face
+    bQ.S = ld(fadj + FA * yface_index(O, B, k, j) + 4);          // right state of the south face
+    if (j == nx - 1) acc(bQ.E, outer_own_adj<CART>(B, PHX_E, ld(fadj + FA * xface_index(O, B, k, nx) + 4), k, nx));
+    if (j == 0) acc(bQ.W, outer_own_adj<CART>(B, PHX_W, ld(fadj + FA * xface_index(O, B, k, 0)), k, 0));
+    if (k == ny - 1) acc(bQ.N, outer_own_adj<CART>(B, PHX_N, ld(fadj + FA * yface_index(O, B, ny, j) + 4), ny - 1, j));
+    if (k == 0) acc(bQ.S, outer_own_adj<CART>(B, PHX_S, ld(fadj + FA * yface_index(O, B, 0, j)), 0, j));
+    if (is_zero(bQ.E) && is_zero(bQ.W) && is_zero(bQ.N) && is_zero(bQ.S)) {
+        for (int q = 0; q < SLOT; ++q) slot[q] = 0.0;
+        return;
+    }
+    const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
+    const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
+    const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
+    const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
+    const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
+    const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    const ReconBar r = reconstruct_vjp(Uc, Uw, Ue, Us, Un, g, bQ);
+    st(slot + SL_C, r.c);
+    st(slot + SL_W, r.w);
+    st(slot + SL_E, r.e);
+    st(slot + SL_S, r.s);
+    st(slot + SL_N, r.n);
+}
+
+// Adjoint of a ghost value of `side` that is a function of the block's own edge cell (wall:
+// reflection; open and outlet: copy).  Neighbour ghosts are gathered from the neighbour block.
+template <bool CART>
+__host__ __device__ inline Q4 ghost_own_adj(const DevBlock& B, int side, const Q4& b, int k, int j) {
+    const int st_ = B.side[side];
+    if (st_ == PHX_SIDE_WALL) {
+        double c, s;
+        dev::wall_cs<CART>(B, side, k, j, c, s);
+        return reflect(b, c, s);
+    }
+    if (st_ == PHX_SIDE_OPEN || st_ == PHX_SIDE_OUTLET) return b;
+    return q4zero();
+}
+
+// lambda_U of cell (k, j): every path by which U(k, j) enters the residual.
+template <bool CART>
+__host__ __device__ inline Q4 gather_adj(const DevBlock* blocks, const AdjBlock* offs, int b, int k, int j,
+                                         const double* slot, const double* fadj) {
+    const DevBlock& B = blocks[b];
+    const AdjBlock& O = offs[b];
+    const int nx = B.nx, ny = B.ny;
+    Q4 lam = ld(slot + SLOT * cell_index(O, B, k, j) + SL_C);
+    // stencil neighbour of cells of the same block
+    if (j > 0) acc(lam, ld(slot + SLOT * cell_index(O, B, k, j - 1) + SL_E));
+    if (j < nx - 1) acc(lam, ld(slot + SLOT * cell_index(O, B, k, j + 1) + SL_W));
+    if (k > 0) acc(lam, ld(slot + SLOT * cell_index(O, B, k - 1, j) + SL_N));
+    if (k < ny - 1) acc(lam, ld(slot + SLOT * cell_index(O, B, k + 1, j) + SL_S));
+    // west edge
+    if (j == 0) {
+        if (B.side[PHX_W] == PHX_SIDE_NEIGHBOR) {
+            // this cell is the east ghost of the west neighbour: its stencil and its east outer state
+            const int n = B.nbr[PHX_W];
+            const DevBlock& N = blocks[n];
+            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, k, N.nx - 1) + SL_E));
+            acc(lam, ld(fadj + FA * xface_index(offs[n], N, k, N.nx) + 4));
+        } else {
+            acc(lam, ghost_own_adj<CART>(B, PHX_W, ld(slot + SLOT * cell_index(O, B, k, 0) + SL_W), k, -1));
+            if (B.side[PHX_W] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * xface_index(O, B, k, 0)));
+        }
+    }
+    if (j == nx - 1) {
+        if (B.side[PHX_E] == PHX_SIDE_NEIGHBOR) {
+            const int n = B.nbr[PHX_E];
+            const DevBlock& N = blocks[n];
+            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, k, 0) + SL_W));
+            acc(lam, ld(fadj + FA * xface_index(offs[n], N, k, 0)));
+        } else {
+            acc(lam, ghost_own_adj<CART>(B, PHX_E, ld(slot + SLOT * cell_index(O, B, k, nx - 1) + SL_E), k, nx));
+            if (B.side[PHX_E] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * xface_index(O, B, k, nx) + 4));
+        }
+    }
+    if (k == 0) {
+        if (B.side[PHX_S] == PHX_SIDE_NEIGHBOR) {
+            const int n = B.nbr[PHX_S];
+            const DevBlock& N = blocks[n];
+            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, N.ny - 1, j) + SL_N));
+            acc(lam, ld(fadj + FA * yface_index(offs[n], N, N.ny, j) + 4));
+        } else {
+            acc(lam, ghost_own_adj<CART>(B, PHX_S, ld(slot + SLOT * cell_index(O, B, 0, j) + SL_S), -1, j));
+            if (B.side[PHX_S] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * yface_index(O, B, 0, j)));
+        }
+    }
+    if (k == ny - 1) {
+        if (B.side[PHX_N] == PHX_SIDE_NEIGHBOR) {
+            const int n = B.nbr[PHX_N];
+            const DevBlock& N = blocks[n];
+            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, 0, j) + SL_S));
+            acc(lam, ld(fadj + FA * yface_index(offs[n], N, 0, j)));
+        } else {
+            acc(lam, ghost_own_adj<CART>(B, PHX_N, ld(slot + SLOT * cell_index(O, B, ny - 1, j) + SL_N), ny, j));
+            if (B.side[PHX_N] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * yface_index(O, B, ny, j) + 4));
+        }
+    }
+    return lam;
+}
+
+}  // namespace adj
+}  // namespace phx

@@ -1,0 +1,216 @@
+// phx_device.hpp -- block descriptors, geometry accessors, ghost values, boundary-face states and
+// rotated face fluxes shared by the stage kernels and the adjoint kernels.
+//
+// Every function is __host__ __device__: the device kernels call them per lane, and the host build
+// of the same header runs them in the CPU test harness of the adjoint.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/pyhype_hip.h"
+#include "phx_numerics.hpp"
+
+namespace phx {
+namespace dev {
+
+using phx::Phys;
+using phx::Q4;
+
+// Device copy of one phx_block_desc.  `off` is the block's offset (in doubles) into the state;
+// `wall` holds the (cos, sin) pairs of the E, W, N, S boundary normals of a Cartesian block.
+struct DevBlock {
+    int nx, ny, case_id, kind;
+    int side[4];
+    int nbr[4];
+    long long off;
+    const double* geom;
+    double wall[8];
+};
+
+// ---------------------------------------------------------------------------------------------
+// memory helpers
+// ---------------------------------------------------------------------------------------------
+__host__ __device__ inline Q4 ld4(const double* p) {
+    const double2* v = reinterpret_cast<const double2*>(p);
+    const double2 a = v[0], b = v[1];
+    Q4 q;
+    q.v[0] = a.x; q.v[1] = a.y; q.v[2] = b.x; q.v[3] = b.y;
+    return q;
+}
+__host__ __device__ inline void st4(double* p, const Q4& q) {
+    double2* v = reinterpret_cast<double2*>(p);
+    v[0] = make_double2(q.v[0], q.v[1]);
+    v[1] = make_double2(q.v[2], q.v[3]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// geometry accessors (layouts documented in include/pyhype_hip.h)
+// ---------------------------------------------------------------------------------------------
+struct CartG {
+    const double* g;
+    int nx, ny;
+    __host__ __device__ double col(int q, int j) const { return g[8 + q * nx + j]; }
+    __host__ __device__ double row(int q, int k) const { return g[8 + 11 * nx + q * ny + k]; }
+    __host__ __device__ double A(int k, int j) const { return g[8 + 11 * nx + 11 * ny + (long)k * nx + j]; }
+};
+enum { C_xlN, C_xlS, C_ylN, C_ylS, C_LN, C_LS, C_dxE, C_dxW, C_dxN, C_dxS, C_dx };
+enum { R_xlE, R_xlW, R_ylE, R_ylW, R_LE, R_LW, R_dyE, R_dyW, R_dyN, R_dyS, R_dy };
+
+struct GenG {
+    const double* g;
+    int nx, ny;
+    __host__ __device__ long nV() const { return (long)ny * (nx + 1); }
+    __host__ __device__ long nH() const { return (long)(ny + 1) * nx; }
+    __host__ __device__ double V(int q, int k, int jf) const { return g[q * nV() + (long)k * (nx + 1) + jf]; }
+    __host__ __device__ double H(int q, int kf, int j) const { return g[7 * nV() + q * nH() + (long)kf * nx + j]; }
+    __host__ __device__ double C(int q, int k, int j) const {
+        return g[7 * nV() + 7 * nH() + q * (long)ny * nx + (long)k * nx + j];
+    }
+};
+enum { F_L, F_xl, F_yl, F_xq, F_yq, F_c, F_s };
+enum { CL_A, CL_xc, CL_yc, CL_dx, CL_dy };
+
+template <bool CART>
+__host__ __device__ inline void wall_cs(const DevBlock& B, int side, int k, int j, double& c, double& s) {
+    if (CART) {
+        c = B.wall[2 * side];
+        s = B.wall[2 * side + 1];
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        switch (side) {
+            case PHX_E: c = G.V(F_c, k, B.nx); s = G.V(F_s, k, B.nx); break;
+            case PHX_W: c = G.V(F_c, k, 0); s = G.V(F_s, k, 0); break;
+            case PHX_N: c = G.H(F_c, B.ny, j); s = G.H(F_s, B.ny, j); break;
+            default: c = G.H(F_c, 0, j); s = G.H(F_s, 0, j); break;
+        }
+    }
+}
+
+// Value of cell (k, j) of block B, where (k, j) may lie one cell outside the block on one side
+// (never a corner): Blocks.apply_boundary_condition / GhostBlock.send_boundary_data
+// (blocks/base.py:448-471, blocks/ghost.py:187-216) followed by the ghost's BC (ghost.py:243-278).
+template <bool CART>
+__host__ __device__ inline Q4 fetch(const DevBlock* __restrict__ blocks, const DevBlock& B, const double* __restrict__ U,
+                    int k, int j) {
+    const int nx = B.nx, ny = B.ny;
+    if (k >= 0 && k < ny && j >= 0 && j < nx) return ld4(U + B.off + 4 * ((long)k * nx + j));
+    int side, kk = k, jj = j;
+    if (j < 0) { side = PHX_W; jj = 0; }
+    else if (j >= nx) { side = PHX_E; jj = nx - 1; }
+    else if (k < 0) { side = PHX_S; kk = 0; }
+    else { side = PHX_N; kk = ny - 1; }
+    const int st = B.side[side];
+    if (st == PHX_SIDE_NEIGHBOR) {
+        const DevBlock& N = blocks[B.nbr[side]];
+        int nk = k, nj = j;
+        if (side == PHX_E) nj = 0;
+        else if (side == PHX_W) nj = N.nx - 1;
+        else if (side == PHX_N) nk = 0;
+        else nk = N.ny - 1;
+        return ld4(U + N.off + 4 * ((long)nk * N.nx + nj));
+    }
+    Q4 q = ld4(U + B.off + 4 * ((long)kk * nx + jj));
+    if (st == PHX_SIDE_WALL) {
+        double c, s;
+        wall_cs<CART>(B, side, k, j, c, s);
+        q = phx::reflect(q, c, s);
+    }
+    return q;
+}
+
+// Outer state of a boundary face: MUSCL._get_boundary_flux_states (fvm/base.py:305-325) and the
+// BC applied to it (fvm/base.py:352-362,421-431).  `own` is the block's limited face value,
+// `ghost` the ghost cell value (the ghost block's limited value with phi = 0, grad = 0).
+template <bool CART>
+__host__ __device__ inline Q4 outer_state(const DevBlock& B, int side, const Q4& own, const Q4& ghost,
+                                          int k, int j) {
+    const int st = B.side[side];
+    if (st == PHX_SIDE_NEIGHBOR || st == PHX_SIDE_OPEN) return ghost;
+    if (st == PHX_SIDE_OUTLET) return own;
+    double c, s;
+    wall_cs<CART>(B, side, k, j, c, s);
+    return phx::reflect(own, c, s);
+}
+
+template <bool CART, int FLUX>
+__host__ __device__ inline Q4 xflux(const DevBlock& B, const Q4& L, const Q4& R, int k, int jf, const Phys& ph) {
+    if (CART) return phx::face_flux<FLUX>(L, R, ph);
+    const GenG G{B.geom, B.nx, B.ny};
+    const double c = G.V(F_c, k, jf), s = G.V(F_s, k, jf);
+    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    return phx::unrotate(F, c, s);
+}
+
+template <bool CART, int FLUX>
+__host__ __device__ inline Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R, int kf, int j, const Phys& ph) {
+    if (CART) {
+        const Q4 F = phx::face_flux<FLUX>(phx::rotate90(L), phx::rotate90(R), ph);
+        return phx::unrotate90(F);
+    }
+    const GenG G{B.geom, B.nx, B.ny};
+    const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
+    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    return phx::unrotate(F, c, s);
+}
+
+template <bool CART>
+__host__ __device__ inline phx::CellGeo cell_geo(const DevBlock& B, int k, int j) {
+    phx::CellGeo g;
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        g.xlE = G.row(R_xlE, k); g.xlW = G.row(R_xlW, k);
+        g.ylE = G.row(R_ylE, k); g.ylW = G.row(R_ylW, k);
+        g.xlN = G.col(C_xlN, j); g.xlS = G.col(C_xlS, j);
+        g.ylN = G.col(C_ylN, j); g.ylS = G.col(C_ylS, j);
+        g.ainv = 1.0 / G.A(k, j);
+        g.dxE = G.col(C_dxE, j); g.dxW = G.col(C_dxW, j);
+        g.dxN = G.col(C_dxN, j); g.dxS = G.col(C_dxS, j);
+        g.dyE = G.row(R_dyE, k); g.dyW = G.row(R_dyW, k);
+        g.dyN = G.row(R_dyN, k); g.dyS = G.row(R_dyS, k);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        g.xlE = G.V(F_xl, k, j + 1); g.xlW = -G.V(F_xl, k, j);
+        g.ylE = G.V(F_yl, k, j + 1); g.ylW = -G.V(F_yl, k, j);
+        g.xlN = G.H(F_xl, k + 1, j); g.xlS = -G.H(F_xl, k, j);
+        g.ylN = G.H(F_yl, k + 1, j); g.ylS = -G.H(F_yl, k, j);
+        g.ainv = 1.0 / G.C(CL_A, k, j);
+        const double xc = G.C(CL_xc, k, j), yc = G.C(CL_yc, k, j);
+        g.dxE = G.V(F_xq, k, j + 1) - xc; g.dyE = G.V(F_yq, k, j + 1) - yc;
+        g.dxW = G.V(F_xq, k, j) - xc; g.dyW = G.V(F_yq, k, j) - yc;
+        g.dxN = G.H(F_xq, k + 1, j) - xc; g.dyN = G.H(F_yq, k + 1, j) - yc;
+        g.dxS = G.H(F_xq, k, j) - xc; g.dyS = G.H(F_yq, k, j) - yc;
+    }
+    return g;
+}
+
+// face lengths of cell (k, j): west, east, south, north, and its area
+template <bool CART>
+__host__ __device__ inline void cell_L(const DevBlock& B, int k, int j, double& LW, double& LE, double& LS,
+                                       double& LN, double& A) {
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        LW = G.row(R_LW, k); LE = G.row(R_LE, k);
+        LS = G.col(C_LS, j); LN = G.col(C_LN, j);
+        A = G.A(k, j);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        LW = G.V(F_L, k, j); LE = G.V(F_L, k, j + 1);
+        LS = G.H(F_L, k, j); LN = G.H(F_L, k + 1, j);
+        A = G.C(CL_A, k, j);
+    }
+}
+
+template <bool CART>
+__host__ __device__ inline void cell_dxdy(const DevBlock& B, int k, int j, double& dx, double& dy) {
+    if (CART) {
+        const CartG G{B.geom, B.nx, B.ny};
+        dx = G.col(C_dx, j);
+        dy = G.row(R_dy, k);
+    } else {
+        const GenG G{B.geom, B.nx, B.ny};
+        dx = G.C(CL_dx, k, j);
+        dy = G.C(CL_dy, k, j);
+    }
+}
+
+}  // namespace dev
+}  // namespace phx

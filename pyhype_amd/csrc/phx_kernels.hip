@@ -20,32 +20,21 @@
 #include <vector>
 
 #include "../../include/pyhype_hip.h"
+#include "phx_device.hpp"
 #include "phx_numerics.hpp"
+#include "phx_plan.hpp"
 
 using phx::Phys;
 using phx::Q4;
+using namespace phx::dev;
+using namespace phx::host;
 
 namespace {
 
 constexpr int WAVE = 64;
 constexpr int HALO = 2;
 constexpr int STRIP = WAVE - 2 * HALO;  // owned columns per wavefront
-constexpr int MAX_OUT = 8;
-constexpr int MAX_ACC = 8;
 constexpr unsigned long long DT_INF_BITS = 0x7FF0000000000000ull;  // +inf
-
-struct DevBlock {
-    int nx, ny, case_id, kind;
-    int side[4];
-    int nbr[4];
-    long long off;
-    const double* geom;
-    double wall[8];
-};
-
-struct WorkItem {
-    int block, j0, r0, r1;
-};
 
 struct StageArgs {
     const DevBlock* blocks;
@@ -66,21 +55,7 @@ struct StageArgs {
     Phys ph;
 };
 
-// ---------------------------------------------------------------------------------------------
-// memory helpers
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ Q4 ld4(const double* p) {
-    const double2* v = reinterpret_cast<const double2*>(p);
-    const double2 a = v[0], b = v[1];
-    Q4 q;
-    q.v[0] = a.x; q.v[1] = a.y; q.v[2] = b.x; q.v[3] = b.y;
-    return q;
-}
-__device__ __forceinline__ void st4(double* p, const Q4& q) {
-    double2* v = reinterpret_cast<double2*>(p);
-    v[0] = make_double2(q.v[0], q.v[1]);
-    v[1] = make_double2(q.v[2], q.v[3]);
-}
+// cross-lane moves of a 4-vector
 __device__ __forceinline__ Q4 shfl_up1(const Q4& q) {
     Q4 r;
 #pragma unroll
@@ -92,176 +67,6 @@ __device__ __forceinline__ Q4 shfl_down1(const Q4& q) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) r.v[k] = __shfl_down(q.v[k], 1, WAVE);
     return r;
-}
-
-// ---------------------------------------------------------------------------------------------
-// geometry accessors (layouts documented in include/pyhype_hip.h)
-// ---------------------------------------------------------------------------------------------
-struct CartG {
-    const double* g;
-    int nx, ny;
-    __device__ double col(int q, int j) const { return g[8 + q * nx + j]; }
-    __device__ double row(int q, int k) const { return g[8 + 11 * nx + q * ny + k]; }
-    __device__ double A(int k, int j) const { return g[8 + 11 * nx + 11 * ny + (long)k * nx + j]; }
-};
-enum { C_xlN, C_xlS, C_ylN, C_ylS, C_LN, C_LS, C_dxE, C_dxW, C_dxN, C_dxS, C_dx };
-enum { R_xlE, R_xlW, R_ylE, R_ylW, R_LE, R_LW, R_dyE, R_dyW, R_dyN, R_dyS, R_dy };
-
-struct GenG {
-    const double* g;
-    int nx, ny;
-    __device__ long nV() const { return (long)ny * (nx + 1); }
-    __device__ long nH() const { return (long)(ny + 1) * nx; }
-    __device__ double V(int q, int k, int jf) const { return g[q * nV() + (long)k * (nx + 1) + jf]; }
-    __device__ double H(int q, int kf, int j) const { return g[7 * nV() + q * nH() + (long)kf * nx + j]; }
-    __device__ double C(int q, int k, int j) const {
-        return g[7 * nV() + 7 * nH() + q * (long)ny * nx + (long)k * nx + j];
-    }
-};
-enum { F_L, F_xl, F_yl, F_xq, F_yq, F_c, F_s };
-enum { CL_A, CL_xc, CL_yc, CL_dx, CL_dy };
-
-template <bool CART>
-__device__ __forceinline__ void wall_cs(const DevBlock& B, int side, int k, int j, double& c, double& s) {
-    if (CART) {
-        c = B.wall[2 * side];
-        s = B.wall[2 * side + 1];
-    } else {
-        const GenG G{B.geom, B.nx, B.ny};
-        switch (side) {
-            case PHX_E: c = G.V(F_c, k, B.nx); s = G.V(F_s, k, B.nx); break;
-            case PHX_W: c = G.V(F_c, k, 0); s = G.V(F_s, k, 0); break;
-            case PHX_N: c = G.H(F_c, B.ny, j); s = G.H(F_s, B.ny, j); break;
-            default: c = G.H(F_c, 0, j); s = G.H(F_s, 0, j); break;
-        }
-    }
-}
-
-// Value of cell (k, j) of block B, where (k, j) may lie one cell outside the block on one side
-// (never a corner): Blocks.apply_boundary_condition / GhostBlock.send_boundary_data
-// (blocks/base.py:448-471, blocks/ghost.py:187-216) followed by the ghost's BC (ghost.py:243-278).
-template <bool CART>
-__device__ Q4 fetch(const DevBlock* __restrict__ blocks, const DevBlock& B, const double* __restrict__ U,
-                    int k, int j) {
-    const int nx = B.nx, ny = B.ny;
-    if (k >= 0 && k < ny && j >= 0 && j < nx) return ld4(U + B.off + 4 * ((long)k * nx + j));
-    int side, kk = k, jj = j;
-    if (j < 0) { side = PHX_W; jj = 0; }
-    else if (j >= nx) { side = PHX_E; jj = nx - 1; }
-    else if (k < 0) { side = PHX_S; kk = 0; }
-    else { side = PHX_N; kk = ny - 1; }
-    const int st = B.side[side];
-    if (st == PHX_SIDE_NEIGHBOR) {
-        const DevBlock& N = blocks[B.nbr[side]];
-        int nk = k, nj = j;
-        if (side == PHX_E) nj = 0;
-        else if (side == PHX_W) nj = N.nx - 1;
-        else if (side == PHX_N) nk = 0;
-        else nk = N.ny - 1;
-        return ld4(U + N.off + 4 * ((long)nk * N.nx + nj));
-    }
-    Q4 q = ld4(U + B.off + 4 * ((long)kk * nx + jj));
-    if (st == PHX_SIDE_WALL) {
-        double c, s;
-        wall_cs<CART>(B, side, k, j, c, s);
-        q = phx::reflect(q, c, s);
-    }
-    return q;
-}
-
-// Outer state of a boundary face: MUSCL._get_boundary_flux_states (fvm/base.py:305-325) and the
-// BC applied to it (fvm/base.py:352-362,421-431).  `own` is the block's limited face value,
-// `ghost` the ghost cell value (the ghost block's limited value with phi = 0, grad = 0).
-template <bool CART>
-__device__ __forceinline__ Q4 outer_state(const DevBlock& B, int side, const Q4& own, const Q4& ghost,
-                                          int k, int j) {
-    const int st = B.side[side];
-    if (st == PHX_SIDE_NEIGHBOR || st == PHX_SIDE_OPEN) return ghost;
-    if (st == PHX_SIDE_OUTLET) return own;
-    double c, s;
-    wall_cs<CART>(B, side, k, j, c, s);
-    return phx::reflect(own, c, s);
-}
-
-template <bool CART, int FLUX>
-__device__ __forceinline__ Q4 xflux(const DevBlock& B, const Q4& L, const Q4& R, int k, int jf, const Phys& ph) {
-    if (CART) return phx::face_flux<FLUX>(L, R, ph);
-    const GenG G{B.geom, B.nx, B.ny};
-    const double c = G.V(F_c, k, jf), s = G.V(F_s, k, jf);
-    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
-    return phx::unrotate(F, c, s);
-}
-
-template <bool CART, int FLUX>
-__device__ __forceinline__ Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R, int kf, int j, const Phys& ph) {
-    if (CART) {
-        const Q4 F = phx::face_flux<FLUX>(phx::rotate90(L), phx::rotate90(R), ph);
-        return phx::unrotate90(F);
-    }
-    const GenG G{B.geom, B.nx, B.ny};
-    const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
-    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
-    return phx::unrotate(F, c, s);
-}
-
-template <bool CART>
-__device__ __forceinline__ phx::CellGeo cell_geo(const DevBlock& B, int k, int j) {
-    phx::CellGeo g;
-    if (CART) {
-        const CartG G{B.geom, B.nx, B.ny};
-        g.xlE = G.row(R_xlE, k); g.xlW = G.row(R_xlW, k);
-        g.ylE = G.row(R_ylE, k); g.ylW = G.row(R_ylW, k);
-        g.xlN = G.col(C_xlN, j); g.xlS = G.col(C_xlS, j);
-        g.ylN = G.col(C_ylN, j); g.ylS = G.col(C_ylS, j);
-        g.ainv = 1.0 / G.A(k, j);
-        g.dxE = G.col(C_dxE, j); g.dxW = G.col(C_dxW, j);
-        g.dxN = G.col(C_dxN, j); g.dxS = G.col(C_dxS, j);
-        g.dyE = G.row(R_dyE, k); g.dyW = G.row(R_dyW, k);
-        g.dyN = G.row(R_dyN, k); g.dyS = G.row(R_dyS, k);
-    } else {
-        const GenG G{B.geom, B.nx, B.ny};
-        g.xlE = G.V(F_xl, k, j + 1); g.xlW = -G.V(F_xl, k, j);
-        g.ylE = G.V(F_yl, k, j + 1); g.ylW = -G.V(F_yl, k, j);
-        g.xlN = G.H(F_xl, k + 1, j); g.xlS = -G.H(F_xl, k, j);
-        g.ylN = G.H(F_yl, k + 1, j); g.ylS = -G.H(F_yl, k, j);
-        g.ainv = 1.0 / G.C(CL_A, k, j);
-        const double xc = G.C(CL_xc, k, j), yc = G.C(CL_yc, k, j);
-        g.dxE = G.V(F_xq, k, j + 1) - xc; g.dyE = G.V(F_yq, k, j + 1) - yc;
-        g.dxW = G.V(F_xq, k, j) - xc; g.dyW = G.V(F_yq, k, j) - yc;
-        g.dxN = G.H(F_xq, k + 1, j) - xc; g.dyN = G.H(F_yq, k + 1, j) - yc;
-        g.dxS = G.H(F_xq, k, j) - xc; g.dyS = G.H(F_yq, k, j) - yc;
-    }
-    return g;
-}
-
-// face lengths of cell (k, j): west, east, south, north, and its area
-template <bool CART>
-__device__ __forceinline__ void cell_L(const DevBlock& B, int k, int j, double& LW, double& LE, double& LS,
-                                       double& LN, double& A) {
-    if (CART) {
-        const CartG G{B.geom, B.nx, B.ny};
-        LW = G.row(R_LW, k); LE = G.row(R_LE, k);
-        LS = G.col(C_LS, j); LN = G.col(C_LN, j);
-        A = G.A(k, j);
-    } else {
-        const GenG G{B.geom, B.nx, B.ny};
-        LW = G.V(F_L, k, j); LE = G.V(F_L, k, j + 1);
-        LS = G.H(F_L, k, j); LN = G.H(F_L, k + 1, j);
-        A = G.C(CL_A, k, j);
-    }
-}
-
-template <bool CART>
-__device__ __forceinline__ void cell_dxdy(const DevBlock& B, int k, int j, double& dx, double& dy) {
-    if (CART) {
-        const CartG G{B.geom, B.nx, B.ny};
-        dx = G.col(C_dx, j);
-        dy = G.row(R_dy, k);
-    } else {
-        const GenG G{B.geom, B.nx, B.ny};
-        dx = G.C(CL_dx, k, j);
-        dy = G.C(CL_dy, k, j);
-    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -519,64 +324,13 @@ __global__ void k_ghosts(const DevBlock* blocks, int nblocks, const long long* g
 // host side
 // =============================================================================================
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const std::string& msg) {
+int phx::host::fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define PHX_CK(call)                                                                       \
-    do {                                                                                   \
-        hipError_t e_ = (call);                                                            \
-        if (e_ != hipSuccess)                                                              \
-            return fail(PHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
-    } while (0)
-
-enum BufId { BUF_USER = 0, BUF_S0 = 1, BUF_S1 = 2, BUF_ACC0 = 3 };
-
-struct StageOut {
-    int dst, src;
-    double a;
-};
-struct StageProg {
-    int in;
-    std::vector<StageOut> outs;
-    int state_out;
-};
-
-}  // namespace
-
-struct phx_plan {
-    int device = 0;
-    int ncu = 0;
-    int nblocks = 0, ncases = 0, flux = 0;
-    Phys ph{};
-    std::vector<phx_block_desc> desc;
-    std::vector<DevBlock> hblocks;
-    DevBlock* dblocks = nullptr;
-    double* dgeom = nullptr;
-    int64_t state_len = 0;
-    int rows_per_item = 32, grid_waves = 0, waves_per_simd = 2;
-    std::vector<WorkItem> items[2];  // [0] general, [1] cartesian
-    WorkItem* ditems[2] = {nullptr, nullptr};
-    int grid[2] = {0, 0};
-    std::vector<StageProg> progs;
-    int nacc = 0;
-    double* sbuf[2] = {nullptr, nullptr};
-    double* acc[MAX_ACC] = {};
-    double *dt = nullptr, *t = nullptr, *tfinal = nullptr;
-    unsigned long long* dtmin = nullptr;
-    int *step = nullptr, *bad = nullptr;
-    long long* dghost_off = nullptr;
-    hipStream_t cap_stream = nullptr;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
-    double* graph_U = nullptr;
-    bool copy_back = false;  // one-stage tables: result lands in sbuf[0] and is copied to U
-};
 
 namespace {
 
@@ -739,6 +493,7 @@ int64_t phx_state_len(const phx_plan* P) { return P ? P->state_len : -1; }
 int phx_plan_destroy(phx_plan* P) {
     if (!P) return PHX_OK;
     (void)hipDeviceSynchronize();
+    phx::host::adjoint_release(P);
     if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
@@ -853,6 +608,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipMalloc(&P->dghost_off, D->nblocks * sizeof(long long)));
     PHX_CKP(hipMemcpy(P->dghost_off, ghost_off.data(), D->nblocks * sizeof(long long), hipMemcpyHostToDevice));
 
+    P->nstages = D->nstages;
+    P->butcher.assign(D->butcher_a, D->butcher_a + (size_t)D->nstages * D->nstages);
     rc = build_programs(P, D->nstages, D->butcher_a);
     if (rc) return bail(rc);
     rc = build_items(P);
@@ -939,6 +696,26 @@ int phx_run_frozen(phx_plan* P, double* U, int32_t nsteps, const double* dt_sche
     return PHX_OK;
 }
 
+int phx_run_record(phx_plan* P, double* U, int32_t nsteps, const double* dt_sched, double* stages, void* stream) {
+    if (!P || !U || ((!dt_sched || !stages) && nsteps > 0)) return fail(PHX_E_INVALID, "null argument");
+    if (nsteps < 0) return fail(PHX_E_INVALID, "nsteps must be >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t len = (size_t)P->state_len;
+    const int S = (int)P->progs.size();
+    for (int n = 0; n < nsteps; ++n) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+                           P->dt, P->step, P->bad, dt_sched + (size_t)n * P->ncases, 1);
+        for (int s = 0; s < S; ++s) {
+            double* dst = stages + ((size_t)n * S + s) * len;
+            PHX_CK(hipMemcpyAsync(dst, buf_ptr(P, U, P->progs[s].in), len * sizeof(double), hipMemcpyDeviceToDevice, st));
+            launch_prog_stage(P, U, (size_t)s, st);
+        }
+        if (P->copy_back) PHX_CK(hipMemcpyAsync(U, P->sbuf[0], len * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
 int phx_stage(phx_plan* P, double* U, int32_t stage, void* stream) {
     if (!P || !U) return fail(PHX_E_INVALID, "null argument");
     if (stage < 0 || stage >= (int)P->progs.size()) return fail(PHX_E_INVALID, "stage index out of range");
@@ -989,12 +766,6 @@ int phx_status(phx_plan* P, int32_t* bad_step, int32_t* step, double* t_host, do
     if (t_host) PHX_CK(hipMemcpy(t_host, P->t, P->ncases * sizeof(double), hipMemcpyDeviceToHost));
     if (dt_host) PHX_CK(hipMemcpy(dt_host, P->dt, P->ncases * sizeof(double), hipMemcpyDeviceToHost));
     return PHX_OK;
-}
-
-int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* lamU, void* stream) {
-    (void)U; (void)lamR; (void)lamU; (void)stream;
-    if (!P) return fail(PHX_E_INVALID, "null argument");
-    return fail(PHX_E_STATE, "phx_residual_vjp is not available in this build");
 }
 
 int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd) {

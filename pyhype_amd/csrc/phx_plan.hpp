@@ -1,0 +1,109 @@
+// phx_plan.hpp -- host-side plan object shared by the stage and adjoint translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pyhype_hip.h"
+#include "phx_device.hpp"
+
+namespace phx {
+namespace host {
+
+constexpr int MAX_OUT = 8;  // outputs of one stage program (stage state + accumulators)
+constexpr int MAX_ACC = 8;  // accumulator buffers of the RK programs
+
+// One work item of the stage kernel: block, first owned column, row range [r0, r1).
+struct WorkItem {
+    int block, j0, r0, r1;
+};
+
+enum BufId { BUF_USER = 0, BUF_S0 = 1, BUF_S1 = 2, BUF_ACC0 = 3 };
+
+struct StageOut {
+    int dst, src;
+    double a;
+};
+struct StageProg {
+    int in;
+    std::vector<StageOut> outs;
+    int state_out;
+};
+
+// Records `msg` as the thread's last error (phx_last_error) and returns `code`.
+int fail(int code, const std::string& msg);
+
+// Per-block offsets of the adjoint work arrays: cells (in cells) and faces (in faces; the
+// ny*(nx+1) x-faces of the block first, then its (ny+1)*nx y-faces).
+struct AdjBlock {
+    long long cell, face;
+};
+
+// Launch tile of the element-wise adjoint kernels: `count` consecutive elements of one block.
+struct AdjTile {
+    int block, start, count;
+};
+
+}  // namespace host
+}  // namespace phx
+
+#define PHX_CK(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return ::phx::host::fail(PHX_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct phx_plan {
+    int device = 0;
+    int ncu = 0;
+    int nblocks = 0, ncases = 0, flux = 0;
+    phx::Phys ph{};
+    std::vector<phx_block_desc> desc;
+    std::vector<phx::dev::DevBlock> hblocks;
+    phx::dev::DevBlock* dblocks = nullptr;
+    double* dgeom = nullptr;
+    int64_t state_len = 0;
+    int rows_per_item = 32, grid_waves = 0, waves_per_simd = 2;
+    std::vector<phx::host::WorkItem> items[2];  // [0] general, [1] cartesian
+    phx::host::WorkItem* ditems[2] = {nullptr, nullptr};
+    int grid[2] = {0, 0};
+    std::vector<phx::host::StageProg> progs;
+    std::vector<double> butcher;  // nstages x nstages, row-major
+    int nstages = 0;
+    int nacc = 0;
+    double* sbuf[2] = {nullptr, nullptr};
+    double* acc[phx::host::MAX_ACC] = {};
+    double *dt = nullptr, *t = nullptr, *tfinal = nullptr;
+    unsigned long long* dtmin = nullptr;
+    int *step = nullptr, *bad = nullptr;
+    long long* dghost_off = nullptr;
+    hipStream_t cap_stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    double* graph_U = nullptr;
+    bool copy_back = false;  // one-stage tables: result lands in sbuf[0] and is copied to U
+
+    // discrete adjoint work arrays, allocated on the first adjoint call
+    bool adj_ready = false;
+    long long adj_cells = 0, adj_faces = 0;
+    phx::host::AdjBlock* dadj = nullptr;
+    std::vector<phx::host::AdjTile> adj_cell_tiles, adj_face_tiles;
+    phx::host::AdjTile *dadj_cell_tiles = nullptr, *dadj_face_tiles = nullptr;
+    double* adj_rb = nullptr;     // limited face states E, W, N, S of every cell (16 doubles)
+    double* adj_face = nullptr;   // adjoints of the left/right states of every face (8 doubles)
+    double* adj_slot = nullptr;   // reconstruction adjoints: cell, W, E, S, N (20 doubles per cell)
+    double* adj_kappa = nullptr;  // residual seed of one stage
+    double* adj_lam[PHX_MAX_STAGES] = {};  // stage-state adjoints of one step
+};
+
+namespace phx {
+namespace host {
+// Frees the adjoint work arrays of a plan (no-op when none were allocated).
+void adjoint_release(phx_plan* P);
+}  // namespace host
+}  // namespace phx

@@ -75,3 +75,62 @@ class ObliqueShockIC(InitialCondition):
         behind = block.mesh.x < self.x_shock + block.mesh.y / np.sqrt(3.0)
         block.state.data = np.where(behind, post.data, pre.data)
         block.state.make_non_dimensional()
+
+
+class SmoothExplosionIC(InitialCondition):
+    """The explosion of ``ExplosionInitialCondition`` with differentiable data: the square's edges
+    are tanh ramps of half-width ``width``; density and pressure carry a smooth background variation
+    of relative amplitude ``amplitude`` and the air a smooth background velocity of ``amplitude``
+    times 343 m/s (phases ``phases``), so that no two neighbouring cells hold exactly equal values
+    in any variable.  The piecewise-constant original is a point where the discrete scheme is not
+    differentiable (every cell of its flat regions, and the zero momentum everywhere, sits on the
+    limiter's d = 0 branch or on a max/min tie), so the discrete adjoint is evaluated at this
+    smoothed state."""
+
+    def __init__(self, width=0.1, amplitude=0.01, phases=(0.3, 1.1, 2.0, 2.9, 0.7, 1.9)):
+        self.width = float(width)
+        self.amplitude = float(amplitude)
+        phases = tuple(float(p) for p in phases)
+        self.phases = phases + (0.7, 1.9)[len(phases) - 4:] if len(phases) < 6 else phases
+
+    def _box(self, x, y):
+        w = self.width
+        sx = 0.25 * (1.0 + np.tanh((x - 3.0) / w)) * (1.0 + np.tanh((7.0 - x) / w))
+        sy = 0.25 * (1.0 + np.tanh((y - 3.0) / w)) * (1.0 + np.tanh((7.0 - y) / w))
+        return sx * sy
+
+    def primitive(self, x, y):
+        """(rho, u, v, p) in dimensional units at points x, y."""
+        b = self._box(x, y)
+        a, (p1, p2, p3, p4, p5, p6) = self.amplitude, self.phases
+        rho = (1.1742 + (4.6968 - 1.1742) * b) * (1.0 + a * np.sin(0.7 * x + p1) * np.cos(0.9 * y + p2))
+        p = (101100.0 + (404400.0 - 101100.0) * b) * (1.0 + a * np.cos(0.8 * x + p3) * np.sin(0.6 * y + p4))
+        u = a * 343.0 * np.sin(0.5 * x + p5) * np.cos(0.4 * y + p6)
+        v = a * 343.0 * np.cos(0.45 * x + p6) * np.sin(0.55 * y + p5)
+        return rho, u, v, p
+
+    def apply_to_block(self, block):
+        x, y = block.mesh.x[..., 0], block.mesh.y[..., 0]
+        rho, u, v, p = self.primitive(x, y)
+        W = np.stack([rho, u, v, p], axis=-1)
+        block.state.data = PrimitiveState(fluid=block.config.fluid, array=W).to_type(ConservativeState).data
+        block.state.make_non_dimensional()
+
+
+class DensityBumpIC(InitialCondition):
+    """``base`` with the density multiplied by 1 + amplitude*exp(-|x - c|^2 / (2 sigma^2))
+    (velocity and pressure unchanged)."""
+
+    def __init__(self, base, amplitude, x0, y0, sigma):
+        self.base, self.amplitude, self.center, self.sigma = base, float(amplitude), (float(x0), float(y0)), float(sigma)
+
+    def primitive(self, x, y):
+        rho, u, v, p = self.base.primitive(x, y)
+        r2 = (x - self.center[0]) ** 2 + (y - self.center[1]) ** 2
+        return rho * (1.0 + self.amplitude * np.exp(-r2 / (2.0 * self.sigma ** 2))), u, v, p
+
+    def apply_to_block(self, block):
+        x, y = block.mesh.x[..., 0], block.mesh.y[..., 0]
+        W = np.stack(self.primitive(x, y), axis=-1)
+        block.state.data = PrimitiveState(fluid=block.config.fluid, array=W).to_type(ConservativeState).data
+        block.state.make_non_dimensional()

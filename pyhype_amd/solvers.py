@@ -119,11 +119,13 @@ class Euler2DBatch:
         self.failed: Dict[int, int] = {}
 
     # ---------------------------------------------------------------- set-up
-    def apply_initial_condition(self):
-        """Euler2D.apply_initial_condition (solvers/Euler2D.py:90-92), then upload."""
+    def apply_initial_condition(self, initial_condition=None):
+        """Euler2D.apply_initial_condition (solvers/Euler2D.py:90-92), then upload.  An explicit
+        ``initial_condition`` replaces the configured one of every case."""
         for c, blocks in enumerate(self.case_blocks):
+            ic = self.configs[c].initial_condition if initial_condition is None else initial_condition
             for blk in blocks.values():
-                self.configs[c].initial_condition.apply_to_block(blk)
+                ic.apply_to_block(blk)
         self.upload()
 
     def upload(self):

@@ -12,7 +12,9 @@
   1.1742); Roe/RK4, CFL 0.8, 100 steps, per-case dt.  Scaling the density by the same ratio
   (equal temperature) makes most drawn cases unrealizable within a few steps in the reference
   scheme at CFL 0.8; ``density_scaled=True`` selects that variant.
-* ``explosion_adjoint``: 1 block 256x256 on [0,10]^2, explosion, Roe, SSP-RK2, CFL 0.4, 100 steps.
+* ``explosion_adjoint``: 1 block 256x256 on [0,10]^2, explosion, Roe, SSP-RK2, CFL 0.4, 100 steps;
+  ``adjoint_problem`` adds the differentiable base state and the density-bump target of the
+  discrete-adjoint objective J = sum (rho(T) - rho_target(T))^2.
 
 Every function accepts a scale-down (``n``, ``nb``...) so tests can run the same set-up small.
 """
@@ -23,7 +25,8 @@ from typing import List, Tuple
 import numpy as np
 
 from .fluids import Air
-from .initial_conditions import ExplosionInitialCondition, ObliqueShockIC, SquareExplosionIC
+from .initial_conditions import (DensityBumpIC, ExplosionInitialCondition, ObliqueShockIC, SmoothExplosionIC,
+                                 SquareExplosionIC)
 from .mesh import RectagularMeshGenerator
 from .solver_config import SolverConfig
 
@@ -92,6 +95,23 @@ def explosion_batch(seed: int = 0, ncases: int = 64, sizes=None,
 
 def explosion_adjoint(n: int = 256):
     return _config(n, n, ic=ExplosionInitialCondition(), integrator="SSPRK2", cfl=0.4), single_block_mesh()
+
+
+def adjoint_problem(n: int = 256, seed: int = 0, width: float = 0.1, amplitude: float = 0.01):
+    """Set-up of the discrete-adjoint workload on ``explosion_adjoint``'s mesh and scheme: the
+    linearisation point is the smoothed explosion (``SmoothExplosionIC``, background phases drawn
+    from ``seed``); the target is the same state with a Gaussian density bump of relative
+    amplitude 1e-2 and sigma 1 at a centre drawn from ``seed``.  Returns (config, mesh, target IC,
+    parameters)."""
+    rng = np.random.default_rng(seed)
+    phases = tuple(float(p) for p in rng.uniform(0.0, 2.0 * np.pi, 6))
+    cx, cy = (float(v) for v in rng.uniform(2.0, 8.0, 2))
+    base = SmoothExplosionIC(width=width, amplitude=amplitude, phases=phases)
+    target = DensityBumpIC(base, 1e-2, cx, cy, 1.0)
+    cfg, mesh = explosion_adjoint(n)
+    cfg.initial_condition = base
+    return cfg, mesh, target, {"seed": seed, "phases": phases, "bump_center": (cx, cy), "bump_sigma": 1.0,
+                               "bump_amplitude": 1e-2, "width": width, "amplitude": amplitude}
 
 
 WORKLOADS = {

@@ -32,17 +32,27 @@ def pytest_collection_modifyitems(config, items):
             it.add_marker(skip)
 
 
+def _build_shim(name, tmp_path_factory):
+    import ctypes
+
+    src = ROOT / "tests" / "native" / f"{name}.hip"
+    out = tmp_path_factory.mktemp("shim") / f"{name}.so"
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
+           "-Xclang", "-packed-fp32-ops", "-fPIC", "-shared", f"-I{ROOT / 'include'}", str(src), "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        pytest.fail(f"{name} build failed:\n" + res.stderr)
+    return ctypes.CDLL(str(out))
+
+
 @pytest.fixture(scope="session")
 def numerics_shim(tmp_path_factory):
     """Host build of the device numerics header (same source as the kernels)."""
-    import ctypes
+    return _build_shim("numerics_shim", tmp_path_factory)
 
-    src = ROOT / "tests" / "native" / "numerics_shim.hip"
-    out = tmp_path_factory.mktemp("shim") / "numerics_shim.so"
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
-           "-Xclang", "-packed-fp32-ops", "-fPIC", "-shared", str(src), "-o", str(out)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        pytest.fail("numerics shim build failed:\n" + res.stderr)
-    return ctypes.CDLL(str(out))
+
+@pytest.fixture(scope="session")
+def adjoint_shim(tmp_path_factory):
+    """Host build of the residual/adjoint element operations (same source as the adjoint kernels)."""
+    return _build_shim("adjoint_shim", tmp_path_factory)

@@ -1,0 +1,218 @@
+"""Discrete adjoint on the CPU: the element operations the adjoint kernels run, built for the host.
+
+* the element-wise residual (the forward the adjoint is the transpose of) equals the oracle's
+  residual bit for bit, for Cartesian, general-geometry, multi-block, wall, outlet and HLLL cases;
+* the flux and reconstruction VJPs agree with central differences of their forward functions;
+* the residual VJP, applied to every unit vector, reproduces the finite-difference Jacobian of
+  the residual entry by entry (smooth states; one-sided differences where a limiter or wave-speed
+  switch lies within the step).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import pyhype_oracle as O
+from pyhype_amd import workloads as W
+from pyhype_amd.backend import CaseSpec, layout_cases
+from pyhype_amd.solvers import mesh_dict
+from tests.helpers import oracle_for
+
+dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
+
+
+def _phys(cfg):
+    f = cfg.fluid
+    return np.array([f.gamma(), f.gm1(), f.one_over_gm1(), f.g_over_gm1(), cfg.CFL])
+
+
+CASES = {
+    "single": lambda: (W.explosion_single(10), False),
+    "single_general": lambda: (W.explosion_single(10), True),
+    "multiblock": lambda: (W.explosion_multiblock(6, 2, 4), False),
+    "oblique_hlll": lambda: (W.oblique_shock(8, 3), False),
+    "oblique_hlll_general": lambda: (W.oblique_shock(8, 3), True),
+}
+
+
+def _setup(name):
+    (cfg, mesh), general = CASES[name]()
+    md = mesh_dict(mesh)
+    lay = layout_cases([CaseSpec(md, cfg.nx, cfg.ny)], force_general=general)
+    flux = 0 if cfg.fvm_flux_function_type == "Roe" else 1
+    return cfg, mesh, md, lay, flux
+
+
+def _smooth_state(lay, seed, mach=0.4):
+    """Smooth, tie-free state on every block: sums of random plane waves of the centroids."""
+    rng = np.random.default_rng(seed)
+    U = np.zeros(lay.state_len)
+    waves = [(rng.uniform(0.3, 1.2, 2) * rng.choice([-1, 1], 2), rng.uniform(0, 2 * np.pi)) for _ in range(4)]
+    for e in lay.blocks:
+        x, y = np.asarray(e.geom.xc).reshape(e.ny, e.nx), np.asarray(e.geom.yc).reshape(e.ny, e.nx)
+        w = [np.sin(k[0] * x + k[1] * y + ph) for k, ph in waves]
+        rho = 1.0 + 0.3 * w[0] + 0.01 * x
+        u = mach * (1.2 * w[1] + 0.3)
+        v = mach * w[2]
+        p = 1.0 / 1.4 + 0.25 * w[3] + 0.01 * y
+        q = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)], axis=-1)
+        U[e.offset:e.offset + q.size] = q.ravel()
+    return U
+
+
+def _residual(shim, lay, flux, phys, U):
+    R = np.zeros_like(U)
+    shim.shim_residual(lay.descs, len(lay.blocks), flux, _p(phys), _p(U), _p(R))
+    return R
+
+
+def _vjp(shim, lay, flux, phys, U, w):
+    out = np.zeros_like(U)
+    shim.shim_residual_vjp(lay.descs, len(lay.blocks), flux, _p(phys), _p(U), _p(w), _p(out))
+    return out
+
+
+@pytest.fixture(scope="module")
+def shim(adjoint_shim):
+    s = adjoint_shim
+    s.shim_residual.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp]
+    s.shim_residual_vjp.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
+    s.shim_face_flux.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
+    s.shim_face_flux_vjp.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]
+    s.shim_reconstruct_vjp.argtypes = [dp, dp, dp, dp]
+    s.shim_reconstruct1.argtypes = [dp, dp, dp]
+    return s
+
+
+@pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll"])
+def test_elementwise_residual_equals_oracle(shim, name):
+    """(Forcing the general path on a Cartesian mesh changes the arithmetic, as it would in the
+    reference, so only the meshes' own geometry kinds are compared.)"""
+    cfg, mesh, md, lay, flux = _setup(name)
+    if name == "multiblock":
+        assert not any(e.cartesian for e in lay.blocks)  # the 2x4 generator mesh is not exactly Cartesian
+    U = _smooth_state(lay, seed=1, mach=0.9)
+    R = _residual(shim, lay, flux, _phys(cfg), U)
+    states = {e.num: U[e.offset:e.offset + 4 * e.nx * e.ny].reshape(e.ny, e.nx, 4) for e in lay.blocks}
+    sim = oracle_for(cfg, mesh, states)
+    for e in lay.blocks:
+        ref = O.residual(sim.blocks[e.num], sim.flux_fn)
+        got = R[e.offset:e.offset + ref.size].reshape(ref.shape)
+        np.testing.assert_array_equal(got, ref, err_msg=f"{name} block {e.num}")
+
+
+def _cons(rng, n, mach):
+    rho = rng.uniform(0.5, 2.0, n)
+    u = rng.uniform(-mach, mach, n)
+    v = rng.uniform(-mach, mach, n)
+    p = rng.uniform(0.3, 1.5, n)
+    return np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)], axis=1)
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("mach", [0.3, 2.0])
+def test_face_flux_vjp_matches_central_differences(shim, flux, mach):
+    rng = np.random.default_rng(10 + flux + int(10 * mach))
+    n = 400
+    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8])
+    qL, qR = _cons(rng, n, mach), _cons(rng, n, mach)
+    qR = qL + 0.3 * (qR - qL)  # nearby states, the usual regime of a face
+    lam = rng.standard_normal((n, 4))
+    vL, vR = rng.standard_normal((n, 4)), rng.standard_normal((n, 4))
+    bL, bR = np.zeros((n, 4)), np.zeros((n, 4))
+    shim.shim_face_flux_vjp(flux, n, _p(qL), _p(qR), _p(lam), _p(bL), _p(bR), _p(phys))
+
+    def F(a, b):
+        out = np.zeros((n, 4))
+        shim.shim_face_flux(flux, n, _p(a), _p(b), _p(out), _p(phys))
+        return out
+
+    h = 1e-6
+    fd = ((F(qL + h * vL, qR + h * vR) - F(qL - h * vL, qR - h * vR)) * lam).sum(1) / (2 * h)
+    ad = (bL * vL).sum(1) + (bR * vR).sum(1)
+    err = np.abs(fd - ad) / np.maximum(1.0, np.abs(ad))
+    # a sample within h of a switch (Harten, wave-speed selection, alpha clamp) has a one-sided FD
+    assert np.mean(err < 1e-6) > 0.99, np.sort(err)[-5:]
+    assert np.median(err) < 1e-8
+
+
+def test_reconstruct_vjp_matches_central_differences(shim):
+    rng = np.random.default_rng(5)
+    nfail = 0
+    for trial in range(300):
+        U5 = 1.0 + 0.2 * rng.standard_normal(20)
+        # cell geometry of a skewed quad: face weights sum to ~0, offsets around the centroid
+        g = np.zeros(17)
+        g[0:4] = [1.0, -1.1, 0.05, -0.02]       # xlE xlW xlN xlS
+        g[4:8] = [0.1, -0.05, 0.9, -1.0]        # ylE ylW ylN ylS
+        g[8] = 1.0 / 0.95
+        g[9:13] = [0.5, -0.5, 0.02, -0.03]      # dxE dxW dxN dxS
+        g[13:17] = [0.01, 0.02, 0.5, -0.5]      # dyE dyW dyN dyS
+        g[[0, 1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13, 14, 15, 16]] += 0.05 * rng.standard_normal(16)
+        bQ = rng.standard_normal(16)
+        v = rng.standard_normal(20)
+        out = np.zeros(20)
+        shim.shim_reconstruct_vjp(_p(U5), _p(g), _p(bQ), _p(out))
+
+        def Q(u):
+            r = np.zeros(16)
+            shim.shim_reconstruct1(_p(u), _p(g), _p(r))
+            return r
+
+        h = 1e-7
+        fd = ((Q(U5 + h * v) - Q(U5 - h * v)) * bQ).sum() / (2 * h)
+        ad = (out * v).sum()
+        if abs(fd - ad) > 1e-6 * max(1.0, abs(ad)):
+            nfail += 1
+    assert nfail <= 3, nfail
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_residual_vjp_is_the_transposed_jacobian(shim, name):
+    """Every entry of (dR/dU)^T, assembled column by column from the VJP, against finite
+    differences of the residual, entry by entry.  The scheme is piecewise smooth (limiter
+    max/min selections, Harten and wave-speed switches): an entry whose central difference
+    straddles such a switch must then agree with one of the one-sided differences instead, and
+    such entries must stay rare."""
+    cfg, mesh, md, lay, flux = _setup(name)
+    phys = _phys(cfg)
+    U = _smooth_state(lay, seed=2, mach=0.9 if flux == 1 else 0.4)
+    n = U.size
+    JT = np.zeros((n, n))
+    for r in range(n):
+        w = np.zeros(n)
+        w[r] = 1.0
+        JT[:, r] = _vjp(shim, lay, flux, phys, U, w)
+    R0 = _residual(shim, lay, flux, phys, U)
+    kinks = unexplained = 0
+    for i in range(n):
+        h = 1e-7 * max(1.0, abs(U[i]))
+        e = np.zeros(n)
+        e[i] = h
+        Rp, Rm = _residual(shim, lay, flux, phys, U + e), _residual(shim, lay, flux, phys, U - e)
+        fc, fp, fm = (Rp - Rm) / (2 * h), (Rp - R0) / h, (R0 - Rm) / h
+        scale = np.maximum(1.0, np.abs(JT[i]))
+        off = np.abs(fc - JT[i]) > 1e-5 * scale
+        one_sided = np.minimum(np.abs(fp - JT[i]), np.abs(fm - JT[i])) < 1e-4 * scale
+        kinks += int(np.any(off & one_sided))
+        unexplained += int(np.sum(off & ~one_sided))
+    assert unexplained == 0
+    assert kinks <= max(2, n // 100), kinks  # perturbed entries with a switch within the step
+    assert np.count_nonzero(JT) > 20 * n  # the stencil is really there
+
+
+def test_residual_vjp_is_linear_and_deterministic(shim):
+    cfg, mesh, md, lay, flux = _setup("multiblock")
+    phys = _phys(cfg)
+    U = _smooth_state(lay, seed=3)
+    rng = np.random.default_rng(0)
+    w1, w2 = rng.standard_normal(U.size), rng.standard_normal(U.size)
+    a = _vjp(shim, lay, flux, phys, U, w1)
+    b = _vjp(shim, lay, flux, phys, U, w2)
+    c = _vjp(shim, lay, flux, phys, U, 2.0 * w1 - 3.0 * w2)
+    np.testing.assert_allclose(c, 2.0 * a - 3.0 * b, rtol=1e-12, atol=1e-12 * np.abs(c).max())
+    np.testing.assert_array_equal(a, _vjp(shim, lay, flux, phys, U, w1))

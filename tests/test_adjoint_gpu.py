@@ -1,0 +1,163 @@
+"""Discrete adjoint on the GPU (gpu).
+
+* phx_residual_vjp equals the host build of the same element operations (whose transposed
+  Jacobian the CPU tests check entry by entry against finite differences);
+* phx_step_vjp equals the composition of residual VJPs given by the Runge-Kutta recurrence;
+* phx_run_record reproduces phx_run_frozen bit for bit and records the stage inputs;
+* the checkpointed reverse sweep is independent of the checkpoint interval and agrees with
+  central differences of the objective J = sum (rho(T) - rho_target(T))^2.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from pyhype_amd import Euler2D
+from pyhype_amd import workloads as W
+from pyhype_amd.adjoint import AdjointProblem, Trajectory, frozen_schedule, trajectory
+from tests.test_adjoint_cpu import CASES, _p, _phys, _smooth_state, dp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def shim(adjoint_shim):
+    adjoint_shim.shim_residual_vjp.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
+    return adjoint_shim
+
+
+def _solver(name):
+    (cfg, mesh), general = CASES[name]()
+    s = Euler2D(cfg, mesh, force_general=general)
+    return cfg, s
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_residual_vjp_matches_host_build(shim, name):
+    import torch
+
+    cfg, s = _solver(name)
+    P = s.plan
+    U = _smooth_state(P, seed=4, mach=0.9 if cfg.fvm_flux_function_type == "HLLL" else 0.4)
+    w = np.random.default_rng(1).standard_normal(U.size)
+    got = P.residual_vjp(torch.as_tensor(U, device="cuda"), torch.as_tensor(w, device="cuda")).cpu().numpy()
+    ref = np.zeros_like(U)
+    flux = 0 if cfg.fvm_flux_function_type == "Roe" else 1
+    shim.shim_residual_vjp(P._descs, len(P.blocks), flux, _p(_phys(cfg)), _p(U), _p(w), _p(ref))
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    print(name, "bitwise equal:", np.array_equal(got, ref))
+
+
+@pytest.mark.parametrize("integrator", ["SSPRK2", "RK4"])
+def test_step_vjp_is_the_rk_recurrence(integrator):
+    import torch
+
+    cfg, mesh = W.explosion_multiblock(12, 2, 2)
+    cfg.time_integrator = integrator
+    s = Euler2D(cfg, mesh)
+    P = s.plan
+    U0 = torch.as_tensor(_smooth_state(P, seed=5), device="cuda")
+    P.U.copy_(U0)
+    dt = P.dt()                                  # CFL dt of U0, frozen
+    stages = torch.zeros((1, P.nstages, P.state_len), dtype=torch.float64, device="cuda")
+    P.run_record(dt.view(1, -1), stages)
+    lamS = torch.as_tensor(np.random.default_rng(2).standard_normal(P.state_len), device="cuda")
+    lam = lamS.clone()
+    P.step_vjp(stages[0], dt, lam)
+    # reference: kappa_i = dt sum_{l>i} a[l-1][i] lambda_l, lambda_i = VJP(U_i, kappa_i)
+    a = P._a
+    S = P.nstages
+    lams = {S: lamS}
+    d = float(dt[0])
+    for i in range(S - 1, -1, -1):
+        kappa = sum(d * a[l - 1, i] * lams[l] for l in range(i + 1, S + 1) if a[l - 1, i] != 0.0)
+        if not torch.is_tensor(kappa):
+            kappa = torch.zeros_like(lamS)
+        lams[i] = P.residual_vjp(stages[0, i].contiguous(), kappa.contiguous())
+    ref = lamS + sum(lams[l] for l in range(0, S))
+    assert torch.allclose(lam, ref, rtol=1e-12, atol=1e-12 * float(ref.abs().max()))
+
+
+def test_run_record_matches_run_frozen():
+    import torch
+
+    cfg, mesh = W.explosion_single(32)
+    cfg.time_integrator = "RK4"
+    s = Euler2D(cfg, mesh)
+    s.apply_initial_condition()
+    P = s.plan
+    U0 = P.U.clone()
+    dt = frozen_schedule(P, 5)
+    P.U.copy_(U0)
+    P.run_frozen(dt)
+    ref = P.U.clone()
+    P.U.copy_(U0)
+    stages = torch.zeros((5, P.nstages, P.state_len), dtype=torch.float64, device="cuda")
+    P.run_record(dt, stages)
+    assert torch.equal(P.U, ref)
+    assert torch.equal(stages[0, 0], U0)
+    # the input of stage 0 of step n is the state after step n-1
+    P.U.copy_(U0)
+    P.run_frozen(dt[:2])
+    assert torch.equal(stages[2, 0], P.U)
+
+
+def _problem(n, steps, every, width=0.1):
+    cfg, mesh, target, params = W.adjoint_problem(n, seed=0, width=width)
+    s = Euler2D(cfg, mesh)
+    return AdjointProblem(s, target, nsteps=steps, checkpoint_every=every)
+
+
+def test_gradient_independent_of_checkpoint_interval():
+    import torch
+
+    prob = _problem(40, 12, 5)
+    J, g5 = prob.gradient()
+    for every in (1, 12):
+        prob.traj = Trajectory(prob.plan, prob.dt, every)
+        J2, g = prob.gradient()
+        assert J2 == J
+        assert torch.equal(g, g5)
+
+
+def test_gradient_matches_central_differences():
+    """12x12 smoothed explosion (ramp half-width 1), 4 SSP-RK2 steps, checkpoints every 3:
+    <dJ/dU0, v> on seeded unit directions, and every entry of dJ/dU0, against central differences.
+    The discrete map is piecewise smooth (limiter max/min selections, |u| and Harten switches);
+    on this trajectory the directional differences at h = 1e-7 |U0| cross none of them (the host
+    build of the same operations gives errors of 2e-9 to 4e-9 there).  Entry-wise, a perturbation
+    that straddles a switch must agree with one of its one-sided differences instead."""
+    import torch
+
+    prob = _problem(12, 4, 3, width=1.0)
+    J, g = prob.gradient()
+    U0 = prob.U0
+    rng = np.random.default_rng(0)
+    h = 1e-7 * float(U0.norm())
+    errs = []
+    for _ in range(4):
+        v = torch.as_tensor(rng.standard_normal(U0.numel()), device="cuda")
+        v /= v.norm()
+        fd = (prob.value(U0 + h * v) - prob.value(U0 - h * v)) / (2 * h)
+        ad = float((g * v).sum())
+        errs.append(abs(fd - ad) / abs(ad))
+    print("relative FD errors", errs)
+    assert max(errs) < 1e-6, errs
+    gh = g.cpu().numpy()
+    J0 = prob.value(U0)
+    scale = np.abs(gh).max()
+    kinks = bad = 0
+    for i in range(U0.numel()):
+        hi = 1e-7 * max(1.0, abs(float(U0[i])))
+        e = torch.zeros_like(U0)
+        e[i] = hi
+        Jp, Jm = prob.value(U0 + e), prob.value(U0 - e)
+        fc, fp, fm = (Jp - Jm) / (2 * hi), (Jp - J0) / hi, (J0 - Jm) / hi
+        if abs(fc - gh[i]) > 1e-6 * scale:
+            if min(abs(fp - gh[i]), abs(fm - gh[i])) < 1e-4 * scale:
+                kinks += 1
+            else:
+                bad += 1
+    print("entries", U0.numel(), "kinks", kinks)
+    assert bad == 0
+    assert kinks <= U0.numel() // 50
