@@ -10,6 +10,7 @@ with every step executed on the device.  ``run_steps(n)`` advances exactly ``n``
 from __future__ import annotations
 
 import logging
+import pathlib
 from typing import Dict, List, Optional, Sequence
 
 import numpy as np
@@ -150,8 +151,9 @@ class Euler2DBatch:
         if n <= 0:
             return
         self.plan.run(n, use_graph=use_graph, check=self.strict)
-        bad, step, t, _ = self.plan.status()
+        bad, step, t, dt = self.plan.status()
         self.t = t
+        self.dt_last = dt
         self.num_time_step = step
         self.failed = {c: int(b) for c, b in enumerate(bad) if b >= 0}
 
@@ -176,7 +178,7 @@ class Euler2DBatch:
 
 
 class Euler2D(Euler2DBatch):
-    """Single-case solver with the reference's constructor and ``solve()``."""
+    """Single-case solver with the reference's constructor, ``solve()`` and file output."""
 
     strict = True
 
@@ -184,6 +186,12 @@ class Euler2D(Euler2DBatch):
         super().__init__([config], [mesh_config], **kw)
         self.config = config
         self.t_final = config.t_final * config.fluid.far_field.a
+        # Solver.__init__ (solvers/base.py:78-86)
+        self.write_path = None
+        if getattr(config, "write_solution", False):
+            self.write_path = pathlib.Path(config.write_solution_base) / config.write_solution_name
+            self.write_path.mkdir(exist_ok=True, parents=True)
+            _log.info(self.write_path)
 
     @property
     def blocks(self):
@@ -192,15 +200,61 @@ class Euler2D(Euler2DBatch):
     def state(self, block_num: int) -> np.ndarray:
         return self.states()[0][block_num]
 
+    # ---------------------------------------------------------------- file output
+    def write_mesh(self):
+        """Solver.write_mesh (solvers/base.py:142-155): cell centroids of every block as
+        ``<write_path>/mesh/mesh_{x,y}_blk_<n>.npy`` with shape (ny, nx, 1)."""
+        path = self.write_path / "mesh"
+        path.mkdir(parents=True, exist_ok=True)
+        for b in self.blocks:
+            np.save(str(path / "mesh_x_blk_") + str(b.global_block_num), b.mesh.x)
+            np.save(str(path / "mesh_y_blk_") + str(b.global_block_num), b.mesh.y)
+
+    def write_solution(self):
+        """Solver.write_solution (solvers/base.py:157-172): in mode ``every_n_timesteps`` and when
+        ``num_time_step % write_every_n_timesteps == 0``, the conservative state of every block as
+        ``<write_path>/<num_time_step>/<write_solution_name>_blk_<n>.npy`` (ny, nx, 4)."""
+        cfg = self.config
+        if cfg.write_solution_mode == "every_n_timesteps" and self.num_time_step % cfg.write_every_n_timesteps == 0:
+            path = self.write_path / str(self.num_time_step)
+            path.mkdir(parents=True, exist_ok=True)
+            states = self.plan.get_states()[0]
+            for b in self.blocks:
+                np.save(str(path / cfg.write_solution_name) + "_blk_" + str(b.global_block_num),
+                        states[b.global_block_num])
+
     def solve(self, chunk: int = 50, max_steps: Optional[int] = None):
-        """Run to t_final (solvers/Euler2D.py:97-100,175-216)."""
+        """Run to t_final (solvers/Euler2D.py:97-100,175-216).
+
+        With ``config.write_solution`` the mesh is written first and the solution at the reference's
+        points: ``write_solution`` runs after a step's update and before ``num_time_step`` is
+        incremented, so the files in directory ``k`` (k a multiple of write_every_n_timesteps) hold
+        the state after step k + 1.  Steps between output points run on the device in chunks."""
         self.apply_initial_condition()
+        writing = self.write_path is not None
+        if writing:
+            self.write_mesh()
+        every = max(1, int(getattr(self.config, "write_every_n_timesteps", 1) or 1))
         steps = 0
         while self.t[0] < self.t_final and (max_steps is None or steps < max_steps):
             if self.num_time_step % 50 == 0:
                 _log.info("Simulation time: %s, Timestep number: %s", self.t[0] / self.fluid.far_field.a,
                           self.num_time_step)
             n = chunk if max_steps is None else min(chunk, max_steps - steps)
+            last = float(getattr(self, "dt_last", [0.0])[0])
+            if last <= 0.0 or self.t_final - self.t[0] < 1.5 * n * last:
+                n = 1  # first step, or near t_final: single steps, so the count stops where the reference's does
+            if writing and self.config.write_solution_mode == "every_n_timesteps":
+                # the next output point: the step whose pre-increment counter is a multiple of `every`
+                k = self.num_time_step
+                nxt = k if k % every == 0 else k + (every - k % every)
+                n = min(n, nxt - k + 1)
             self.run_steps(n)
             steps += n
+            if writing:
+                # run_steps advanced num_time_step by n; the reference writes with the counter of the
+                # step just taken, before its increment
+                self.num_time_step -= 1
+                self.write_solution()
+                self.num_time_step += 1
         self.sync_to_host()

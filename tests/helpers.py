@@ -35,3 +35,16 @@ def max_rel_err_blocks(gs, rs):
 
 def bitwise_equal_blocks(gs, rs):
     return all(np.array_equal(gs[n], rs[n]) for n in rs)
+
+
+def skewed_mesh():
+    """Two genuinely non-rectangular blocks sharing a slanted edge (a wedge-like channel): west
+    outlet, east wall, walls north and south."""
+    def blk(n, sw, se, nw, ne, nbE, nbW, bcE, bcW):
+        return {"nBLK": n, "NW": nw, "NE": ne, "SW": sw, "SE": se,
+                "NeighborE": nbE, "NeighborW": nbW, "NeighborN": None, "NeighborS": None,
+                "NeighborNE": None, "NeighborNW": None, "NeighborSE": None, "NeighborSW": None,
+                "BCTypeE": bcE, "BCTypeW": bcW, "BCTypeN": "Reflection", "BCTypeS": "Reflection",
+                "BCTypeNE": None, "BCTypeNW": None, "BCTypeSE": None, "BCTypeSW": None}
+    return {0: blk(0, [0.0, 0.0], [5.0, 0.8], [0.3, 10.0], [5.6, 9.1], 1, None, None, "OutletDirichlet"),
+            1: blk(1, [5.0, 0.8], [10.0, 1.9], [5.6, 9.1], [10.4, 8.0], None, 0, "Reflection", None)}

@@ -16,7 +16,7 @@ from oracle import pyhype_oracle as O
 from pyhype_amd import workloads as W
 from pyhype_amd.backend import CaseSpec, layout_cases
 from pyhype_amd.solvers import mesh_dict
-from tests.helpers import oracle_for
+from tests.helpers import oracle_for, skewed_mesh
 
 dp = C.POINTER(C.c_double)
 
@@ -36,6 +36,7 @@ CASES = {
     "multiblock": lambda: (W.explosion_multiblock(6, 2, 4), False),
     "oblique_hlll": lambda: (W.oblique_shock(8, 3), False),
     "oblique_hlll_general": lambda: (W.oblique_shock(8, 3), True),
+    "skewed": lambda: ((W.explosion_single(8)[0], skewed_mesh()), False),
 }
 
 
@@ -88,7 +89,7 @@ def shim(adjoint_shim):
     return s
 
 
-@pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll"])
+@pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll", "skewed"])
 def test_elementwise_residual_equals_oracle(shim, name):
     """(Forcing the general path on a Cartesian mesh changes the arithmetic, as it would in the
     reference, so only the meshes' own geometry kinds are compared.)"""

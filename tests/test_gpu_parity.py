@@ -6,7 +6,7 @@ import pytest
 from oracle import pyhype_oracle as O
 from pyhype_amd import Euler2D, Euler2DBatch, RealizabilityError
 from pyhype_amd import workloads as W
-from tests.helpers import bitwise_equal_blocks, max_rel_err_blocks, oracle_for
+from tests.helpers import bitwise_equal_blocks, max_rel_err_blocks, oracle_for, skewed_mesh
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-9
@@ -161,3 +161,52 @@ def test_unrealizable_state_raises():
     s.set_state([st])
     with pytest.raises(RealizabilityError):
         s.run_steps(1)
+
+
+def test_solve_writes_reference_files_and_stops_at_t_final(tmp_path):
+    """Euler2D.solve with write_solution: mesh files, solution directories k = 0, n, 2n, ... holding
+    the state after step k+1 (Solver.write_solution runs before the step counter is incremented),
+    and the step count at which the reference loop stops (t >= t_final)."""
+    cfg, mesh = W.explosion_single(20)
+    a = cfg.fluid.far_field.a
+    cfg.t_final = 0.004            # seconds; 0.004*343 non-dimensional
+    cfg.write_solution = True
+    cfg.write_solution_base = str(tmp_path)
+    cfg.write_solution_name = "expl"
+    cfg.write_every_n_timesteps = 7
+    s = Euler2D(cfg, mesh)
+    s.solve(chunk=5)
+    # oracle: the reference loop with its write points
+    h = Euler2D(W.explosion_single(20)[0], mesh)
+    h.apply_initial_condition()
+    sim = oracle_for(cfg, mesh, h.states()[0])
+    written = {}
+    k = 0
+    while sim.t < cfg.t_final * a:
+        sim.step()
+        if k % 7 == 0:
+            written[k] = sim.states()
+        k += 1
+    assert s.num_time_step == k
+    root = tmp_path / "expl"
+    np.testing.assert_array_equal(np.load(root / "mesh" / "mesh_x_blk_0.npy"), s.case_blocks[0][0].mesh.x)
+    dirs = sorted(int(p.name) for p in root.iterdir() if p.name != "mesh")
+    assert dirs == sorted(written)
+    for step, st in written.items():
+        np.testing.assert_array_equal(np.load(root / str(step) / "expl_blk_0.npy"), st[0], err_msg=f"dir {step}")
+
+
+def test_skewed_multiblock_residual_and_steps_match_oracle():
+    cfg, _ = W.explosion_single(18)
+    mesh = skewed_mesh()
+    s = _solver(cfg, mesh)
+    assert not any(e.cartesian for e in s.plan.blocks)
+    st = _perturbed(s.states()[0], seed=8)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(6)
+    sim.run(6)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
