@@ -47,8 +47,11 @@ extern "C" {
 #define PHX_FLUX_ROE 0
 #define PHX_FLUX_HLLL 1
 
-/* limiters (limiters/factory.py:28-39) */
+/* limiters (limiters/factory.py:28-39, limiters/limiters.py:25-62) */
 #define PHX_LIMITER_VENKATAKRISHNAN 0
+#define PHX_LIMITER_BARTH_JESPERSEN 1
+#define PHX_LIMITER_VAN_ALBADA 2
+#define PHX_LIMITER_VAN_LEER 3
 
 /* geometry kinds of a block's packed geometry array */
 #define PHX_GEOM_CARTESIAN 1 /* axis-aligned block, separable 1-D arrays (see below) */
@@ -176,10 +179,18 @@ int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double
 int phx_step_vjp(phx_plan* plan, const double* stages, const double* dt_dev, double* lam, void* stream);
 
 /* Tunables of the stage launch: rows per work item, persistent grid size in waves (0 = one
- * workgroup per item) and the register budget variant (waves per SIMD, 2 or 3; 0 = 2); rebuilds
- * the work list.  Used by the autotuner in pyhype_amd/autotune.py.  No counterpart in the
- * reference. */
+ * workgroup per item) and the register budget variant (waves per SIMD, 2, 3 or 4; 0 = 2; the
+ * fused kernel uses at most 3); rebuilds the work list.  Used by the autotuner in
+ * pyhype_amd/autotune.py.  No counterpart in the reference. */
 int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd);
+
+/* Stage kernel variant: 0 = fused rolling rows (default; best for large meshes), 1 = split into a
+ * reconstruction + x-face kernel and a y-face + residual + update kernel, which reconstructs
+ * every cell exactly once (best for small meshes, where short fused items spend most of their
+ * work on halo rows); rows_b = rows per work item of the second kernel (0 = keep).  Both variants
+ * give identical results.  Rebuilds the work lists; the split variant allocates 12 doubles per
+ * cell.  No counterpart in the reference. */
+int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 
 /* Number of work items and the persistent grid used for the Cartesian and general kernels. */
 int phx_launch_info(const phx_plan* plan, int32_t* items_cart, int32_t* items_gen,

@@ -444,6 +444,7 @@ class Block:
     bc: Dict[str, Optional[str]]
     U: np.ndarray = None
     ghost: Dict[str, np.ndarray] = field(default_factory=dict)
+    limiter_fn: object = None          # limiter function (None: Venkatakrishnan)
 
 
 def boundary_theta(g: Geometry, d: str):
@@ -512,6 +513,27 @@ def venkatakrishnan(s):
     return (s2 + 2 * s) / (s2 + s + 2)
 
 
+def barth_jespersen(s):
+    """BarthJespersen._limiter_func (limiters/limiters.py:25-28): np.minimum(1, s)."""
+    return np.minimum(1, s)
+
+
+def van_albada(s):
+    """VanAlbada._limiter_func (limiters/limiters.py:52-56); slope**2 is s*s."""
+    s2 = s * s
+    return (s2 + s) / (s2 + 1)
+
+
+def van_leer(s):
+    """VanLeer._limiter_func (limiters/limiters.py:59-62)."""
+    return (np.absolute(s) + s) / (s + 1)
+
+
+# SlopeLimiterFactory.create (limiters/factory.py:28-39)
+LIMITERS = {"Venkatakrishnan": venkatakrishnan, "BarthJespersen": barth_jespersen, "VanAlbada": van_albada,
+            "VanLeer": van_leer}
+
+
 def compute_slope(dmax, dmin, d):
     """SlopeLimiter._compute_slope (limiters/base.py:189-221)."""
     s = np.ones_like(d)
@@ -523,8 +545,9 @@ def compute_slope(dmax, dmin, d):
     return s
 
 
-def limiter_phi(U, ghost, hot: Dict[str, np.ndarray]):
-    """SlopeLimiter._get_slope / _limit (limiters/base.py:47-108,147-187) for Venkatakrishnan."""
+def limiter_phi(U, ghost, hot: Dict[str, np.ndarray], limiter_fn=venkatakrishnan):
+    """SlopeLimiter._get_slope / _limit (limiters/base.py:47-108,147-187) with the limiter
+    function of the configured limiter (default Venkatakrishnan)."""
     EW = np.concatenate((ghost["W"], U, ghost["E"]), axis=1)
     NS = np.concatenate((ghost["S"], U, ghost["N"]), axis=0)
     vals = (U, EW[:, :-2], EW[:, 2:], NS[:-2, :], NS[2:, :])
@@ -534,7 +557,8 @@ def limiter_phi(U, ghost, hot: Dict[str, np.ndarray]):
     for d in DIRS:
         q = U + hot[d]          # unlimited solution at the quadrature point
         dd = q - U
-        phis.append(venkatakrishnan(compute_slope(dmax, dmin, dd)))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            phis.append(limiter_fn(compute_slope(dmax, dmin, dd)))
     phi = np.minimum.reduce(tuple(phis))
     return np.where(phi < 0, 0, phi)
 
@@ -558,7 +582,7 @@ def residual(b: Block, flux_fn):
     g = b.geom
     gx, gy = green_gauss(U, b.ghost, g)
     hot = {d: high_order_term(gx, gy, g, d) for d in DIRS}
-    phi = limiter_phi(U, b.ghost, hot)
+    phi = limiter_phi(U, b.ghost, hot, b.limiter_fn or venkatakrishnan)
     face = {d: U + phi * hot[d] for d in DIRS}   # limited_solution_at_quadrature_point
     bnd = {"E": face["E"][:, -1:], "W": face["W"][:, :1], "N": face["N"][-1:], "S": face["S"][:1]}
     # ---- east-west sweep (fvm/base.py:327-390)
@@ -675,8 +699,10 @@ class Simulation:
     and Solver.get_dt (solvers/base.py:114-136)."""
 
     def __init__(self, mesh: Dict[int, dict], nx: int, ny: int, *, flux: str = "Roe",
-                 integrator="RK4", CFL: float = 0.8, t_final: float = 1e30, a_inf: float = 1.0):
+                 integrator="RK4", CFL: float = 0.8, t_final: float = 1e30, a_inf: float = 1.0,
+                 limiter: str = "Venkatakrishnan"):
         self.flux_fn = {"Roe": flux_roe, "HLLL": flux_hlll}[flux]
+        self.limiter_fn = LIMITERS[limiter]
         self.a = BUTCHER[integrator] if isinstance(integrator, str) else integrator
         self.CFL = CFL
         self.t = 0.0
@@ -688,7 +714,7 @@ class Simulation:
             vert = {k: d[k] for k in ("SW", "SE", "NW", "NE")}
             g = block_geometry(vert, nx, ny)
             self.blocks[n] = Block(n, g, {s: d["Neighbor" + s] for s in DIRS},
-                                   {s: d["BCType" + s] for s in DIRS})
+                                   {s: d["BCType" + s] for s in DIRS}, limiter_fn=self.limiter_fn)
 
     def set_state(self, states: Dict[int, np.ndarray]):
         for n, b in self.blocks.items():

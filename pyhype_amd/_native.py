@@ -15,12 +15,17 @@ E, W, N, S = 0, 1, 2, 3
 SIDE_NEIGHBOR, SIDE_OPEN, SIDE_WALL, SIDE_OUTLET = 0, 1, 2, 3
 FLUX_ROE, FLUX_HLLL = 0, 1
 LIMITER_VENKATAKRISHNAN = 0
+LIMITER_BARTH_JESPERSEN = 1
+LIMITER_VAN_ALBADA = 2
+LIMITER_VAN_LEER = 3
+LIMITERS = {"Venkatakrishnan": LIMITER_VENKATAKRISHNAN, "BarthJespersen": LIMITER_BARTH_JESPERSEN,
+            "VanAlbada": LIMITER_VAN_ALBADA, "VanLeer": LIMITER_VAN_LEER}
 GEOM_GENERAL, GEOM_CARTESIAN = 0, 1
 
 EXPORTS = (
     "phx_plan_create", "phx_plan_destroy", "phx_state_len", "phx_reset", "phx_run", "phx_run_frozen",
     "phx_run_record", "phx_stage", "phx_residual", "phx_dt", "phx_ghosts", "phx_status", "phx_residual_vjp",
-    "phx_step_vjp", "phx_set_launch", "phx_launch_info", "phx_last_error",
+    "phx_step_vjp", "phx_set_launch", "phx_set_variant", "phx_launch_info", "phx_last_error",
 )
 
 
@@ -74,6 +79,7 @@ def load(path: Path = LIB_PATH):
         "phx_residual_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_step_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_set_launch": ([vp, i32, i32, i32], C.c_int),
+        "phx_set_variant": ([vp, i32, i32], C.c_int),
         "phx_launch_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "phx_last_error": ([], C.c_char_p),
     }

@@ -1,11 +1,13 @@
-"""Launch-shape autotuner for the stage kernel.
+"""Launch-shape autotuner for the stage kernels.
 
-The stage kernel's work item is a 60-column strip of one block over ``rows_per_item`` rows; each
+Fused variant: the work item is a 60-column strip of one block over ``rows_per_item`` rows; each
 item recomputes one reconstruction row above and below its range, so short items waste work while
-long items leave SIMDs idle on small meshes.  ``autotune`` times whole steps (hipGraph replay)
-for a set of candidate shapes and keeps the fastest; results are cached in a JSON file keyed by
-a signature of the workload (geometry kind, flux, stages, block shapes, device).
-The state, time and step counter of the plan are restored after tuning.
+long items leave SIMDs idle on small meshes.  Split variant: a reconstruction kernel with no row
+halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per item).
+``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
+fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
+flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
+after tuning.
 """
 from __future__ import annotations
 
@@ -29,7 +31,9 @@ def signature(plan) -> str:
     shp = ",".join(f"{n}x{a}x{b}" for (a, b), n in sorted(shapes.items()))
     dev = getattr(torch.cuda.get_device_properties(0), "gcnArchName", "gpu").split(":")[0] \
         if torch.cuda.is_available() else "none"
-    return f"{kinds}|{plan.flux}|{len(plan.table)}|{shp}|{dev}"
+    lim = getattr(plan, "limiter", "Venkatakrishnan")
+    flux = plan.flux if lim == "Venkatakrishnan" else f"{plan.flux}/{lim}"
+    return f"{kinds}|{flux}|{len(plan.table)}|{shp}|{dev}"
 
 
 def load_cache(path: Path = CACHE) -> Dict[str, dict]:
@@ -54,34 +58,49 @@ def _time_steps(plan, steps: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def autotune(plan, steps: int = 10, rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,),
-             waves: Sequence[int] = (2, 3), path: Optional[Path] = CACHE, verbose: bool = False) -> dict:
-    """Time candidate (rows_per_item, grid_waves, waves_per_simd) shapes on the plan's current
-    state; apply and cache the best.  Returns the best shape, its ms/step and all trials."""
+def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: Sequence[int] = (2, 3),
+               split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (4, 8, 16, 32),
+               split_waves: Sequence[int] = (3, 4)):
+    """The default search space: fused shapes, then split shapes."""
+    out = [{"variant": 0, "rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "rows_b": 0}
+           for w in waves for g in grids for r in rows]
+    out += [{"variant": 1, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
+            for w in split_waves for r in split_rows for rb in split_rows_b]
+    return out
+
+
+def _apply(plan, c):
+    plan.set_launch(c["rows_per_item"], c["grid_waves"], c["waves_per_simd"], variant=c.get("variant", 0),
+                    rows_b=c.get("rows_b", 0))
+
+
+def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, path: Optional[Path] = CACHE,
+             verbose: bool = False) -> dict:
+    """Time candidate launch shapes (``candidates()`` by default) on the plan's current state;
+    apply and cache the best.  Returns the best shape, its ms/step and all trials."""
     import numpy as np
 
+    shapes = candidates() if shapes is None else shapes
     U_saved = plan.U.clone()
     _, step0, t0, _ = plan.status()
     trials = []
-    for w in waves:
-        for g in grids:
-            for r in rows:
-                plan.set_launch(r, g, w)
-                plan.U.copy_(U_saved)
-                plan.reset(t0, step0)
-                ms = _time_steps(plan, steps) * 1e3
-                trials.append({"rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "ms_per_step": ms})
-                if verbose:
-                    print(f"autotune rows={r} grid={g} waves={w}: {ms:.4f} ms/step", flush=True)
+    for c in shapes:
+        _apply(plan, c)
+        plan.U.copy_(U_saved)
+        plan.reset(t0, step0)
+        ms = _time_steps(plan, steps) * 1e3
+        trials.append(dict(c, ms_per_step=ms))
+        if verbose:
+            print(f"autotune {c}: {ms:.4f} ms/step", flush=True)
     best = min(trials, key=lambda t: t["ms_per_step"])
-    plan.set_launch(best["rows_per_item"], best["grid_waves"], best["waves_per_simd"])
+    _apply(plan, best)
     plan.U.copy_(U_saved)
     plan.reset(np.asarray(t0), step0)
     result = dict(best, trials=trials)
     if path is not None:
         cache = load_cache(path)
-        cache[signature(plan)] = {k: best[k] for k in ("rows_per_item", "grid_waves", "waves_per_simd",
-                                                          "ms_per_step")}
+        cache[signature(plan)] = {k: best[k] for k in ("variant", "rows_per_item", "rows_b", "grid_waves",
+                                                          "waves_per_simd", "ms_per_step")}
         Path(path).write_text(json.dumps(cache, indent=1, sort_keys=True))
     return result
 
@@ -91,5 +110,7 @@ def apply_cached(plan, path: Path = CACHE) -> bool:
     hit = lookup(plan, path)
     if hit is None:
         return False
-    plan.set_launch(int(hit["rows_per_item"]), int(hit["grid_waves"]), int(hit.get("waves_per_simd", 2)))
+    _apply(plan, {"rows_per_item": int(hit["rows_per_item"]), "grid_waves": int(hit["grid_waves"]),
+                  "waves_per_simd": int(hit.get("waves_per_simd", 2)), "variant": int(hit.get("variant", 0)),
+                  "rows_b": int(hit.get("rows_b", 0))})
     return True

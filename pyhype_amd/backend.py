@@ -148,7 +148,8 @@ def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layo
 
 class DevicePlan:
     def __init__(self, cases: Sequence[CaseSpec], *, flux: str, table: List[List[float]], fluid, cfl: float,
-                 rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False):
+                 rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False,
+                 limiter: str = "Venkatakrishnan"):
         torch = _torch()
         self.torch = torch
         self.lib = nat.load()
@@ -157,6 +158,9 @@ class DevicePlan:
         if flux_id is None:
             raise ValueError(f"Flux function type {flux} is not available.")
         self.flux = flux
+        if limiter not in nat.LIMITERS:
+            raise ValueError("MUSCL: Slope limiter type not specified.")
+        self.limiter = limiter
         self.table = table
         S = len(table)
         a = np.zeros((S, S))
@@ -177,7 +181,7 @@ class DevicePlan:
         pd.blocks = descs
         pd.ncases = len(self.cases)
         pd.flux = flux_id
-        pd.limiter = nat.LIMITER_VENKATAKRISHNAN
+        pd.limiter = nat.LIMITERS[limiter]
         pd.nstages = S
         pd.butcher_a = self._a.ctypes.data_as(C.POINTER(C.c_double))
         pd.gamma = g
@@ -342,11 +346,23 @@ class DevicePlan:
         nat.check(self.lib.phx_step_vjp(self.handle, self.ptr(stages), self.ptr(dt), self.ptr(lam), self.stream()),
                   "phx_step_vjp")
 
-    def set_launch(self, rows_per_item: int, grid_waves: int = 0, waves_per_simd: int = 2):
+    def set_launch(self, rows_per_item: int, grid_waves: int = 0, waves_per_simd: int = 2,
+                   variant: Optional[int] = None, rows_b: int = 0):
+        """Launch shape of the stage: rows per work item, persistent grid (0 = one wave per item),
+        register budget in waves per SIMD; optionally the stage variant (0 fused, 1 split) and the
+        rows per item of the split variant's second kernel."""
+        if variant is not None:
+            nat.check(self.lib.phx_set_variant(self.handle, int(variant), int(rows_b)), "phx_set_variant")
         nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves), int(waves_per_simd)),
                   "phx_set_launch")
-        self.launch = {"rows_per_item": int(rows_per_item), "grid_waves": int(grid_waves),
-                       "waves_per_simd": int(waves_per_simd)}
+        launch = dict(getattr(self, "launch", {}))
+        launch.update({"rows_per_item": int(rows_per_item), "grid_waves": int(grid_waves),
+                       "waves_per_simd": int(waves_per_simd)})
+        if variant is not None:
+            launch["variant"] = int(variant)
+            if rows_b:
+                launch["rows_b"] = int(rows_b)
+        self.launch = launch
 
     def launch_info(self):
         v = [C.c_int32() for _ in range(4)]

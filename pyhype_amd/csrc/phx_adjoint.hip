@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                    const double* U, double* rb) {
+                                                    const double* U, double* rb, int lim) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
@@ -56,12 +56,12 @@ __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, cons
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = rb + phx::adj::RB * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out);
-    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out, lim);
+    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out, lim);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                   const double* U, const double* fadj, double* slot) {
+                                                   const double* U, const double* fadj, double* slot, int lim) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
@@ -69,8 +69,8 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out);
-    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out, lim);
+    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, lim);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
@@ -174,7 +174,8 @@ int adj_setup(phx_plan* P) {
 void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, double* lamU, hipStream_t st) {
     const dim3 blk(TILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
-    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb);
+    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
+                       P->ph.limiter);
     if (P->flux == PHX_FLUX_ROE)
         hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
                            P->adj_face, P->ph);
@@ -182,7 +183,7 @@ void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, doub
         hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
                            P->adj_face, P->ph);
     hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
-                       P->adj_slot);
+                       P->adj_slot, P->ph.limiter);
     hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
                        lamU);
 }

@@ -383,8 +383,18 @@ __host__ __device__ inline double venk_prime(double s) {
     return (-s * s + 4.0 * s + 4.0) / (den * den);
 }
 
+// Derivative of limiter_fn(lim, s) with respect to s (s >= 0 on every reachable branch).
+__host__ __device__ inline double limiter_prime(int lim, double s) {
+    switch (lim) {
+        case 1: return (s < 1.0) ? 1.0 : 0.0;                                        // min(1, s)
+        case 2: { const double den = s * s + 1.0; return (-s * s + 2.0 * s + 1.0) / (den * den); }
+        case 3: return (s >= 0.0) ? 2.0 / ((s + 1.0) * (s + 1.0)) : 0.0;            // 2s/(s+1)
+        default: return venk_prime(s);
+    }
+}
+
 __host__ __device__ inline ReconBar reconstruct_vjp(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
-                                                    const Q4& Un, const CellGeo& g, const Recon& bQ) {
+                                                    const Q4& Un, const CellGeo& g, const Recon& bQ, int lim = 0) {
     ReconBar out;
     for (int k = 0; k < 4; ++k) {
         const double u = U.v[k];
@@ -407,10 +417,8 @@ __host__ __device__ inline ReconBar reconstruct_vjp(const Q4& U, const Q4& Uw, c
         double phif[4], sf[4], df[4];
         for (int f = 0; f < 4; ++f) {
             df[f] = (u + hf[f]) - u;
-            const double num = (df[f] > 0.0) ? dmax : dmin;
-            sf[f] = (df[f] > 0.0 || df[f] < 0.0) ? num / df[f] : 1.0;
-            const double s2 = sf[f] * sf[f];
-            phif[f] = (s2 + 2.0 * sf[f]) / (s2 + sf[f] + 2.0);
+            sf[f] = face_slope(dmax, dmin, df[f]);
+            phif[f] = limiter_fn(lim, sf[f]);
         }
         int fmin = 0;
         for (int f = 1; f < 4; ++f)
@@ -430,7 +438,7 @@ __host__ __device__ inline ReconBar reconstruct_vjp(const Q4& U, const Q4& Uw, c
             const double dfs = df[fmin];
             if (dfs > 0.0 || dfs < 0.0) {
                 const double sv = sf[fmin];
-                const double bsv = bphi * venk_prime(sv);
+                const double bsv = bphi * limiter_prime(lim, sv);
                 const double bnum = bsv / dfs;
                 const double bd = -bsv * sv / dfs;
                 if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }

@@ -65,14 +65,14 @@ __host__ __device__ inline bool is_zero(const Q4& a) {
 // ---------------------------------------------------------------------------------------------
 template <bool CART>
 __host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
-                                           double* rb) {
+                                           double* rb, int lim) {
     const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
     const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
     const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
     const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
     const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
     const CellGeo g = dev::cell_geo<CART>(B, k, j);
-    const Recon r = reconstruct(Uc, Uw, Ue, Us, Un, g);
+    const Recon r = reconstruct<true>(Uc, Uw, Ue, Us, Un, g, lim);
     st(rb + RB_E, r.E);
     st(rb + RB_W, r.W);
     st(rb + RB_N, r.N);
@@ -242,7 +242,7 @@ __host__ __device__ inline Q4 outer_own_adj(const DevBlock& B, int side, const Q
 // pass 3 reverse: adjoints of the four face states of cell (k, j) -> reconstruction VJP.
 template <bool CART>
 __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock& B, const double* U,
-                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot) {
+                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot, int lim) {
     const int nx = B.nx, ny = B.ny;
     Recon bQ;
     bQ.E = ld(fadj + FA * xface_index(O, B, k, j + 1));          // left state of the east face
@@ -263,7 +263,7 @@ __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock&
     const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
     const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
     const CellGeo g = dev::cell_geo<CART>(B, k, j);
-    const ReconBar r = reconstruct_vjp(Uc, Uw, Ue, Us, Un, g, bQ);
+    const ReconBar r = reconstruct_vjp(Uc, Uw, Ue, Us, Un, g, bQ, lim);
     st(slot + SL_C, r.c);
     st(slot + SL_W, r.w);
     st(slot + SL_E, r.e);

@@ -52,6 +52,8 @@ struct StageArgs {
     int* bad;          // per case: first step whose result was unrealizable (INT_MAX = none)
     const int* step;   // index of the step being computed (1-based)
     int skip_dead;     // skip the blocks of cases that failed at an earlier step
+    double* sn;        // split stage: limited N and S states per cell (8 doubles, offset 2*off)
+    double* p1;        // split stage: LW*(2 FW) - LE*(2 FE) per cell (4 doubles, state layout)
     Phys ph;
 };
 
@@ -69,10 +71,57 @@ __device__ __forceinline__ Q4 shfl_down1(const Q4& q) {
     return r;
 }
 
+// Residual of owned cell (kk, j) -> every Butcher output of the stage program
+// (explicit_runge_kutta.py:63-80); on the last stage also the cell's CFL quantity and realizability.
+template <bool CART>
+__device__ __forceinline__ void emit_row(const StageArgs& A, const DevBlock& B, int kk, int j, double dt, const Q4& Res,
+                                         double& lane_dt, bool& lane_bad) {
+    const long idx = B.off + 4 * ((long)kk * B.nx + j);
+    if (A.Rout) st4(A.Rout + idx, Res);
+#pragma unroll
+    for (int m = 0; m < MAX_OUT; ++m) {
+        if (m >= A.nout) break;
+        const Q4 s = ld4(A.src[m] + idx);
+        Q4 o;
+        const double am = A.a[m];
+        if (am == 0.0) {
+            o = s;
+        } else {
+            const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o.v[c] = s.v[c] + cdt * Res.v[c];
+        }
+        st4(A.dst[m] + idx, o);
+        if (m == A.state_out && A.dtmin != nullptr) {
+            double dx, dy;
+            cell_dxdy<CART>(B, kk, j, dx, dy);
+            const double t = phx::cell_dt(o, dx, dy, A.ph);
+            if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
+            else lane_dt = fmin(lane_dt, t);
+        }
+    }
+}
+
+// wave-level min of the CFL quantities and any-unrealizable of one work item -> per-case atomics
+__device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& B, int lane, double lane_dt,
+                                            bool lane_bad) {
+    if (A.dtmin != nullptr) {
+        double m = lane_dt;
+#pragma unroll
+        for (int o = WAVE / 2; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, WAVE));
+        const bool anybad = __any(lane_bad);
+        if (lane == 0) {
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
+            if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // one work item of one RK stage
 // ---------------------------------------------------------------------------------------------
-template <bool CART, int FLUX>
+template <bool CART, int FLUX, bool GEN>
 __device__ void stage_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     // a case that became unrealizable is frozen: the reference aborts that run (Euler2D.py:144-152)
@@ -113,7 +162,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
         r.E = r.W = r.N = r.S = phx::q4zero();
         if (real && recon) {
             const phx::CellGeo g = cell_geo<CART>(B, k, j);
-            r = phx::reconstruct(Uc, Uw, Ue, Um, Up, g);
+            r = phx::reconstruct<GEN>(Uc, Uw, Ue, Um, Up, g, ph.limiter);
         }
         const Q4 qsend = (j == -1) ? Uc : r.E;
         const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 (ghost value for j == 0)
@@ -137,30 +186,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
                 Q4 Res;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * F.v[c]))) / prevA;
-                const long idx = B.off + 4 * ((long)kk * nx + j);
-                if (A.Rout) st4(A.Rout + idx, Res);
-#pragma unroll
-                for (int m = 0; m < MAX_OUT; ++m) {
-                    if (m >= A.nout) break;
-                    const Q4 s = ld4(A.src[m] + idx);
-                    Q4 o;
-                    const double am = A.a[m];
-                    if (am == 0.0) {
-                        o = s;
-                    } else {
-                        const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) o.v[c] = s.v[c] + cdt * Res.v[c];
-                    }
-                    st4(A.dst[m] + idx, o);
-                    if (m == A.state_out && A.dtmin != nullptr) {
-                        double dx, dy;
-                        cell_dxdy<CART>(B, kk, j, dx, dy);
-                        const double t = phx::cell_dt(o, dx, dy, ph);
-                        if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
-                        else lane_dt = fmin(lane_dt, t);
-                    }
-                }
+                emit_row<CART>(A, B, kk, j, dt, Res, lane_dt, lane_bad);
             }
             if (xrow) {
                 double LW, LE, LS, LN, Ac;
@@ -177,17 +203,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
         Uc = Up;
     }
 
-    if (A.dtmin != nullptr) {
-        double m = lane_dt;
-#pragma unroll
-        for (int o = WAVE / 2; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o, WAVE));
-        const bool anybad = __any(lane_bad);
-        if (lane == 0) {
-            const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
-            if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
-        }
-    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
 // Work distribution: 8 XCDs receive workgroups round-robin; each XCD gets a contiguous range of the
@@ -207,11 +223,129 @@ __device__ __forceinline__ void item_range(int n, int& lo, int& hi, int& first, 
 }
 
 // MINW: minimum waves per SIMD the register allocation must allow (2 or 3; tuned per workload)
-template <bool CART, int FLUX, int MINW>
+// GEN: limiter other than Venkatakrishnan, selected at run time
+template <bool CART, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX, GEN>(A, A.items[i]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// split stage: for small meshes, where short rolling-row items would spend most of their work
+// on the halo rows they reconstruct again, the stage runs as two kernels: every cell is
+// reconstructed exactly once (k_recon_x, with the x-faces), then the y-faces, residual and
+// Butcher updates run in a second rolling pass that reads the stored N/S states (k_yflux).
+// Same arithmetic, same association order as stage_item.
+// ---------------------------------------------------------------------------------------------
+constexpr int STRIP_A = WAVE - 2;  // k_recon_x: owned columns per wave (one halo lane each side)
+constexpr int STRIP_B = WAVE;      // k_yflux: y-faces need no column halo
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int nx = B.nx;
+    const int j = it.j0 - 1 + threadIdx.x % WAVE;
+    const bool valid = (j >= -1) && (j <= nx);
+    const bool inner = (j >= 0) && (j < nx);
+    const bool gcol = (j == -1) || (j == nx);
+    const bool recon = inner && (j <= it.j0 + STRIP_A);
+    const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP_A, nx));
+    const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP_A);
+    const Phys ph = A.ph;
+
+    Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
+    if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, it.r0 - 1, j);
+    if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, it.r0, j);
+    for (int k = it.r0; k < it.r1; ++k) {
+        if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+        else if (gcol && k + 1 < it.r1) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+        const Q4 Uw = shfl_up1(Uc);
+        const Q4 Ue = shfl_down1(Uc);
+        phx::Recon r;
+        r.E = r.W = r.N = r.S = phx::q4zero();
+        if (recon) {
+            const phx::CellGeo g = cell_geo<CART>(B, k, j);
+            r = phx::reconstruct<GEN>(Uc, Uw, Ue, Um, Up, g, ph.limiter);
+        }
+        const Q4 qsend = (j == -1) ? Uc : r.E;
+        const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 (ghost value for j == 0)
+        Q4 FW = phx::q4zero();
+        if (xface) {
+            const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
+            const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
+            FW = xflux<CART, FLUX>(B, L, R, k, j, ph);
+        }
+        const Q4 FE = shfl_down1(FW);
+        if (owner) {
+            double LW, LE, LS, LN, Ac;
+            cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+            Q4 part;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) part.v[c] = LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c]);
+            const long cell = (long)k * nx + j;
+            st4(A.p1 + B.off + 4 * cell, part);
+            st4(A.sn + 2 * B.off + 8 * cell, r.N);
+            st4(A.sn + 2 * B.off + 8 * cell + 4, r.S);
+        }
+        Um = Uc;
+        Uc = Up;
+    }
+}
+
+template <bool CART, int FLUX>
+__device__ void yflux_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = B.nx, ny = B.ny;
+    const int j = it.j0 + lane;
+    const bool owner = j < nx;
+    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const Phys ph = A.ph;
+    const double* sn = A.sn + 2 * B.off;
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    if (owner) {
+        Q4 Fprev = phx::q4zero();
+        for (int f = it.r0; f <= it.r1; ++f) {
+            // face between rows f-1 and f
+            Q4 L, R;
+            if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
+            else L = ld4(sn + 8 * ((long)(f - 1) * nx + j));
+            if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
+            else R = ld4(sn + 8 * ((long)f * nx + j) + 4);
+            const Q4 F = yflux<CART, FLUX>(B, L, R, f, j, ph);
+            if (f > it.r0) {
+                const int kk = f - 1;
+                double LW, LE, LS, LN, Ac;
+                cell_L<CART>(B, kk, j, LW, LE, LS, LN, Ac);
+                const Q4 p = ld4(A.p1 + B.off + 4 * ((long)kk * nx + j));
+                Q4 Res;
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * Fprev.v[c])) - LN * (2.0 * F.v[c]))) / Ac;
+                emit_row<CART>(A, B, kk, j, dt, Res, lane_dt, lane_bad);
+            }
+            Fprev = F;
+        }
+    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+template <bool CART, int FLUX, int MINW, bool GEN>
+__global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, A.items[i]);
+}
+
+template <bool CART, int FLUX, int MINW>
+__global__ void __launch_bounds__(WAVE, MINW) k_yflux(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) yflux_item<CART, FLUX>(A, A.items[i]);
 }
 
 // standalone CFL reduction over the owned cells of every work item
@@ -219,6 +353,7 @@ struct DtArgs {
     const DevBlock* blocks;
     const WorkItem* items;
     int nitems;
+    int strip;  // owned columns per item of this work list
     const double* U;
     unsigned long long* dtmin;
     int* bad;
@@ -235,7 +370,7 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
         const WorkItem it = A.items[i];
         const DevBlock& B = A.blocks[it.block];
         const int j = it.j0 + lane;
-        const bool own = (lane < STRIP) && (j < B.nx);
+        const bool own = (lane < A.strip) && (j < B.nx);
         double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
         bool lane_bad = false;
         if (own) {
@@ -378,28 +513,45 @@ int build_programs(phx_plan* P, int S, const double* a) {
     return PHX_OK;
 }
 
+int upload_items(std::vector<WorkItem> (&items)[2], WorkItem* (&ditems)[2], int (&grid)[2], int grid_waves) {
+    for (int kind = 0; kind < 2; ++kind) {
+        const size_t n = items[kind].size();
+        if (n) {
+            PHX_CK(hipMalloc(&ditems[kind], n * sizeof(WorkItem)));
+            PHX_CK(hipMemcpy(ditems[kind], items[kind].data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
+        }
+        grid[kind] = (int)((grid_waves > 0 && (size_t)grid_waves < n) ? grid_waves : n);
+    }
+    return PHX_OK;
+}
+
+// Work lists: per block, row ranges of `rows` rows times column strips.  The fused variant uses
+// STRIP owned columns per item; the split variant uses STRIP_A for its reconstruction kernel and
+// STRIP_B with rows_b rows for its y-face kernel.
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
+        P->items_b[kind].clear();
         if (P->ditems[kind]) { (void)hipFree(P->ditems[kind]); P->ditems[kind] = nullptr; }
+        if (P->ditems_b[kind]) { (void)hipFree(P->ditems_b[kind]); P->ditems_b[kind] = nullptr; }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
+    const int strip = P->variant == 1 ? STRIP_A : STRIP;
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
         const int kind = d.geom_kind == PHX_GEOM_CARTESIAN ? 1 : 0;
         for (int r0 = 0; r0 < d.ny; r0 += rows)
-            for (int j0 = 0; j0 < d.nx; j0 += STRIP)
+            for (int j0 = 0; j0 < d.nx; j0 += strip)
                 P->items[kind].push_back({b, j0, r0, std::min(r0 + rows, d.ny)});
+        if (P->variant == 1)
+            for (int r0 = 0; r0 < d.ny; r0 += P->rows_b)
+                for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
+                    P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
     }
-    for (int kind = 0; kind < 2; ++kind) {
-        const size_t n = P->items[kind].size();
-        if (n) {
-            PHX_CK(hipMalloc(&P->ditems[kind], n * sizeof(WorkItem)));
-            PHX_CK(hipMemcpy(P->ditems[kind], P->items[kind].data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
-        }
-        const int gw = P->grid_waves;
-        P->grid[kind] = (int)((gw > 0 && (size_t)gw < n) ? gw : n);
-    }
+    int rc = upload_items(P->items, P->ditems, P->grid, P->grid_waves);
+    if (rc) return rc;
+    rc = upload_items(P->items_b, P->ditems_b, P->grid_b, P->grid_waves);
+    if (rc) return rc;
     if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
     if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
     P->graph_U = nullptr;
@@ -418,33 +570,76 @@ StageArgs base_args(phx_plan* P) {
     return A;
 }
 
-void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
-    for (int kind = 0; kind < 2; ++kind) {
-        const int n = (int)P->items[kind].size();
-        if (!n) continue;
-        A.items = P->ditems[kind];
-        A.nitems = n;
-        dim3 grid(P->grid[kind]), block(WAVE);
-        const bool w3 = P->waves_per_simd == 3;
-        const bool roe = P->flux == PHX_FLUX_ROE;
-        if (kind == 1) {
-            if (roe) {
-                if (w3) hipLaunchKernelGGL((k_stage<true, 0, 3>), grid, block, 0, st, A);
-                else hipLaunchKernelGGL((k_stage<true, 0, 2>), grid, block, 0, st, A);
-            } else {
-                if (w3) hipLaunchKernelGGL((k_stage<true, 1, 3>), grid, block, 0, st, A);
-                else hipLaunchKernelGGL((k_stage<true, 1, 2>), grid, block, 0, st, A);
-            }
-        } else {
-            if (roe) {
-                if (w3) hipLaunchKernelGGL((k_stage<false, 0, 3>), grid, block, 0, st, A);
-                else hipLaunchKernelGGL((k_stage<false, 0, 2>), grid, block, 0, st, A);
-            } else {
-                if (w3) hipLaunchKernelGGL((k_stage<false, 1, 3>), grid, block, 0, st, A);
-                else hipLaunchKernelGGL((k_stage<false, 1, 2>), grid, block, 0, st, A);
-            }
-        }
+// Launchers of the three stage kernel families.  M = requested waves per SIMD, G = generic
+// limiter.  The instantiated register budgets are limited to those that compile without heavy
+// spilling: fused <= 3 (2 for the generic limiter), reconstruction 3-4 (3 generic), y-faces 4.
+struct k_stage_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        constexpr int MM = G ? 2 : (M > 3 ? 3 : M);
+        hipLaunchKernelGGL((k_stage<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
     }
+};
+struct k_recon_x_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        constexpr int MM = G ? 3 : (M < 3 ? 3 : M);
+        hipLaunchKernelGGL((k_recon_x<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
+    }
+};
+struct k_yflux_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yflux<C, F, 4>), g, dim3(WAVE), 0, st, A);
+    }
+};
+
+// one launch of kernel family T over the items of one geometry kind; minw = register budget in
+// waves per SIMD (2, 3 or 4)
+template <class T>
+void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items, int nitems, int grid, int minw,
+                 hipStream_t st) {
+    A.items = items;
+    A.nitems = nitems;
+    const dim3 g(grid);
+    const bool roe = P->flux == PHX_FLUX_ROE;
+    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN;
+#define PHX_LAUNCH_M(C, F)                                                                       \
+    do {                                                                                         \
+        if (gen) T::template launch<C, F, 2, true>(g, st, A);                                    \
+        else if (minw == 4) T::template launch<C, F, 4, false>(g, st, A);                        \
+        else if (minw == 3) T::template launch<C, F, 3, false>(g, st, A);                        \
+        else T::template launch<C, F, 2, false>(g, st, A);                                       \
+    } while (0)
+    if (kind == 1) {
+        if (roe) PHX_LAUNCH_M(true, 0);
+        else PHX_LAUNCH_M(true, 1);
+    } else {
+        if (roe) PHX_LAUNCH_M(false, 0);
+        else PHX_LAUNCH_M(false, 1);
+    }
+#undef PHX_LAUNCH_M
+}
+
+void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
+    if (P->variant == 1) {
+        A.sn = P->sn;
+        A.p1 = P->p1;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items[kind].empty())
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                           P->waves_per_simd, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_b[kind].empty())
+                launch_kind<k_yflux_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
+                                         4, st);
+        return;
+    }
+    // the fused kernel needs more than 128 registers: at most 3 waves per SIMD
+    for (int kind = 0; kind < 2; ++kind)
+        if (!P->items[kind].empty())
+            launch_kind<k_stage_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                     std::min(P->waves_per_simd, 3), st);
 }
 
 void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
@@ -452,7 +647,8 @@ void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
-        DtArgs D{P->dblocks, P->ditems[kind], n, U, P->dtmin, P->bad, P->step, P->ph};
+        DtArgs D{P->dblocks, P->ditems[kind], n, P->variant == 1 ? STRIP_A : STRIP, U, P->dtmin, P->bad, P->step,
+                 P->ph};
         if (kind == 1) hipLaunchKernelGGL(k_dt<true>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
         else hipLaunchKernelGGL(k_dt<false>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
     }
@@ -499,8 +695,8 @@ int phx_plan_destroy(phx_plan* P) {
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
-    void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->sbuf[0], P->sbuf[1], P->dt, P->t,
-                    P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
+    void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->sn, P->p1,
+                    P->sbuf[0], P->sbuf[1], P->dt, P->t, P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int m = 0; m < MAX_ACC; ++m)
@@ -517,7 +713,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     if (D->ncases < 1) return fail(PHX_E_INVALID, "ncases must be >= 1");
     if (D->flux != PHX_FLUX_ROE && D->flux != PHX_FLUX_HLLL)
         return fail(PHX_E_INVALID, "Flux function type is not available.");
-    if (D->limiter != PHX_LIMITER_VENKATAKRISHNAN) return fail(PHX_E_INVALID, "MUSCL: Slope limiter type not specified.");
+    if (D->limiter < PHX_LIMITER_VENKATAKRISHNAN || D->limiter > PHX_LIMITER_VAN_LEER)
+        return fail(PHX_E_INVALID, "MUSCL: Slope limiter type not specified.");
     if (D->nstages < 1 || D->nstages > PHX_MAX_STAGES || !D->butcher_a)
         return fail(PHX_E_INVALID, "invalid Butcher table");
     if (!D->t_final) return fail(PHX_E_INVALID, "t_final is null");
@@ -571,7 +768,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     P->nblocks = D->nblocks;
     P->ncases = D->ncases;
     P->flux = D->flux;
-    P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl};
+    P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl, D->limiter};
     P->desc.assign(D->blocks, D->blocks + D->nblocks);
     P->state_len = total;
     P->rows_per_item = D->rows_per_item > 0 ? D->rows_per_item : 32;
@@ -769,12 +966,25 @@ int phx_status(phx_plan* P, int32_t* bad_step, int32_t* step, double* t_host, do
 }
 
 int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd) {
-    if (!P || rows_per_item < 1 || grid_waves < 0 || waves_per_simd < 0 || waves_per_simd > 3 || waves_per_simd == 1)
+    if (!P || rows_per_item < 1 || grid_waves < 0 || waves_per_simd < 0 || waves_per_simd > 4 || waves_per_simd == 1)
         return fail(PHX_E_INVALID, "invalid launch parameters");
     PHX_CK(hipDeviceSynchronize());
     P->rows_per_item = rows_per_item;
     P->grid_waves = grid_waves;
     P->waves_per_simd = waves_per_simd == 0 ? 2 : waves_per_simd;
+    return build_items(P);
+}
+
+int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
+    if (!P || variant < 0 || variant > 1 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    PHX_CK(hipDeviceSynchronize());
+    if (variant == 1 && !P->sn) {
+        const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
+        PHX_CK(hipMalloc(&P->sn, 2 * L * sizeof(double)));
+        PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
+    }
+    P->variant = variant;
+    if (rows_b > 0) P->rows_b = rows_b;
     return build_items(P);
 }
 

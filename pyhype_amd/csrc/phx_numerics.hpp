@@ -20,6 +20,7 @@ struct Q4 {
 // g_over_gm1 = gamma/(gamma - 1.0) (fluids/base.py:58-64).
 struct Phys {
     double gamma, gm1, one_over_gm1, g_over_gm1, cfl;
+    int limiter;  // PHX_LIMITER_* (0 = Venkatakrishnan)
 };
 
 struct Prim {
@@ -230,24 +231,51 @@ __host__ __device__ inline double np_minimum(double a, double b) {
     return (a != a) ? a : ((b != b) ? b : ((a <= b) ? a : b));
 }
 
-// Venkatakrishnan._venkata (limiters/limiters.py:38-49) on the slope of one face, with the slope
-// of SlopeLimiter._compute_slope (limiters/base.py:189-221).
-__host__ __device__ inline double venkat_face(double dmax, double dmin, double d) {
+// Slope of one face, SlopeLimiter._compute_slope (limiters/base.py:189-221): dmax/d for d > 0,
+// dmin/d for d < 0, 1 otherwise.
+__host__ __device__ inline double face_slope(double dmax, double dmin, double d) {
     const double num = (d > 0.0) ? dmax : dmin;
-    const double s = (d > 0.0 || d < 0.0) ? num / d : 1.0;
+    return (d > 0.0 || d < 0.0) ? num / d : 1.0;
+}
+
+// Limiter functions of limiters/limiters.py:25-62 (slope**2 is s*s).
+__host__ __device__ inline double venkatakrishnan_fn(double s) {
     const double s2 = s * s;
     return (s2 + 2.0 * s) / (s2 + s + 2.0);
+}
+__host__ __device__ inline double limiter_fn(int lim, double s) {
+    switch (lim) {
+        case 1: return np_minimum(1.0, s);                                    // BarthJespersen
+        case 2: { const double s2 = s * s; return (s2 + s) / (s2 + 1.0); }   // VanAlbada
+        case 3: return (fabs(s) + s) / (s + 1.0);                             // VanLeer
+        default: return venkatakrishnan_fn(s);                                 // Venkatakrishnan
+    }
+}
+
+// Limiter value of one face.  GEN = false: Venkatakrishnan (the default, compiled without the
+// limiter switch); GEN = true: the limiter `lim`.
+template <bool GEN>
+__host__ __device__ inline double limit_face(double dmax, double dmin, double d, int lim) {
+    const double s = face_slope(dmax, dmin, d);
+    return GEN ? limiter_fn(lim, s) : venkatakrishnan_fn(s);
+}
+
+// Venkatakrishnan._venkata (limiters/limiters.py:38-49) on the slope of one face.
+__host__ __device__ inline double venkat_face(double dmax, double dmin, double d) {
+    return limit_face<false>(dmax, dmin, d, 0);
 }
 
 // One cell of the second-order reconstruction:
 //  * interface averages (blocks/quad_block.py:181-218),
 //  * Green-Gauss gradient (gradients/greengauss.py:134-155),
 //  * high-order term gx*dx + gy*dy (blocks/base.py:282-289),
-//  * Venkatakrishnan limiter with the 5-point min/max incl. ghosts (limiters/base.py:47-187),
+//  * slope limiter with the 5-point min/max incl. ghosts (limiters/base.py:47-187): Venkatakrishnan,
+//    or with GEN the limiter `lim` (BarthJespersen, VanAlbada, VanLeer),
 //  * limited face values U + phi*hot (fvm/SecondOrderMUSCL.py:106-126).
 // U is the cell, Uw/Ue/Us/Un its west/east/south/north neighbours (ghost values at block edges).
+template <bool GEN = false>
 __host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
-                                             const Q4& Un, const CellGeo& g) {
+                                             const Q4& Un, const CellGeo& g, int lim = 0) {
     Recon r;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -267,10 +295,10 @@ __host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4
         double mn = fmin(fmin(fmin(fmin(u, Uw.v[k]), Ue.v[k]), Us.v[k]), Un.v[k]);
         const double dmax = mx - u;
         const double dmin = mn - u;
-        const double pE = venkat_face(dmax, dmin, (u + hE) - u);
-        const double pW = venkat_face(dmax, dmin, (u + hW) - u);
-        const double pN = venkat_face(dmax, dmin, (u + hN) - u);
-        const double pS = venkat_face(dmax, dmin, (u + hS) - u);
+        const double pE = limit_face<GEN>(dmax, dmin, (u + hE) - u, lim);
+        const double pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
+        const double pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
+        const double pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
         double phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
         phi = (phi < 0.0) ? 0.0 : phi;                  // np.where(phi < 0, 0, phi)
         r.E.v[k] = u + phi * hE;

@@ -71,6 +71,13 @@ struct phx_plan {
     std::vector<phx::host::WorkItem> items[2];  // [0] general, [1] cartesian
     phx::host::WorkItem* ditems[2] = {nullptr, nullptr};
     int grid[2] = {0, 0};
+    // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, then y-faces + update)
+    int variant = 0, rows_b = 8;
+    std::vector<phx::host::WorkItem> items_b[2];  // second kernel of the split variant
+    phx::host::WorkItem* ditems_b[2] = {nullptr, nullptr};
+    int grid_b[2] = {0, 0};
+    double* sn = nullptr;  // split variant: N/S states per cell
+    double* p1 = nullptr;  // split variant: x-face part of the residual per cell
     std::vector<phx::host::StageProg> progs;
     std::vector<double> butcher;  // nstages x nstages, row-major
     int nstages = 0;

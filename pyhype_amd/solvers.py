@@ -64,7 +64,7 @@ def validate_config(config) -> List[List[float]]:
         raise ValueError(f"Gradient type {config.fvm_gradient_type} is not available.")
     if config.fvm_flux_function_type not in ("Roe", "HLLL"):
         raise ValueError(f"Flux function type {config.fvm_flux_function_type} is not available.")
-    if config.fvm_slope_limiter_type != "Venkatakrishnan":
+    if config.fvm_slope_limiter_type not in ("VanLeer", "VanAlbada", "Venkatakrishnan", "BarthJespersen"):
         raise ValueError("MUSCL: Slope limiter type not specified.")
     if config.reconstruction_type is not ConservativeState:
         raise ValueError("only ConservativeState reconstruction is implemented")
@@ -94,7 +94,7 @@ class Euler2DBatch:
         self.table = validate_config(c0)
         for c in configs[1:]:
             validate_config(c)
-            for attr in ("fvm_flux_function_type", "time_integrator", "CFL"):
+            for attr in ("fvm_flux_function_type", "fvm_slope_limiter_type", "time_integrator", "CFL"):
                 if getattr(c, attr) != getattr(c0, attr):
                     raise ValueError(f"batched cases must share {attr}")
             if type(c.fluid) is not type(c0.fluid):
@@ -104,7 +104,7 @@ class Euler2DBatch:
         specs = [CaseSpec(mesh_dict(m), c.nx, c.ny, c.t_final * c.fluid.far_field.a) for c, m in zip(configs, meshes)]
         self.plan = DevicePlan(specs, flux=c0.fvm_flux_function_type, table=self.table, fluid=c0.fluid,
                                cfl=c0.CFL, rows_per_item=rows_per_item, grid_waves=grid_waves,
-                               force_general=force_general)
+                               force_general=force_general, limiter=c0.fvm_slope_limiter_type)
         self.tuned = False
         if rows_per_item == 0:
             from . import autotune

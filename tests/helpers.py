@@ -12,7 +12,8 @@ def oracle_for(config, mesh, states0):
     """Oracle Simulation with the same blocks, scheme and initial state as a pyhype_amd solver."""
     sim = O.Simulation(mesh_dict(mesh), config.nx, config.ny, flux=config.fvm_flux_function_type,
                        integrator=butcher_table(config.time_integrator, config), CFL=config.CFL,
-                       t_final=config.t_final, a_inf=config.fluid.far_field.a)
+                       t_final=config.t_final, a_inf=config.fluid.far_field.a,
+                       limiter=config.fvm_slope_limiter_type)
     sim.set_state({n: np.array(a) for n, a in states0.items()})
     return sim
 

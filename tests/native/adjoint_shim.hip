@@ -15,7 +15,8 @@ using phx::host::AdjBlock;
 
 namespace {
 
-Phys make_phys(const double* p) { return Phys{p[0], p[1], p[2], p[3], p[4]}; }
+// p: gamma, gm1, 1/gm1, gamma/gm1, CFL, limiter id
+Phys make_phys(const double* p) { return Phys{p[0], p[1], p[2], p[3], p[4], (int)p[5]}; }
 Q4 q4(const double* p) { Q4 q; for (int k = 0; k < 4; ++k) q.v[k] = p[k]; return q; }
 void put(double* p, const Q4& q) { for (int k = 0; k < 4; ++k) p[k] = q.v[k]; }
 
@@ -47,15 +48,15 @@ HostPlan make_plan(const phx_block_desc* d, int nb) {
     return P;
 }
 
-void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb) {
+void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, int lim) {
     rb.assign((size_t)P.cells * adj::RB, 0.0);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 double* out = rb.data() + adj::RB * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true>(P.blocks.data(), B, U, k, j, out);
-                else adj::recon_cell<false>(P.blocks.data(), B, U, k, j, out);
+                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true>(P.blocks.data(), B, U, k, j, out, lim);
+                else adj::recon_cell<false>(P.blocks.data(), B, U, k, j, out, lim);
             }
     }
 }
@@ -63,7 +64,7 @@ void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb) {
 template <int FLUX>
 void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
     std::vector<double> rb, flux((size_t)P.faces * 4);
-    recon_all(P, U, rb);
+    recon_all(P, U, rb, ph.limiter);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -85,7 +86,7 @@ void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
 template <int FLUX>
 void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, double* lamU, const Phys& ph) {
     std::vector<double> rb, fadj((size_t)P.faces * adj::FA), slot((size_t)P.cells * adj::SLOT, 0.0);
-    recon_all(P, U, rb);
+    recon_all(P, U, rb, ph.limiter);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -102,8 +103,10 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 double* out = slot.data() + adj::SLOT * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN) adj::cell_adj<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out);
-                else adj::cell_adj<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out);
+                if (B.kind == PHX_GEOM_CARTESIAN)
+                    adj::cell_adj<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph.limiter);
+                else
+                    adj::cell_adj<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph.limiter);
             }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
@@ -160,16 +163,16 @@ void shim_face_flux_vjp(int flux, int n, const double* qL, const double* qR, con
 
 // one cell: U5 = (cell, W, E, S, N) x 4, g = 17 doubles in CellGeo order, bQ = (E, W, N, S) x 4,
 // out = (cell, W, E, S, N) x 4
-void shim_reconstruct_vjp(const double* U5, const double* g, const double* bQ, double* out) {
+void shim_reconstruct_vjp(const double* U5, const double* g, const double* bQ, double* out, int lim) {
     CellGeo G{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], g[9], g[10], g[11], g[12], g[13], g[14], g[15], g[16]};
     Recon b;
     b.E = q4(bQ); b.W = q4(bQ + 4); b.N = q4(bQ + 8); b.S = q4(bQ + 12);
-    const ReconBar r = reconstruct_vjp(q4(U5), q4(U5 + 4), q4(U5 + 8), q4(U5 + 12), q4(U5 + 16), G, b);
+    const ReconBar r = reconstruct_vjp(q4(U5), q4(U5 + 4), q4(U5 + 8), q4(U5 + 12), q4(U5 + 16), G, b, lim);
     put(out, r.c); put(out + 4, r.w); put(out + 8, r.e); put(out + 12, r.s); put(out + 16, r.n);
 }
-void shim_reconstruct1(const double* U5, const double* g, double* out) {
+void shim_reconstruct1(const double* U5, const double* g, double* out, int lim) {
     CellGeo G{g[0], g[1], g[2], g[3], g[4], g[5], g[6], g[7], g[8], g[9], g[10], g[11], g[12], g[13], g[14], g[15], g[16]};
-    const Recon r = reconstruct(q4(U5), q4(U5 + 4), q4(U5 + 8), q4(U5 + 12), q4(U5 + 16), G);
+    const Recon r = reconstruct<true>(q4(U5), q4(U5 + 4), q4(U5 + 8), q4(U5 + 12), q4(U5 + 16), G, lim);
     put(out, r.E); put(out + 4, r.W); put(out + 8, r.N); put(out + 12, r.S);
 }
 }

@@ -38,6 +38,16 @@ void shim_reconstruct(int n, const double* U, const double* Uw, const double* Ue
         put(out + 16 * i, r.E); put(out + 16 * i + 4, r.W); put(out + 16 * i + 8, r.N); put(out + 16 * i + 12, r.S);
     }
 }
+// as shim_reconstruct with limiter `lim` (0 Venkatakrishnan, 1 BarthJespersen, 2 VanAlbada, 3 VanLeer)
+void shim_reconstruct_lim(int lim, int n, const double* U, const double* Uw, const double* Ue, const double* Us,
+                          const double* Un, const double* g, double* out) {
+    for (int i = 0; i < n; ++i) {
+        const double* q = g + 17 * i;
+        CellGeo G{q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10], q[11], q[12], q[13], q[14], q[15], q[16]};
+        Recon r = reconstruct<true>(q4(U + 4 * i), q4(Uw + 4 * i), q4(Ue + 4 * i), q4(Us + 4 * i), q4(Un + 4 * i), G, lim);
+        put(out + 16 * i, r.E); put(out + 16 * i + 4, r.W); put(out + 16 * i + 8, r.N); put(out + 16 * i + 12, r.S);
+    }
+}
 void shim_cell_dt(int n, const double* U, const double* dx, const double* dy, double* out, const double* phys) {
     const Phys ph = make_phys(phys);
     for (int i = 0; i < n; ++i) out[i] = cell_dt(q4(U + 4 * i), dx[i], dy[i], ph);

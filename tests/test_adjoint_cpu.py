@@ -25,9 +25,19 @@ def _p(a):
     return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(dp)
 
 
+LIMITER_IDS = {"Venkatakrishnan": 0, "BarthJespersen": 1, "VanAlbada": 2, "VanLeer": 3}
+
+
 def _phys(cfg):
     f = cfg.fluid
-    return np.array([f.gamma(), f.gm1(), f.one_over_gm1(), f.g_over_gm1(), cfg.CFL])
+    return np.array([f.gamma(), f.gm1(), f.one_over_gm1(), f.g_over_gm1(), cfg.CFL,
+                     LIMITER_IDS[cfg.fvm_slope_limiter_type]])
+
+
+def _with_limiter(case, limiter):
+    (cfg, mesh), general = case
+    cfg.fvm_slope_limiter_type = limiter
+    return (cfg, mesh), general
 
 
 CASES = {
@@ -37,6 +47,9 @@ CASES = {
     "oblique_hlll": lambda: (W.oblique_shock(8, 3), False),
     "oblique_hlll_general": lambda: (W.oblique_shock(8, 3), True),
     "skewed": lambda: ((W.explosion_single(8)[0], skewed_mesh()), False),
+    "single_barth": lambda: _with_limiter((W.explosion_single(10), False), "BarthJespersen"),
+    "multiblock_vanalbada": lambda: _with_limiter((W.explosion_multiblock(6, 2, 4), False), "VanAlbada"),
+    "oblique_vanleer": lambda: _with_limiter((W.oblique_shock(8, 3), False), "VanLeer"),
 }
 
 
@@ -84,12 +97,13 @@ def shim(adjoint_shim):
     s.shim_residual_vjp.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
     s.shim_face_flux.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp]
     s.shim_face_flux_vjp.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]
-    s.shim_reconstruct_vjp.argtypes = [dp, dp, dp, dp]
-    s.shim_reconstruct1.argtypes = [dp, dp, dp]
+    s.shim_reconstruct_vjp.argtypes = [dp, dp, dp, dp, C.c_int]
+    s.shim_reconstruct1.argtypes = [dp, dp, dp, C.c_int]
     return s
 
 
-@pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll", "skewed"])
+@pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll", "skewed", "single_barth",
+                                  "multiblock_vanalbada", "oblique_vanleer"])
 def test_elementwise_residual_equals_oracle(shim, name):
     """(Forcing the general path on a Cartesian mesh changes the arithmetic, as it would in the
     reference, so only the meshes' own geometry kinds are compared.)"""
@@ -119,7 +133,7 @@ def _cons(rng, n, mach):
 def test_face_flux_vjp_matches_central_differences(shim, flux, mach):
     rng = np.random.default_rng(10 + flux + int(10 * mach))
     n = 400
-    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8])
+    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8, 0])
     qL, qR = _cons(rng, n, mach), _cons(rng, n, mach)
     qR = qL + 0.3 * (qR - qL)  # nearby states, the usual regime of a face
     lam = rng.standard_normal((n, 4))
@@ -141,8 +155,9 @@ def test_face_flux_vjp_matches_central_differences(shim, flux, mach):
     assert np.median(err) < 1e-8
 
 
-def test_reconstruct_vjp_matches_central_differences(shim):
-    rng = np.random.default_rng(5)
+@pytest.mark.parametrize("lim", [0, 1, 2, 3])
+def test_reconstruct_vjp_matches_central_differences(shim, lim):
+    rng = np.random.default_rng(5 + lim)
     nfail = 0
     for trial in range(300):
         U5 = 1.0 + 0.2 * rng.standard_normal(20)
@@ -157,11 +172,11 @@ def test_reconstruct_vjp_matches_central_differences(shim):
         bQ = rng.standard_normal(16)
         v = rng.standard_normal(20)
         out = np.zeros(20)
-        shim.shim_reconstruct_vjp(_p(U5), _p(g), _p(bQ), _p(out))
+        shim.shim_reconstruct_vjp(_p(U5), _p(g), _p(bQ), _p(out), lim)
 
         def Q(u):
             r = np.zeros(16)
-            shim.shim_reconstruct1(_p(u), _p(g), _p(r))
+            shim.shim_reconstruct1(_p(u), _p(g), _p(r), lim)
             return r
 
         h = 1e-7

@@ -72,8 +72,11 @@ def test_cons_to_prim_and_reflect(numerics_shim):
     np.testing.assert_array_equal(out, ref.reshape(n, 4))
 
 
-def test_reconstruct_matches_oracle_on_block(numerics_shim):
-    """Cell reconstruction inside a random block with walls, using the oracle's geometry."""
+@pytest.mark.parametrize("lim,name", [(-1, "Venkatakrishnan"), (0, "Venkatakrishnan"), (1, "BarthJespersen"),
+                                      (2, "VanAlbada"), (3, "VanLeer")])
+def test_reconstruct_matches_oracle_on_block(numerics_shim, lim, name):
+    """Cell reconstruction inside a random block with walls, using the oracle's geometry; every
+    limiter of the reference (lim = -1: the default-compiled Venkatakrishnan path)."""
     rng = np.random.default_rng(99)
     nx, ny = 13, 9
     for vert in ({"SW": [0, 0], "SE": [10, 0], "NW": [0, 7], "NE": [10, 7]},
@@ -86,7 +89,7 @@ def test_reconstruct_matches_oracle_on_block(numerics_shim):
         O.apply_boundary_condition({0: b})
         gx, gy = O.green_gauss(U, b.ghost, g)
         hot = {d: O.high_order_term(gx, gy, g, d) for d in O.DIRS}
-        phi = O.limiter_phi(U, b.ghost, hot)
+        phi = O.limiter_phi(U, b.ghost, hot, O.LIMITERS[name])
         ref = np.concatenate([U + phi * hot[d] for d in ("E", "W", "N", "S")], axis=2)
         EW = np.concatenate((b.ghost["W"], U, b.ghost["E"]), axis=1)
         NS = np.concatenate((b.ghost["S"], U, b.ghost["N"]), axis=0)
@@ -99,7 +102,10 @@ def test_reconstruct_matches_oracle_on_block(numerics_shim):
                 f["E"].qy - g.yc, f["W"].qy - g.yc, f["N"].qy - g.yc, f["S"].qy - g.yc]
         G = np.stack(cols, axis=-1).reshape(-1, 17)
         out = np.zeros((nx * ny, 16))
-        numerics_shim.shim_reconstruct(nx * ny, dp(U), dp(Uw), dp(Ue), dp(Us), dp(Un), dp(G), dp(out))
+        if lim < 0:
+            numerics_shim.shim_reconstruct(nx * ny, dp(U), dp(Uw), dp(Ue), dp(Us), dp(Un), dp(G), dp(out))
+        else:
+            numerics_shim.shim_reconstruct_lim(lim, nx * ny, dp(U), dp(Uw), dp(Ue), dp(Us), dp(Un), dp(G), dp(out))
         np.testing.assert_array_equal(out.reshape(ny, nx, 16), ref)
 
 

@@ -98,6 +98,37 @@ def test_venkatakrishnan_closed_forms():
     assert np.max(O.venkatakrishnan(grid)) <= O.venkatakrishnan(np.array([smax]))[0] + 1e-15
 
 
+def test_other_limiter_closed_forms():
+    """BarthJespersen min(1, s); VanAlbada (s^2+s)/(s^2+1); VanLeer (|s|+s)/(s+1) = 2s/(s+1) on
+    s >= 0 (limiters/limiters.py:25-62): phi(0) = 0, phi(1) = 1 for all three, symmetric limiters
+    satisfy phi(s)/s = phi(1/s), and the limits at s -> infinity are 1, 1 and 2."""
+    s = np.array([0.0, 0.5, 1.0, 2.0, 1e12])
+    np.testing.assert_array_equal(O.barth_jespersen(s), [0.0, 0.5, 1.0, 1.0, 1.0])
+    np.testing.assert_allclose(O.van_albada(s), [0.0, 0.6, 1.0, 1.2, 1.0], rtol=1e-11)
+    np.testing.assert_allclose(O.van_leer(s), [0.0, 2 / 3, 1.0, 4 / 3, 2.0], rtol=1e-11)
+    r = np.linspace(0.1, 10, 50)
+    for fn in (O.van_albada, O.van_leer):
+        np.testing.assert_allclose(fn(r) / r, fn(1 / r), rtol=1e-12)
+    assert np.isnan(O.barth_jespersen(np.array([np.nan])))[0]          # np.minimum propagates NaN
+
+
+@pytest.mark.parametrize("name,value", [("Venkatakrishnan", 0.75), ("BarthJespersen", 1.0), ("VanAlbada", 1.0),
+                                        ("VanLeer", 1.0)])
+def test_limiter_on_one_dimensional_linear_field(name, value):
+    """x-linear field: E/W faces give s = 2, N/S faces have zero reconstruction delta (s = 1):
+    phi = min(f(2), f(1)) in every interior cell -- 0.75 for Venkatakrishnan, 1 for the others."""
+    nx, ny = 10, 6
+    g = O.block_geometry(RECT, nx, ny)
+    U = np.stack([1.0 + 0.1 * g.xc, 1 + 0 * g.xc, 1 + 0 * g.xc, 5 + 0.2 * g.xc], axis=-1)
+    b = _block_with(U)
+    gx, gy = O.green_gauss(U, b.ghost, g)
+    hot = {d: O.high_order_term(gx, gy, g, d) for d in O.DIRS}
+    phi = O.limiter_phi(U, b.ghost, hot, O.LIMITERS[name])
+    inner = np.s_[1:-1, 1:-1]
+    np.testing.assert_allclose(phi[inner][..., 0], value, rtol=1e-14)
+    np.testing.assert_allclose(phi[inner][..., 3], value, rtol=1e-14)
+
+
 def test_limiter_on_one_dimensional_linear_field_is_three_quarters():
     """x-linear field: E/W faces give s = 2 (phi = 1), N/S faces have zero reconstruction delta
     (s = 1, phi = 0.75): the reference limits every interior cell of a 1-D ramp to 0.75."""
