@@ -238,7 +238,10 @@ __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
 // Butcher updates run in a second rolling pass that reads the stored N/S states (k_yflux).
 // Same arithmetic, same association order as stage_item.
 // ---------------------------------------------------------------------------------------------
-constexpr int STRIP_A = WAVE - 2;  // k_recon_x: owned columns per wave (one halo lane each side)
+// k_recon_x: owned columns per wave.  Two halo lanes per side as in the fused kernel: the halo
+// cell j0-1 is reconstructed (its E state feeds the W face of the first owned cell) and needs its
+// own west neighbour j0-2; likewise on the east side.
+constexpr int STRIP_A = WAVE - 2 * HALO;
 constexpr int STRIP_B = WAVE;      // k_yflux: y-faces need no column halo
 
 template <bool CART, int FLUX, bool GEN>
@@ -246,11 +249,11 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx;
-    const int j = it.j0 - 1 + threadIdx.x % WAVE;
+    const int j = it.j0 - HALO + threadIdx.x % WAVE;
     const bool valid = (j >= -1) && (j <= nx);
     const bool inner = (j >= 0) && (j < nx);
     const bool gcol = (j == -1) || (j == nx);
-    const bool recon = inner && (j <= it.j0 + STRIP_A);
+    const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP_A);
     const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP_A, nx));
     const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP_A);
     const Phys ph = A.ph;

@@ -214,7 +214,7 @@ def test_skewed_multiblock_residual_and_steps_match_oracle():
 
 @pytest.mark.parametrize("case,rows,rows_b,waves", [
     ("single", 1, 4, 3), ("single", 3, 32, 4), ("multiblock", 2, 8, 3), ("oblique", 4, 5, 4),
-    ("skewed", 1, 3, 3)])
+    ("skewed", 1, 3, 3), ("wide", 2, 7, 3), ("wide_multiblock", 3, 16, 4)])
 def test_split_variant_matches_oracle(case, rows, rows_b, waves):
     """The split stage (reconstruction + x-faces, then y-faces + update) gives the oracle's results
     bit for bit: residual, steps (RK4 / SSP-RK2 with the CFL reduction) and the ghost semantics."""
@@ -224,6 +224,10 @@ def test_split_variant_matches_oracle(case, rows, rows_b, waves):
         cfg, mesh = W.explosion_multiblock(14, 2, 4)
     elif case == "oblique":
         cfg, mesh = W.oblique_shock(18, 3)
+    elif case == "wide":            # three column strips per row (interior strip edges)
+        cfg, mesh = W.explosion_single(130)
+    elif case == "wide_multiblock":
+        cfg, mesh = W.explosion_multiblock(66, 2, 2)
     else:
         cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
     s = _solver(cfg, mesh, rows_per_item=rows)
