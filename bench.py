@@ -249,9 +249,12 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
+    split = getattr(plan, "launch", {}).get("variant", 0) == 1
+    kernel = ("k_recon_x + k_yflux (split stage: one RK stage over all blocks)" if split
+              else "k_stage (one RK stage over all blocks)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "k_stage (one RK stage over all blocks)", "stage_ms": round(stage_ms, 5),
+                "kernel": kernel, "stage_ms": round(stage_ms, 5),
                 "alg_bytes_per_cell_stage": b_cell_step / S}
 
     cpu = None

@@ -393,74 +393,86 @@ __host__ __device__ inline double limiter_prime(int lim, double s) {
     }
 }
 
+// Adjoint of reconstruct_var for one conserved variable: bq = adjoints of the limited E, W, N, S
+// values; out = adjoints of (cell, W, E, S, N) values of that variable.
+__host__ __device__ inline void reconstruct_vjp_var(double u, double uw, double ue, double us, double un,
+                                                    const CellGeo& g, const double bq[4], int lim, double out[5]) {
+    const double vals[5] = {u, uw, ue, us, un};
+    const double e = 0.5 * (u + ue), w = 0.5 * (uw + u);
+    const double n = 0.5 * (u + un), s = 0.5 * (us + u);
+    const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+    const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+    const double hx[4] = {g.dxE, g.dxW, g.dxN, g.dxS};
+    const double hy[4] = {g.dyE, g.dyW, g.dyN, g.dyS};
+    double hf[4];
+    for (int f = 0; f < 4; ++f) hf[f] = gx * hx[f] + gy * hy[f];
+    int imax = 0, imin = 0;
+    double mx = vals[0], mn = vals[0];
+    for (int q = 1; q < 5; ++q) {
+        if (vals[q] > mx) { mx = vals[q]; imax = q; }
+        if (vals[q] < mn) { mn = vals[q]; imin = q; }
+    }
+    const double dmax = mx - u, dmin = mn - u;
+    double phif[4], sf[4], df[4];
+    for (int f = 0; f < 4; ++f) {
+        df[f] = (u + hf[f]) - u;
+        sf[f] = face_slope(dmax, dmin, df[f]);
+        phif[f] = limiter_fn(lim, sf[f]);
+    }
+    int fmin = 0;
+    for (int f = 1; f < 4; ++f)
+        if (phif[f] < phif[fmin]) fmin = f;
+    const double phi_raw = phif[fmin];
+    const double phi = (phi_raw < 0.0) ? 0.0 : phi_raw;
+    // Q_f = u + phi h_f
+    double bu = bq[0] + bq[1] + bq[2] + bq[3];
+    double bphi = 0.0, bh[4];
+    for (int f = 0; f < 4; ++f) {
+        bphi += bq[f] * hf[f];
+        bh[f] = phi * bq[f];
+    }
+    double bval[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!(phi_raw < 0.0)) {
+        const double dfs = df[fmin];
+        if (dfs > 0.0 || dfs < 0.0) {
+            const double sv = sf[fmin];
+            const double bsv = bphi * limiter_prime(lim, sv);
+            const double bnum = bsv / dfs;
+            const double bd = -bsv * sv / dfs;
+            if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }
+            else { bval[imin] += bnum; bu -= bnum; }
+            bh[fmin] += bd;
+        }
+    }
+    double bgx = 0.0, bgy = 0.0;
+    for (int f = 0; f < 4; ++f) {
+        bgx += bh[f] * hx[f];
+        bgy += bh[f] * hy[f];
+    }
+    const double be = (bgx * g.xlE + bgy * g.ylE) * g.ainv;
+    const double bw = (bgx * g.xlW + bgy * g.ylW) * g.ainv;
+    const double bn = (bgx * g.xlN + bgy * g.ylN) * g.ainv;
+    const double bs = (bgx * g.xlS + bgy * g.ylS) * g.ainv;
+    bu += 0.5 * (be + bw + bn + bs);
+    out[0] = bu + bval[0];
+    out[1] = 0.5 * bw + bval[1];
+    out[2] = 0.5 * be + bval[2];
+    out[3] = 0.5 * bs + bval[3];
+    out[4] = 0.5 * bn + bval[4];
+}
+
 __host__ __device__ inline ReconBar reconstruct_vjp(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
                                                     const Q4& Un, const CellGeo& g, const Recon& bQ, int lim = 0) {
     ReconBar out;
     for (int k = 0; k < 4; ++k) {
-        const double u = U.v[k];
-        const double vals[5] = {u, Uw.v[k], Ue.v[k], Us.v[k], Un.v[k]};
-        const double e = 0.5 * (u + Ue.v[k]), w = 0.5 * (Uw.v[k] + u);
-        const double n = 0.5 * (u + Un.v[k]), s = 0.5 * (Us.v[k] + u);
-        const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
-        const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
-        const double hx[4] = {g.dxE, g.dxW, g.dxN, g.dxS};
-        const double hy[4] = {g.dyE, g.dyW, g.dyN, g.dyS};
-        double hf[4];
-        for (int f = 0; f < 4; ++f) hf[f] = gx * hx[f] + gy * hy[f];
-        int imax = 0, imin = 0;
-        double mx = vals[0], mn = vals[0];
-        for (int q = 1; q < 5; ++q) {
-            if (vals[q] > mx) { mx = vals[q]; imax = q; }
-            if (vals[q] < mn) { mn = vals[q]; imin = q; }
-        }
-        const double dmax = mx - u, dmin = mn - u;
-        double phif[4], sf[4], df[4];
-        for (int f = 0; f < 4; ++f) {
-            df[f] = (u + hf[f]) - u;
-            sf[f] = face_slope(dmax, dmin, df[f]);
-            phif[f] = limiter_fn(lim, sf[f]);
-        }
-        int fmin = 0;
-        for (int f = 1; f < 4; ++f)
-            if (phif[f] < phif[fmin]) fmin = f;
-        const double phi_raw = phif[fmin];
-        const double phi = (phi_raw < 0.0) ? 0.0 : phi_raw;
         const double bq[4] = {bQ.E.v[k], bQ.W.v[k], bQ.N.v[k], bQ.S.v[k]};
-        // Q_f = u + phi h_f
-        double bu = bq[0] + bq[1] + bq[2] + bq[3];
-        double bphi = 0.0, bh[4];
-        for (int f = 0; f < 4; ++f) {
-            bphi += bq[f] * hf[f];
-            bh[f] = phi * bq[f];
-        }
-        double bval[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (!(phi_raw < 0.0)) {
-            const double dfs = df[fmin];
-            if (dfs > 0.0 || dfs < 0.0) {
-                const double sv = sf[fmin];
-                const double bsv = bphi * limiter_prime(lim, sv);
-                const double bnum = bsv / dfs;
-                const double bd = -bsv * sv / dfs;
-                if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }
-                else { bval[imin] += bnum; bu -= bnum; }
-                bh[fmin] += bd;
-            }
-        }
-        double bgx = 0.0, bgy = 0.0;
-        for (int f = 0; f < 4; ++f) {
-            bgx += bh[f] * hx[f];
-            bgy += bh[f] * hy[f];
-        }
-        const double be = (bgx * g.xlE + bgy * g.ylE) * g.ainv;
-        const double bw = (bgx * g.xlW + bgy * g.ylW) * g.ainv;
-        const double bn = (bgx * g.xlN + bgy * g.ylN) * g.ainv;
-        const double bs = (bgx * g.xlS + bgy * g.ylS) * g.ainv;
-        bu += 0.5 * (be + bw + bn + bs);
-        out.c.v[k] = bu + bval[0];
-        out.w.v[k] = 0.5 * bw + bval[1];
-        out.e.v[k] = 0.5 * be + bval[2];
-        out.s.v[k] = 0.5 * bs + bval[3];
-        out.n.v[k] = 0.5 * bn + bval[4];
+        double o[5];
+        reconstruct_vjp_var(U.v[k], Uw.v[k], Ue.v[k], Us.v[k], Un.v[k], g, bq, lim, o);
+        out.c.v[k] = o[0];
+        out.w.v[k] = o[1];
+        out.e.v[k] = o[2];
+        out.s.v[k] = o[3];
+        out.n.v[k] = o[4];
     }
     return out;
 }

@@ -310,28 +310,42 @@ __device__ void yflux_item(const StageArgs& A, const WorkItem it) {
     const double* sn = A.sn + 2 * B.off;
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
     bool lane_bad = false;
+    Q4 pprev = phx::q4zero();  // x-part of the row whose residual completes at the next face
     if (owner) {
+        // states of face f (between rows f-1 and f); the loads of face f+1 and the x-part of the
+        // next row are issued before the flux of face f so their latency overlaps it
+        auto face_L = [&](int f) -> Q4 {
+            if (f == 0) return outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
+            return ld4(sn + 8 * ((long)(f - 1) * nx + j));
+        };
+        auto face_R = [&](int f, const Q4& L) -> Q4 {
+            if (f == ny) return outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
+            return ld4(sn + 8 * ((long)f * nx + j) + 4);
+        };
         Q4 Fprev = phx::q4zero();
+        Q4 Ln = face_L(it.r0);
+        Q4 Rn = face_R(it.r0, Ln);
+        Q4 pn = ld4(A.p1 + B.off + 4 * ((long)it.r0 * nx + j));
         for (int f = it.r0; f <= it.r1; ++f) {
-            // face between rows f-1 and f
-            Q4 L, R;
-            if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
-            else L = ld4(sn + 8 * ((long)(f - 1) * nx + j));
-            if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
-            else R = ld4(sn + 8 * ((long)f * nx + j) + 4);
+            const Q4 L = Ln, R = Rn, p = pn;
+            if (f < it.r1) {
+                Ln = face_L(f + 1);
+                Rn = face_R(f + 1, Ln);
+                if (f + 1 < it.r1) pn = ld4(A.p1 + B.off + 4 * ((long)(f + 1) * nx + j));
+            }
             const Q4 F = yflux<CART, FLUX>(B, L, R, f, j, ph);
             if (f > it.r0) {
                 const int kk = f - 1;
                 double LW, LE, LS, LN, Ac;
                 cell_L<CART>(B, kk, j, LW, LE, LS, LN, Ac);
-                const Q4 p = ld4(A.p1 + B.off + 4 * ((long)kk * nx + j));
                 Q4 Res;
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
-                    Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * Fprev.v[c])) - LN * (2.0 * F.v[c]))) / Ac;
+                    Res.v[c] = (0.5 * ((pprev.v[c] + LS * (2.0 * Fprev.v[c])) - LN * (2.0 * F.v[c]))) / Ac;
                 emit_row<CART>(A, B, kk, j, dt, Res, lane_dt, lane_bad);
             }
             Fprev = F;
+            pprev = p;
         }
     }
     reduce_item(A, B, lane, lane_dt, lane_bad);

@@ -265,7 +265,7 @@ __host__ __device__ inline double venkat_face(double dmax, double dmin, double d
     return limit_face<false>(dmax, dmin, d, 0);
 }
 
-// One cell of the second-order reconstruction:
+// One cell of the second-order reconstruction (reconstruct_var: one conserved variable):
 //  * interface averages (blocks/quad_block.py:181-218),
 //  * Green-Gauss gradient (gradients/greengauss.py:134-155),
 //  * high-order term gx*dx + gy*dy (blocks/base.py:282-289),
@@ -274,38 +274,44 @@ __host__ __device__ inline double venkat_face(double dmax, double dmin, double d
 //  * limited face values U + phi*hot (fvm/SecondOrderMUSCL.py:106-126).
 // U is the cell, Uw/Ue/Us/Un its west/east/south/north neighbours (ghost values at block edges).
 template <bool GEN = false>
+__host__ __device__ inline void reconstruct_var(double u, double uw, double ue, double us, double un,
+                                                const CellGeo& g, int lim, double& qE, double& qW, double& qN,
+                                                double& qS) {
+    const double e = 0.5 * (u + ue);
+    const double w = 0.5 * (uw + u);
+    const double n = 0.5 * (u + un);
+    const double s = 0.5 * (us + u);
+    const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+    const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+    const double hE = gx * g.dxE + gy * g.dyE;
+    const double hW = gx * g.dxW + gy * g.dyW;
+    const double hN = gx * g.dxN + gy * g.dyN;
+    const double hS = gx * g.dxS + gy * g.dyS;
+    // np.maximum.reduce((U, UW, UE, US, UN)) - U
+    const double mx = fmax(fmax(fmax(fmax(u, uw), ue), us), un);
+    const double mn = fmin(fmin(fmin(fmin(u, uw), ue), us), un);
+    const double dmax = mx - u;
+    const double dmin = mn - u;
+    const double pE = limit_face<GEN>(dmax, dmin, (u + hE) - u, lim);
+    const double pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
+    const double pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
+    const double pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
+    double phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+    phi = (phi < 0.0) ? 0.0 : phi;                  // np.where(phi < 0, 0, phi)
+    qE = u + phi * hE;
+    qW = u + phi * hW;
+    qN = u + phi * hN;
+    qS = u + phi * hS;
+}
+
+template <bool GEN = false>
 __host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
                                              const Q4& Un, const CellGeo& g, int lim = 0) {
     Recon r;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double u = U.v[k];
-        const double e = 0.5 * (u + Ue.v[k]);
-        const double w = 0.5 * (Uw.v[k] + u);
-        const double n = 0.5 * (u + Un.v[k]);
-        const double s = 0.5 * (Us.v[k] + u);
-        const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
-        const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
-        const double hE = gx * g.dxE + gy * g.dyE;
-        const double hW = gx * g.dxW + gy * g.dyW;
-        const double hN = gx * g.dxN + gy * g.dyN;
-        const double hS = gx * g.dxS + gy * g.dyS;
-        // np.maximum.reduce((U, UW, UE, US, UN)) - U
-        double mx = fmax(fmax(fmax(fmax(u, Uw.v[k]), Ue.v[k]), Us.v[k]), Un.v[k]);
-        double mn = fmin(fmin(fmin(fmin(u, Uw.v[k]), Ue.v[k]), Us.v[k]), Un.v[k]);
-        const double dmax = mx - u;
-        const double dmin = mn - u;
-        const double pE = limit_face<GEN>(dmax, dmin, (u + hE) - u, lim);
-        const double pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
-        const double pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
-        const double pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
-        double phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
-        phi = (phi < 0.0) ? 0.0 : phi;                  // np.where(phi < 0, 0, phi)
-        r.E.v[k] = u + phi * hE;
-        r.W.v[k] = u + phi * hW;
-        r.N.v[k] = u + phi * hN;
-        r.S.v[k] = u + phi * hS;
-    }
+    for (int k = 0; k < 4; ++k)
+        reconstruct_var<GEN>(U.v[k], Uw.v[k], Ue.v[k], Us.v[k], Un.v[k], g, lim, r.E.v[k], r.W.v[k], r.N.v[k],
+                             r.S.v[k]);
     return r;
 }
 

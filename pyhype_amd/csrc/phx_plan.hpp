@@ -99,8 +99,8 @@ struct phx_plan {
     bool adj_ready = false;
     long long adj_cells = 0, adj_faces = 0;
     phx::host::AdjBlock* dadj = nullptr;
-    std::vector<phx::host::AdjTile> adj_cell_tiles, adj_face_tiles;
-    phx::host::AdjTile *dadj_cell_tiles = nullptr, *dadj_face_tiles = nullptr;
+    std::vector<phx::host::AdjTile> adj_cell_tiles, adj_cv_tiles, adj_face_tiles;  // cells, cell-variables, faces
+    phx::host::AdjTile *dadj_cell_tiles = nullptr, *dadj_cv_tiles = nullptr, *dadj_face_tiles = nullptr;
     double* adj_rb = nullptr;     // limited face states E, W, N, S of every cell (16 doubles)
     double* adj_face = nullptr;   // adjoints of the left/right states of every face (8 doubles)
     double* adj_slot = nullptr;   // reconstruction adjoints: cell, W, E, S, N (20 doubles per cell)
