@@ -250,7 +250,7 @@ def main():
         except Exception:
             traffic = None
     split = getattr(plan, "launch", {}).get("variant", 0) == 1
-    kernel = ("k_recon_x + k_yflux (split stage: one RK stage over all blocks)" if split
+    kernel = ("k_recon_x + k_yface + k_update (split stage: one RK stage over all blocks)" if split
               else "k_stage (one RK stage over all blocks)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,

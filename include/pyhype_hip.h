@@ -184,12 +184,14 @@ int phx_step_vjp(phx_plan* plan, const double* stages, const double* dt_dev, dou
  * pyhype_amd/autotune.py.  No counterpart in the reference. */
 int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, int32_t waves_per_simd);
 
-/* Stage kernel variant: 0 = fused rolling rows (default; best for large meshes), 1 = split into a
- * reconstruction + x-face kernel and a y-face + residual + update kernel, which reconstructs
- * every cell exactly once (best for small meshes, where short fused items spend most of their
- * work on halo rows); rows_b = rows per work item of the second kernel (0 = keep).  Both variants
- * give identical results.  Rebuilds the work lists; the split variant allocates 12 doubles per
- * cell.  No counterpart in the reference. */
+/* Stage kernel variant: 0 = fused rolling rows (default; best for large meshes), 2 = fused with
+ * workgroups of 4 waves that share their halo rows through LDS (rows_per_item rows per wave),
+ * 1 = split into a
+ * reconstruction + x-face kernel, a y-face kernel and a residual + update kernel, which evaluates
+ * every reconstruction and face flux exactly once (best for small meshes, where short fused items
+ * spend most of their work on halo rows); rows_b = rows per work item of the update kernel
+ * (0 = keep).  Both variants give identical results.  Rebuilds the work lists; the split variant
+ * allocates about 16 doubles per cell.  No counterpart in the reference. */
 int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 
 /* Number of work items and the persistent grid used for the Cartesian and general kernels. */

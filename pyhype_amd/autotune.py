@@ -2,7 +2,8 @@
 
 Fused variant: the work item is a 60-column strip of one block over ``rows_per_item`` rows; each
 item recomputes one reconstruction row above and below its range, so short items waste work while
-long items leave SIMDs idle on small meshes.  Split variant: a reconstruction kernel with no row
+long items leave SIMDs idle on small meshes.  Shared-halo variant: workgroups of 4 waves with
+``rows_per_item`` rows each exchange their boundary rows through LDS.  Split variant: a reconstruction kernel with no row
 halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per item).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
@@ -60,10 +61,12 @@ def _time_steps(plan, steps: int) -> float:
 
 def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: Sequence[int] = (2, 3),
                split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (4, 8, 16, 32),
-               split_waves: Sequence[int] = (3, 4)):
-    """The default search space: fused shapes, then split shapes."""
+               split_waves: Sequence[int] = (3, 4), shared_rows: Sequence[int] = (1, 2, 3, 4, 6)):
+    """The default search space: fused shapes, shared-halo shapes (rows per wave), split shapes."""
     out = [{"variant": 0, "rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "rows_b": 0}
            for w in waves for g in grids for r in rows]
+    out += [{"variant": 2, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}
+            for r in shared_rows]
     out += [{"variant": 1, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in split_rows for rb in split_rows_b]
     return out

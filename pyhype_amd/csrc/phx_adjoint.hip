@@ -2,8 +2,8 @@
 // (vector-Jacobian products), gfx950 kernels and their C ABI (include/pyhype_hip.h).
 //
 // The reverse pass is deterministic: every adjoint value is produced by exactly one lane from a
-// fixed set of terms (no atomics).  Four element-wise passes per residual VJP, one lane per cell,
-// per (cell, variable) or per face, over launch tiles of at most TILE elements of one block:
+// fixed set of terms (no atomics).  Four element-wise passes per residual VJP, each one lane per
+// cell or face, over launch tiles of at most TILE elements of one block:
 //   k_adj_recon   limited face states of every cell at the linearisation point
 //   k_adj_face    seed of every face from kappa, flux VJP -> left/right state adjoints
 //   k_adj_cell    reconstruction VJP of every cell -> cell + 4 stencil-neighbour adjoints
@@ -47,7 +47,6 @@ __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const
     else phx::adj::face_adj<false, FLUX>(blocks, B, U, rb, O, kappa, f, out, ph);
 }
 
-// one lane per (cell, variable): element e of a tile -> cell e/4, variable e%4
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                     const double* U, double* rb, int lim) {
     const AdjTile t = tiles[blockIdx.x];
@@ -55,11 +54,10 @@ __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, cons
     const DevBlock& B = blocks[t.block];
     const AdjBlock O = offs[t.block];
     const int e = t.start + threadIdx.x;
-    const int c = e >> 2, v = e & 3;
-    const int k = c / B.nx, j = c - k * B.nx;
+    const int k = e / B.nx, j = e - k * B.nx;
     double* out = rb + phx::adj::RB * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell_var<true>(blocks, B, U, k, j, v, out, lim);
-    else phx::adj::recon_cell_var<false>(blocks, B, U, k, j, v, out, lim);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out, lim);
+    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out, lim);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
@@ -69,11 +67,10 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     const DevBlock& B = blocks[t.block];
     const AdjBlock O = offs[t.block];
     const int e = t.start + threadIdx.x;
-    const int c = e >> 2, v = e & 3;
-    const int k = c / B.nx, j = c - k * B.nx;
+    const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj_var<true>(blocks, B, U, O, fadj, k, j, v, out, lim);
-    else phx::adj::cell_adj_var<false>(blocks, B, U, O, fadj, k, j, v, out, lim);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out, lim);
+    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, lim);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
@@ -145,7 +142,6 @@ int adj_setup(phx_plan* P) {
     std::vector<AdjBlock> offs(P->nblocks);
     long long faces = 0;
     P->adj_cell_tiles.clear();
-    P->adj_cv_tiles.clear();
     P->adj_face_tiles.clear();
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
@@ -156,8 +152,7 @@ int adj_setup(phx_plan* P) {
         faces += nf;
         for (long long e = 0; e < nc; e += TILE)
             P->adj_cell_tiles.push_back({b, (int)e, (int)std::min<long long>(TILE, nc - e)});
-        for (long long e = 0; e < 4 * nc; e += TILE)
-            P->adj_cv_tiles.push_back({b, (int)e, (int)std::min<long long>(TILE, 4 * nc - e)});
+
         for (long long e = 0; e < nf; e += TILE)
             P->adj_face_tiles.push_back({b, (int)e, (int)std::min<long long>(TILE, nf - e)});
     }
@@ -166,7 +161,6 @@ int adj_setup(phx_plan* P) {
     int rc;
     if ((rc = upload(offs, &P->dadj))) return rc;
     if ((rc = upload(P->adj_cell_tiles, &P->dadj_cell_tiles))) return rc;
-    if ((rc = upload(P->adj_cv_tiles, &P->dadj_cv_tiles))) return rc;
     if ((rc = upload(P->adj_face_tiles, &P->dadj_face_tiles))) return rc;
     const size_t L = (size_t)P->state_len;
     PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * sizeof(double)));
@@ -181,8 +175,7 @@ int adj_setup(phx_plan* P) {
 void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, double* lamU, hipStream_t st) {
     const dim3 blk(TILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
-    const dim3 gv((unsigned)P->adj_cv_tiles.size());
-    hipLaunchKernelGGL(k_adj_recon, gv, blk, 0, st, P->dblocks, P->dadj, P->dadj_cv_tiles, U, P->adj_rb,
+    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
                        P->ph.limiter);
     if (P->flux == PHX_FLUX_ROE)
         hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
@@ -190,7 +183,7 @@ void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, doub
     else
         hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
                            P->adj_face, P->ph);
-    hipLaunchKernelGGL(k_adj_cell, gv, blk, 0, st, P->dblocks, P->dadj, P->dadj_cv_tiles, U, P->adj_face,
+    hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
                        P->adj_slot, P->ph.limiter);
     hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
                        lamU);
@@ -204,14 +197,14 @@ void launch_comb(phx_plan* P, const CombArgs& A, hipStream_t st) {
 }  // namespace
 
 void phx::host::adjoint_release(phx_plan* P) {
-    void* ptrs[] = {P->dadj, P->dadj_cell_tiles, P->dadj_cv_tiles, P->dadj_face_tiles, P->adj_rb, P->adj_face,
-                    P->adj_slot, P->adj_kappa};
+    void* ptrs[] = {P->dadj, P->dadj_cell_tiles, P->dadj_face_tiles, P->adj_rb, P->adj_face, P->adj_slot,
+                    P->adj_kappa};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int s = 0; s < PHX_MAX_STAGES; ++s)
         if (P->adj_lam[s]) (void)hipFree(P->adj_lam[s]);
     P->dadj = nullptr;
-    P->dadj_cell_tiles = P->dadj_cv_tiles = P->dadj_face_tiles = nullptr;
+    P->dadj_cell_tiles = P->dadj_face_tiles = nullptr;
     P->adj_rb = P->adj_face = P->adj_slot = P->adj_kappa = nullptr;
     for (int s = 0; s < PHX_MAX_STAGES; ++s) P->adj_lam[s] = nullptr;
     P->adj_ready = false;

@@ -65,22 +65,19 @@ __host__ __device__ inline bool is_zero(const Q4& a) {
 // variables are independent, and the device runs one lane per cell and variable
 // ---------------------------------------------------------------------------------------------
 template <bool CART>
-__host__ __device__ inline void recon_cell_var(const DevBlock* blocks, const DevBlock& B, const double* U, int k,
-                                               int j, int v, double* rb, int lim) {
+__host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
+                                           double* rb, int lim) {
     const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
     const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
     const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
     const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
     const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
     const CellGeo g = dev::cell_geo<CART>(B, k, j);
-    reconstruct_var<true>(Uc.v[v], Uw.v[v], Ue.v[v], Us.v[v], Un.v[v], g, lim, rb[RB_E + v], rb[RB_W + v],
-                          rb[RB_N + v], rb[RB_S + v]);
-}
-
-template <bool CART>
-__host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
-                                           double* rb, int lim) {
-    for (int v = 0; v < 4; ++v) recon_cell_var<CART>(blocks, B, U, k, j, v, rb, lim);
+    const Recon r = reconstruct<true>(Uc, Uw, Ue, Us, Un, g, lim);
+    st(rb + RB_E, r.E);
+    st(rb + RB_W, r.W);
+    st(rb + RB_N, r.N);
+    st(rb + RB_S, r.S);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -243,43 +240,39 @@ __host__ __device__ inline Q4 outer_own_adj(const DevBlock& B, int side, const Q
     return q4zero();
 }
 
-// cell_adj for one conserved variable v (one lane per cell and variable on the device)
+// pass 3 reverse: adjoints of the four face states of cell (k, j) -> reconstruction VJP (the
+// variables are independent; a variable whose four face adjoints are zero contributes zeros).
 template <bool CART>
-__host__ __device__ inline void cell_adj_var(const DevBlock* blocks, const DevBlock& B, const double* U,
-                                             const AdjBlock& O, const double* fadj, int k, int j, int v, double* slot,
-                                             int lim) {
+__host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock& B, const double* U,
+                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot, int lim) {
     const int nx = B.nx, ny = B.ny;
-    Q4 bE = ld(fadj + FA * xface_index(O, B, k, j + 1));
-    Q4 bW = ld(fadj + FA * xface_index(O, B, k, j) + 4);
-    Q4 bN = ld(fadj + FA * yface_index(O, B, k + 1, j));
-    Q4 bS = ld(fadj + FA * yface_index(O, B, k, j) + 4);
+    Q4 bE = ld(fadj + FA * xface_index(O, B, k, j + 1));          // left state of the east face
+    Q4 bW = ld(fadj + FA * xface_index(O, B, k, j) + 4);          // right state of the west face
+    Q4 bN = ld(fadj + FA * yface_index(O, B, k + 1, j));          // left state of the north face
+    Q4 bS = ld(fadj + FA * yface_index(O, B, k, j) + 4);          // right state of the south face
     if (j == nx - 1) acc(bE, outer_own_adj<CART>(B, PHX_E, ld(fadj + FA * xface_index(O, B, k, nx) + 4), k, nx));
     if (j == 0) acc(bW, outer_own_adj<CART>(B, PHX_W, ld(fadj + FA * xface_index(O, B, k, 0)), k, 0));
     if (k == ny - 1) acc(bN, outer_own_adj<CART>(B, PHX_N, ld(fadj + FA * yface_index(O, B, ny, j) + 4), ny - 1, j));
     if (k == 0) acc(bS, outer_own_adj<CART>(B, PHX_S, ld(fadj + FA * yface_index(O, B, 0, j)), 0, j));
-    const double bq[4] = {bE.v[v], bW.v[v], bN.v[v], bS.v[v]};
-    double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (bq[0] != 0.0 || bq[1] != 0.0 || bq[2] != 0.0 || bq[3] != 0.0) {
-        const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
-        const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
-        const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
-        const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
-        const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
-        const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    for (int q = 0; q < SLOT; ++q) slot[q] = 0.0;
+    if (is_zero(bE) && is_zero(bW) && is_zero(bN) && is_zero(bS)) return;
+    const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
+    const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
+    const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
+    const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
+    const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
+    const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    for (int v = 0; v < 4; ++v) {
+        const double bq[4] = {bE.v[v], bW.v[v], bN.v[v], bS.v[v]};
+        if (bq[0] == 0.0 && bq[1] == 0.0 && bq[2] == 0.0 && bq[3] == 0.0) continue;
+        double o[5];
         reconstruct_vjp_var(Uc.v[v], Uw.v[v], Ue.v[v], Us.v[v], Un.v[v], g, bq, lim, o);
+        slot[SL_C + v] = o[0];
+        slot[SL_W + v] = o[1];
+        slot[SL_E + v] = o[2];
+        slot[SL_S + v] = o[3];
+        slot[SL_N + v] = o[4];
     }
-    slot[SL_C + v] = o[0];
-    slot[SL_W + v] = o[1];
-    slot[SL_E + v] = o[2];
-    slot[SL_S + v] = o[3];
-    slot[SL_N + v] = o[4];
-}
-
-// pass 3 reverse: adjoints of the four face states of cell (k, j) -> reconstruction VJP.
-template <bool CART>
-__host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock& B, const double* U,
-                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot, int lim) {
-    for (int v = 0; v < 4; ++v) cell_adj_var<CART>(blocks, B, U, O, fadj, k, j, v, slot, lim);
 }
 
 // Adjoint of a ghost value of `side` that is a function of the block's own edge cell (wall:

@@ -22,6 +22,7 @@ struct DevBlock {
     int side[4];
     int nbr[4];
     long long off;
+    long long yoff;  // offset (doubles) of the block's (ny+1) x nx x 4 y-face fluxes in the split stage
     const double* geom;
     double wall[8];
 };

@@ -52,8 +52,10 @@ struct StageArgs {
     int* bad;          // per case: first step whose result was unrealizable (INT_MAX = none)
     const int* step;   // index of the step being computed (1-based)
     int skip_dead;     // skip the blocks of cases that failed at an earlier step
+    int rows_wave;     // shared-halo stage: rows per wave
     double* sn;        // split stage: limited N and S states per cell (8 doubles, offset 2*off)
     double* p1;        // split stage: LW*(2 FW) - LE*(2 FE) per cell (4 doubles, state layout)
+    double* yf;        // split stage: y-face fluxes (DevBlock::yoff)
     Phys ph;
 };
 
@@ -232,17 +234,173 @@ __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// shared-halo stage: a workgroup of WGW waves takes WGW * rows_wave consecutive rows of one strip,
+// wave w the rows [a, b) = [r0 + w*rows_wave, ...).  Only the workgroup's outermost waves
+// reconstruct a halo row; the face between two waves of the workgroup is evaluated once, by the
+// upper wave, from the lower wave's last N state passed through LDS, and its flux is passed back.
+// Same arithmetic and association order as stage_item (the first and last rows of a wave complete
+// after the exchange).
+// ---------------------------------------------------------------------------------------------
+constexpr int WGW = 4;
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, Q4* lds_F) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const int nx = B.nx, ny = B.ny;
+    const int R = A.rows_wave;
+    const int a = it.r0 + w * R, b = min(a + R, it.r1);
+    const int wtop = (it.r1 - it.r0 - 1) / R;
+    const bool active = a < it.r1;
+    const bool bottom = (w == 0), top = (w == wtop);
+    const int j = it.j0 - HALO + lane;
+    const bool valid = (j >= -1) && (j <= nx);
+    const bool inner = (j >= 0) && (j < nx);
+    const bool gcol = (j == -1) || (j == nx);
+    const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP);
+    const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP, nx));
+    const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP);
+    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const Phys ph = A.ph;
+
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    // first row of a wave above the bottom: its S face waits for the exchange
+    Q4 first_S = phx::q4zero(), first_part = phx::q4zero(), first_FN = phx::q4zero();
+    double first_LS = 0.0, first_LN = 0.0, first_A = 1.0;
+    // last row of a wave below the top: its N face waits for the exchange
+    Q4 part = phx::q4zero(), QNprev = phx::q4zero();
+    double prevLN = 0.0, prevA = 1.0;
+
+    if (active) {
+        const int kr0 = bottom ? max(a - 1, 0) : a;
+        const int kr1 = top ? min(it.r1, ny) : b - 1;
+        Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
+        if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, kr0 - 1, j);
+        if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, kr0, j);
+        for (int k = kr0; k <= kr1; ++k) {
+            const bool real = (k < ny);
+            if (real) {
+                if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+                else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+            }
+            const Q4 Uw = shfl_up1(Uc);
+            const Q4 Ue = shfl_down1(Uc);
+            phx::Recon r;
+            r.E = r.W = r.N = r.S = phx::q4zero();
+            if (real && recon) {
+                const phx::CellGeo g = cell_geo<CART>(B, k, j);
+                r = phx::reconstruct<GEN>(Uc, Uw, Ue, Um, Up, g, ph.limiter);
+            }
+            const Q4 qsend = (j == -1) ? Uc : r.E;
+            const Q4 QEl = shfl_up1(qsend);
+            const bool xrow = (k >= a) && (k < b);
+            Q4 FW = phx::q4zero();
+            if (xrow && xface) {
+                const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
+                const Q4 R2 = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
+                FW = xflux<CART, FLUX>(B, L, R2, k, j, ph);
+            }
+            const Q4 FE = shfl_down1(FW);
+            if (k >= a && owner) {
+                const bool defer_S = (k == a) && !bottom;
+                Q4 F = phx::q4zero();
+                if (!defer_S) {
+                    const Q4 L = (k == 0) ? outer_state<CART>(B, PHX_S, r.S, Um, k, j) : QNprev;
+                    const Q4 R2 = (k == ny) ? outer_state<CART>(B, PHX_N, QNprev, Uc, k - 1, j) : r.S;
+                    F = yflux<CART, FLUX>(B, L, R2, k, j, ph);
+                    if (k - 1 >= a) {
+                        if (k - 1 == a && !bottom) {
+                            first_FN = F;
+                        } else {
+                            Q4 Res;
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * F.v[c]))) / prevA;
+                            emit_row<CART>(A, B, k - 1, j, dt, Res, lane_dt, lane_bad);
+                        }
+                    }
+                }
+                if (xrow) {
+                    double LW, LE, LS, LN, Ac;
+                    cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+                    if (defer_S) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) first_part.v[c] = LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c]);
+                        first_S = r.S;
+                        first_LS = LS;
+                        first_LN = LN;
+                        first_A = Ac;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            part.v[c] = (LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c])) + LS * (2.0 * F.v[c]);
+                    }
+                    prevLN = LN;
+                    prevA = Ac;
+                }
+            }
+            QNprev = r.N;
+            Um = Uc;
+            Uc = Up;
+        }
+        if (!top && owner) lds_N[w * WAVE + lane] = QNprev;  // N state of row b-1
+    }
+    __syncthreads();
+    Q4 FS_first = phx::q4zero();
+    if (active && !bottom && owner) {
+        // face between a-1 (last row of wave w-1) and a
+        FS_first = yflux<CART, FLUX>(B, lds_N[(w - 1) * WAVE + lane], first_S, a, j, ph);
+        lds_F[w * WAVE + lane] = FS_first;
+    }
+    __syncthreads();
+    if (active && owner) {
+        const bool single = (b - 1 == a);
+        if (!bottom) {
+            const Q4 FN = (single && !top) ? lds_F[(w + 1) * WAVE + lane] : first_FN;
+            Q4 Res;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                Res.v[c] = (0.5 * ((first_part.v[c] + first_LS * (2.0 * FS_first.v[c])) - first_LN * (2.0 * FN.v[c]))) /
+                           first_A;
+            emit_row<CART>(A, B, a, j, dt, Res, lane_dt, lane_bad);
+        }
+        if (!top && !(single && !bottom)) {
+            const Q4 FN = lds_F[(w + 1) * WAVE + lane];
+            Q4 Res;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * FN.v[c]))) / prevA;
+            emit_row<CART>(A, B, b - 1, j, dt, Res, lane_dt, lane_bad);
+        }
+    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+template <bool CART, int FLUX, int MINW, bool GEN>
+__global__ void __launch_bounds__(WGW * WAVE, MINW) k_stage_wg(StageArgs A) {
+    __shared__ Q4 lds_N[WGW * WAVE];
+    __shared__ Q4 lds_F[WGW * WAVE];
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) {
+        stage_wg_item<CART, FLUX, GEN>(A, A.items[i], lds_N, lds_F);
+        __syncthreads();  // the next item reuses the exchange buffers
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // split stage: for small meshes, where short rolling-row items would spend most of their work
-// on the halo rows they reconstruct again, the stage runs as two kernels: every cell is
-// reconstructed exactly once (k_recon_x, with the x-faces), then the y-faces, residual and
-// Butcher updates run in a second rolling pass that reads the stored N/S states (k_yflux).
-// Same arithmetic, same association order as stage_item.
+// on the halo rows they reconstruct again, the stage runs as three kernels: every cell is
+// reconstructed exactly once (k_recon_x, with the x-faces), every y-face flux is evaluated once
+// from the stored N/S states (k_yface, one lane per face), then the residual and Butcher updates
+// (k_update).  Same arithmetic, same association order as stage_item.
 // ---------------------------------------------------------------------------------------------
 // k_recon_x: owned columns per wave.  Two halo lanes per side as in the fused kernel: the halo
 // cell j0-1 is reconstructed (its E state feeds the W face of the first owned cell) and needs its
 // own west neighbour j0-2; likewise on the east side.
 constexpr int STRIP_A = WAVE - 2 * HALO;
-constexpr int STRIP_B = WAVE;      // k_yflux: y-faces need no column halo
+constexpr int STRIP_B = WAVE;      // k_yface, k_update: no column halo
 
 template <bool CART, int FLUX, bool GEN>
 __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
@@ -297,55 +455,52 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
     }
 }
 
+// y-faces of face rows [r0, r1) of a strip of 64 columns (face f between rows f-1 and f; f = 0 and
+// f = ny take the outer states of the S and N boundaries): one flux per lane and face, stored.
 template <bool CART, int FLUX>
-__device__ void yflux_item(const StageArgs& A, const WorkItem it) {
+__device__ void yface_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int nx = B.nx, ny = B.ny;
+    const int j = it.j0 + (int)(threadIdx.x % WAVE);
+    if (j >= nx) return;
+    const Phys ph = A.ph;
+    const double* sn = A.sn + 2 * B.off;
+    for (int f = it.r0; f < it.r1; ++f) {
+        Q4 L, R;
+        if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
+        else L = ld4(sn + 8 * ((long)(f - 1) * nx + j));
+        if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
+        else R = ld4(sn + 8 * ((long)f * nx + j) + 4);
+        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), yflux<CART, FLUX>(B, L, R, f, j, ph));
+    }
+}
+
+// residual ((x-part + LS*(2 F_S)) - LN*(2 F_N))/A of rows [r0, r1) of a 64-column strip, then every
+// Butcher output, the CFL quantity and realizability (as the fused kernel).
+template <bool CART>
+__device__ void update_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int nx = B.nx, ny = B.ny;
+    const int nx = B.nx;
     const int j = it.j0 + lane;
-    const bool owner = j < nx;
     const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
-    const Phys ph = A.ph;
-    const double* sn = A.sn + 2 * B.off;
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
     bool lane_bad = false;
-    Q4 pprev = phx::q4zero();  // x-part of the row whose residual completes at the next face
-    if (owner) {
-        // states of face f (between rows f-1 and f); the loads of face f+1 and the x-part of the
-        // next row are issued before the flux of face f so their latency overlaps it
-        auto face_L = [&](int f) -> Q4 {
-            if (f == 0) return outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
-            return ld4(sn + 8 * ((long)(f - 1) * nx + j));
-        };
-        auto face_R = [&](int f, const Q4& L) -> Q4 {
-            if (f == ny) return outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
-            return ld4(sn + 8 * ((long)f * nx + j) + 4);
-        };
-        Q4 Fprev = phx::q4zero();
-        Q4 Ln = face_L(it.r0);
-        Q4 Rn = face_R(it.r0, Ln);
-        Q4 pn = ld4(A.p1 + B.off + 4 * ((long)it.r0 * nx + j));
-        for (int f = it.r0; f <= it.r1; ++f) {
-            const Q4 L = Ln, R = Rn, p = pn;
-            if (f < it.r1) {
-                Ln = face_L(f + 1);
-                Rn = face_R(f + 1, Ln);
-                if (f + 1 < it.r1) pn = ld4(A.p1 + B.off + 4 * ((long)(f + 1) * nx + j));
-            }
-            const Q4 F = yflux<CART, FLUX>(B, L, R, f, j, ph);
-            if (f > it.r0) {
-                const int kk = f - 1;
-                double LW, LE, LS, LN, Ac;
-                cell_L<CART>(B, kk, j, LW, LE, LS, LN, Ac);
-                Q4 Res;
+    if (j < nx) {
+        const double* yf = A.yf + B.yoff + 4 * (long)j;
+        Q4 FS = ld4(yf + 4 * (long)it.r0 * nx);
+        for (int k = it.r0; k < it.r1; ++k) {
+            const Q4 FN = ld4(yf + 4 * (long)(k + 1) * nx);
+            const Q4 p = ld4(A.p1 + B.off + 4 * ((long)k * nx + j));
+            double LW, LE, LS, LN, Ac;
+            cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+            Q4 Res;
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    Res.v[c] = (0.5 * ((pprev.v[c] + LS * (2.0 * Fprev.v[c])) - LN * (2.0 * F.v[c]))) / Ac;
-                emit_row<CART>(A, B, kk, j, dt, Res, lane_dt, lane_bad);
-            }
-            Fprev = F;
-            pprev = p;
+            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+            emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+            FS = FN;
         }
     }
     reduce_item(A, B, lane, lane_dt, lane_bad);
@@ -358,11 +513,18 @@ __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, A.items[i]);
 }
 
-template <bool CART, int FLUX, int MINW>
-__global__ void __launch_bounds__(WAVE, MINW) k_yflux(StageArgs A) {
+template <bool CART, int FLUX>
+__global__ void __launch_bounds__(WAVE) k_yface(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) yflux_item<CART, FLUX>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) yface_item<CART, FLUX>(A, A.items[i]);
+}
+
+template <bool CART>
+__global__ void __launch_bounds__(WAVE) k_update(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) update_item<CART>(A, A.items[i]);
 }
 
 // standalone CFL reduction over the owned cells of every work item
@@ -543,31 +705,41 @@ int upload_items(std::vector<WorkItem> (&items)[2], WorkItem* (&ditems)[2], int 
 }
 
 // Work lists: per block, row ranges of `rows` rows times column strips.  The fused variant uses
-// STRIP owned columns per item; the split variant uses STRIP_A for its reconstruction kernel and
-// STRIP_B with rows_b rows for its y-face kernel.
+// STRIP owned columns per item; the split variant uses STRIP_A for its reconstruction kernel,
+// STRIP_B with `rows` face rows (ny + 1 of them) for the y-face kernel and STRIP_B with rows_b rows
+// for the update kernel.
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
         P->items_b[kind].clear();
+        P->items_f[kind].clear();
         if (P->ditems[kind]) { (void)hipFree(P->ditems[kind]); P->ditems[kind] = nullptr; }
         if (P->ditems_b[kind]) { (void)hipFree(P->ditems_b[kind]); P->ditems_b[kind] = nullptr; }
+        if (P->ditems_f[kind]) { (void)hipFree(P->ditems_f[kind]); P->ditems_f[kind] = nullptr; }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
     const int strip = P->variant == 1 ? STRIP_A : STRIP;
+    const int item_rows = P->variant == 2 ? WGW * rows : rows;  // shared-halo items: WGW waves of `rows`
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
         const int kind = d.geom_kind == PHX_GEOM_CARTESIAN ? 1 : 0;
-        for (int r0 = 0; r0 < d.ny; r0 += rows)
+        for (int r0 = 0; r0 < d.ny; r0 += item_rows)
             for (int j0 = 0; j0 < d.nx; j0 += strip)
-                P->items[kind].push_back({b, j0, r0, std::min(r0 + rows, d.ny)});
-        if (P->variant == 1)
+                P->items[kind].push_back({b, j0, r0, std::min(r0 + item_rows, d.ny)});
+        if (P->variant == 1) {
+            for (int r0 = 0; r0 <= d.ny; r0 += rows)
+                for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
+                    P->items_f[kind].push_back({b, j0, r0, std::min(r0 + rows, d.ny + 1)});
             for (int r0 = 0; r0 < d.ny; r0 += P->rows_b)
                 for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
                     P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
+        }
     }
     int rc = upload_items(P->items, P->ditems, P->grid, P->grid_waves);
     if (rc) return rc;
     rc = upload_items(P->items_b, P->ditems_b, P->grid_b, P->grid_waves);
+    if (rc) return rc;
+    rc = upload_items(P->items_f, P->ditems_f, P->grid_f, P->grid_waves);
     if (rc) return rc;
     if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
     if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
@@ -587,14 +759,21 @@ StageArgs base_args(phx_plan* P) {
     return A;
 }
 
-// Launchers of the three stage kernel families.  M = requested waves per SIMD, G = generic
-// limiter.  The instantiated register budgets are limited to those that compile without heavy
-// spilling: fused <= 3 (2 for the generic limiter), reconstruction 3-4 (3 generic), y-faces 4.
+// Launchers of the stage kernel families.  M = requested waves per SIMD, G = generic limiter.
+// The instantiated register budgets are limited to those that compile without heavy spilling:
+// fused <= 3 (2 for the generic limiter), reconstruction 3-4 (3 generic); the y-face and update
+// kernels need no limiter and are built once.
 struct k_stage_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
         constexpr int MM = G ? 2 : (M > 3 ? 3 : M);
         hipLaunchKernelGGL((k_stage<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
+    }
+};
+struct k_stage_wg_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_stage_wg<C, F, 2, G>), g, dim3(WGW * WAVE), 0, st, A);
     }
 };
 struct k_recon_x_tag {
@@ -604,10 +783,16 @@ struct k_recon_x_tag {
         hipLaunchKernelGGL((k_recon_x<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
     }
 };
-struct k_yflux_tag {
+struct k_yface_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_yflux<C, F, 4>), g, dim3(WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_yface<C, F>), g, dim3(WAVE), 0, st, A);
+    }
+};
+struct k_update_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_update<C>), g, dim3(WAVE), 0, st, A);
     }
 };
 
@@ -646,10 +831,23 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
             if (!P->items[kind].empty())
                 launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
                                            P->waves_per_simd, st);
+        A.yf = P->yf;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_f[kind].empty())
+                launch_kind<k_yface_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(), P->grid_f[kind],
+                                         2, st);
         for (int kind = 0; kind < 2; ++kind)
             if (!P->items_b[kind].empty())
-                launch_kind<k_yflux_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
-                                         4, st);
+                launch_kind<k_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
+                                          2, st);
+        return;
+    }
+    if (P->variant == 2) {
+        A.rows_wave = P->rows_per_item > 0 ? P->rows_per_item : 1;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items[kind].empty())
+                launch_kind<k_stage_wg_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                            2, st);
         return;
     }
     // the fused kernel needs more than 128 registers: at most 3 waves per SIMD
@@ -712,7 +910,8 @@ int phx_plan_destroy(phx_plan* P) {
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
-    void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->sn, P->p1,
+    void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->ditems_f[0],
+                    P->ditems_f[1], P->sn, P->p1, P->yf,
                     P->sbuf[0], P->sbuf[1], P->dt, P->t, P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -811,6 +1010,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         h.nx = d.nx; h.ny = d.ny; h.case_id = d.case_id; h.kind = d.geom_kind;
         for (int s = 0; s < 4; ++s) { h.side[s] = d.side_type[s]; h.nbr[s] = d.neighbor[s]; }
         h.off = d.state_offset;
+        h.yoff = P->yf_len;
+        P->yf_len += 4LL * (d.ny + 1) * d.nx;
         h.geom = P->dgeom + goff[b];
         if (d.geom_kind == PHX_GEOM_CARTESIAN)
             for (int q = 0; q < 8; ++q) h.wall[q] = d.geom[q];
@@ -993,12 +1194,13 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 1 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 2 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     PHX_CK(hipDeviceSynchronize());
     if (variant == 1 && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
         PHX_CK(hipMalloc(&P->sn, 2 * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
+        PHX_CK(hipMalloc(&P->yf, (size_t)std::max<long long>(P->yf_len, 4) * sizeof(double)));
     }
     P->variant = variant;
     if (rows_b > 0) P->rows_b = rows_b;

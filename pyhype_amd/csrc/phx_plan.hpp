@@ -71,13 +71,18 @@ struct phx_plan {
     std::vector<phx::host::WorkItem> items[2];  // [0] general, [1] cartesian
     phx::host::WorkItem* ditems[2] = {nullptr, nullptr};
     int grid[2] = {0, 0};
-    // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, then y-faces + update)
+    // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, y-faces, update)
     int variant = 0, rows_b = 8;
-    std::vector<phx::host::WorkItem> items_b[2];  // second kernel of the split variant
+    std::vector<phx::host::WorkItem> items_b[2];  // split variant: update kernel
     phx::host::WorkItem* ditems_b[2] = {nullptr, nullptr};
     int grid_b[2] = {0, 0};
+    std::vector<phx::host::WorkItem> items_f[2];  // split variant: y-face kernel
+    phx::host::WorkItem* ditems_f[2] = {nullptr, nullptr};
+    int grid_f[2] = {0, 0};
     double* sn = nullptr;  // split variant: N/S states per cell
     double* p1 = nullptr;  // split variant: x-face part of the residual per cell
+    double* yf = nullptr;  // split variant: y-face fluxes
+    long long yf_len = 0;  // doubles of yf (sum over blocks of (ny+1)*nx*4)
     std::vector<phx::host::StageProg> progs;
     std::vector<double> butcher;  // nstages x nstages, row-major
     int nstages = 0;
@@ -99,8 +104,8 @@ struct phx_plan {
     bool adj_ready = false;
     long long adj_cells = 0, adj_faces = 0;
     phx::host::AdjBlock* dadj = nullptr;
-    std::vector<phx::host::AdjTile> adj_cell_tiles, adj_cv_tiles, adj_face_tiles;  // cells, cell-variables, faces
-    phx::host::AdjTile *dadj_cell_tiles = nullptr, *dadj_cv_tiles = nullptr, *dadj_face_tiles = nullptr;
+    std::vector<phx::host::AdjTile> adj_cell_tiles, adj_face_tiles;
+    phx::host::AdjTile *dadj_cell_tiles = nullptr, *dadj_face_tiles = nullptr;
     double* adj_rb = nullptr;     // limited face states E, W, N, S of every cell (16 doubles)
     double* adj_face = nullptr;   // adjoints of the left/right states of every face (8 doubles)
     double* adj_slot = nullptr;   // reconstruction adjoints: cell, W, E, S, N (20 doubles per cell)

@@ -275,3 +275,48 @@ def test_other_limiters_match_oracle(limiter, variant):
     s.run_steps(4)
     sim.run(4)
     assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+@pytest.mark.parametrize("case,rows", [("single", 1), ("single", 3), ("multiblock", 1), ("multiblock", 2),
+                                       ("oblique", 2), ("skewed", 1), ("wide", 2), ("wide_multiblock", 4)])
+def test_shared_halo_variant_matches_oracle(case, rows):
+    """Workgroups of 4 waves sharing halo rows through LDS (variant 2): residual and steps bit for
+    bit, including items whose upper waves have no rows (block heights not a multiple of 4*rows)."""
+    if case == "single":
+        cfg, mesh = W.explosion_single(26)
+    elif case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(14, 2, 4)
+    elif case == "oblique":
+        cfg, mesh = W.oblique_shock(18, 3)
+    elif case == "wide":
+        cfg, mesh = W.explosion_single(130)
+    elif case == "wide_multiblock":
+        cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    else:
+        cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
+    s = _solver(cfg, mesh)
+    s.plan.set_launch(rows, 0, 2, variant=2)
+    st = _perturbed(s.states()[0], seed=13)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(5)
+    sim.run(5)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+def test_shared_halo_variant_batched_cases_and_limiter():
+    cfgs, meshes, _ = W.explosion_batch(seed=3, ncases=4, sizes=[40, 64, 33, 70])
+    for c in cfgs:
+        c.fvm_slope_limiter_type = "VanAlbada"
+    a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b.plan.set_launch(1, 0, 2, variant=2)
+    for s in (a, b):
+        s.apply_initial_condition()
+        s.run_steps(6)
+    for x, y in zip(a.states(), b.states()):
+        assert bitwise_equal_blocks(x, y)
+    assert a.failed == b.failed
