@@ -33,18 +33,48 @@ namespace {
 constexpr int TILE = 256;
 constexpr int MAX_TERMS = PHX_MAX_STAGES + 2;
 
+// sum_m c_m * (dt of the cell's case if scaled_m) * in_m at one cell: the residual seeds and the
+// output terms of a Runge-Kutta step VJP, evaluated where they are used
+struct Terms {
+    const double* in[MAX_TERMS];
+    double c[MAX_TERMS];
+    int scaled[MAX_TERMS];
+    int n;
+    const double* dt;
+};
+
+__device__ __forceinline__ Q4 eval_terms(const Terms& T, const DevBlock& B, long long cell) {
+    Q4 r = phx::q4zero();
+    const double dt = T.dt ? T.dt[B.case_id] : 1.0;
+#pragma unroll
+    for (int m = 0; m < MAX_TERMS; ++m) {
+        if (m >= T.n) break;
+        const double c = T.scaled[m] ? T.c[m] * dt : T.c[m];
+        const double* p = T.in[m] + 4 * cell;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r.v[q] += c * p[q];
+    }
+    return r;
+}
+
+struct TermSeed {
+    Terms t;
+    __device__ Q4 operator()(const DevBlock& B, long long cell) const { return eval_terms(t, B, cell); }
+};
+
 template <int FLUX>
 __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                   const double* U, const double* rb, const double* kappa,
-                                                   double* fadj, Phys ph) {
+                                                   const double* U, const double* rb, Terms kappa, double* fadj,
+                                                   Phys ph) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
     const AdjBlock O = offs[t.block];
     const long long f = (long long)t.start + threadIdx.x;
     double* out = fadj + phx::adj::FA * (O.face + f);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::face_adj<true, FLUX>(blocks, B, U, rb, O, kappa, f, out, ph);
-    else phx::adj::face_adj<false, FLUX>(blocks, B, U, rb, O, kappa, f, out, ph);
+    const TermSeed seed{kappa};
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::face_adj<true, FLUX>(blocks, B, U, rb, O, seed, f, out, ph);
+    else phx::adj::face_adj<false, FLUX>(blocks, B, U, rb, O, seed, f, out, ph);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
@@ -73,46 +103,18 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, lim);
 }
 
+// lamU = (dR/dU)^T kappa + extra terms (pointwise; extra may include lamU itself)
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                     const double* slot, const double* fadj, double* lamU) {
+                                                     const double* slot, const double* fadj, Terms extra, double* lamU) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
-    const Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
-                       ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj)
-                       : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj);
+    Q4 lam = (B.kind == PHX_GEOM_CARTESIAN) ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj)
+                                            : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj);
+    if (extra.n > 0) phx::adj::acc(lam, eval_terms(extra, B, B.off / 4 + (long long)k * B.nx + j));
     phx::adj::st(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
-}
-
-// out = sum_m c_m * (dt of the cell's case if scaled_m) * in_m, over the cells of every block
-struct CombArgs {
-    const double* in[MAX_TERMS];
-    double c[MAX_TERMS];
-    int scaled[MAX_TERMS];
-    int n;
-    const double* dt;
-    double* out;
-};
-
-__global__ void __launch_bounds__(TILE) k_adj_comb(const DevBlock* blocks, const AdjTile* tiles, CombArgs A) {
-    const AdjTile t = tiles[blockIdx.x];
-    if ((int)threadIdx.x >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const double dt = A.dt ? A.dt[B.case_id] : 1.0;
-    const long long base = B.off + 4LL * (t.start + threadIdx.x);
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int m = 0; m < MAX_TERMS; ++m) {
-        if (m >= A.n) break;
-        const double c = A.scaled[m] ? A.c[m] * dt : A.c[m];
-        const double* p = A.in[m] + base;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] += c * p[q];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) A.out[base + q] = r[q];
 }
 
 template <class T>
@@ -166,46 +168,41 @@ int adj_setup(phx_plan* P) {
     PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * sizeof(double)));
     PHX_CK(hipMalloc(&P->adj_face, (size_t)faces * phx::adj::FA * sizeof(double)));
     PHX_CK(hipMalloc(&P->adj_slot, (size_t)P->adj_cells * phx::adj::SLOT * sizeof(double)));
-    PHX_CK(hipMalloc(&P->adj_kappa, L * sizeof(double)));
-    for (int s = 0; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->adj_lam[s], L * sizeof(double)));
+    for (int s = 1; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->adj_lam[s], L * sizeof(double)));
     P->adj_ready = true;
     return PHX_OK;
 }
 
-void launch_residual_vjp(phx_plan* P, const double* U, const double* kappa, double* lamU, hipStream_t st) {
+void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
+                         hipStream_t st) {
     const dim3 blk(TILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
     hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
                        P->ph.limiter);
     if (P->flux == PHX_FLUX_ROE)
-        hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
-                           P->adj_face, P->ph);
+        hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
+                           kappa, P->adj_face, P->ph);
     else
-        hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb, kappa,
-                           P->adj_face, P->ph);
+        hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
+                           kappa, P->adj_face, P->ph);
     hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
                        P->adj_slot, P->ph.limiter);
     hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
-                       lamU);
+                       extra, lamU);
 }
 
-void launch_comb(phx_plan* P, const CombArgs& A, hipStream_t st) {
-    hipLaunchKernelGGL(k_adj_comb, dim3((unsigned)P->adj_cell_tiles.size()), dim3(TILE), 0, st, P->dblocks,
-                       P->dadj_cell_tiles, A);
-}
 
 }  // namespace
 
 void phx::host::adjoint_release(phx_plan* P) {
-    void* ptrs[] = {P->dadj, P->dadj_cell_tiles, P->dadj_face_tiles, P->adj_rb, P->adj_face, P->adj_slot,
-                    P->adj_kappa};
+    void* ptrs[] = {P->dadj, P->dadj_cell_tiles, P->dadj_face_tiles, P->adj_rb, P->adj_face, P->adj_slot};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (int s = 0; s < PHX_MAX_STAGES; ++s)
         if (P->adj_lam[s]) (void)hipFree(P->adj_lam[s]);
     P->dadj = nullptr;
     P->dadj_cell_tiles = P->dadj_face_tiles = nullptr;
-    P->adj_rb = P->adj_face = P->adj_slot = P->adj_kappa = nullptr;
+    P->adj_rb = P->adj_face = P->adj_slot = nullptr;
     for (int s = 0; s < PHX_MAX_STAGES; ++s) P->adj_lam[s] = nullptr;
     P->adj_ready = false;
 }
@@ -216,7 +213,13 @@ int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* l
     if (!P || !U || !lamR || !lamU) return fail(PHX_E_INVALID, "null argument");
     int rc = adj_setup(P);
     if (rc) return rc;
-    launch_residual_vjp(P, U, lamR, lamU, (hipStream_t)stream);
+    Terms seed, none;
+    memset(&seed, 0, sizeof(seed));
+    memset(&none, 0, sizeof(none));
+    seed.in[0] = lamR;
+    seed.c[0] = 1.0;
+    seed.n = 1;
+    launch_residual_vjp(P, U, seed, none, lamU, (hipStream_t)stream);
     PHX_CK(hipGetLastError());
     return PHX_OK;
 }
@@ -235,31 +238,32 @@ int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double
     const double* a = P->butcher.data();
     auto lam_of = [&](int l) -> const double* { return l == S ? lam : P->adj_lam[l]; };
     for (int i = S - 1; i >= 0; --i) {
-        CombArgs C;
-        memset(&C, 0, sizeof(C));
+        // seed kappa_i, evaluated inside the face kernel
+        Terms kap, extra;
+        memset(&kap, 0, sizeof(kap));
+        memset(&extra, 0, sizeof(extra));
         for (int l = i + 1; l <= S; ++l) {
             const double c = a[(size_t)(l - 1) * S + i];
             if (c == 0.0) continue;
-            C.in[C.n] = lam_of(l);
-            C.c[C.n] = c;
-            C.scaled[C.n] = 1;
-            ++C.n;
+            kap.in[kap.n] = lam_of(l);
+            kap.c[kap.n] = c;
+            kap.scaled[kap.n] = 1;
+            ++kap.n;
         }
-        C.dt = dt_dev;
-        C.out = P->adj_kappa;
-        launch_comb(P, C, st);
-        launch_residual_vjp(P, stages + (size_t)i * L, P->adj_kappa, P->adj_lam[i], st);
+        kap.dt = dt_dev;
+        if (i > 0) {
+            launch_residual_vjp(P, stages + (size_t)i * L, kap, extra, P->adj_lam[i], st);
+        } else {
+            // lambda_0 = lambda_S + sum_{l=1}^{S-1} lambda_l + (dR/dU(U_0))^T kappa_0, written over lam
+            extra.in[extra.n] = lam;
+            extra.c[extra.n++] = 1.0;
+            for (int l = 1; l < S; ++l) {
+                extra.in[extra.n] = P->adj_lam[l];
+                extra.c[extra.n++] = 1.0;
+            }
+            launch_residual_vjp(P, stages, kap, extra, lam, st);
+        }
     }
-    CombArgs C;
-    memset(&C, 0, sizeof(C));
-    C.in[C.n] = lam;
-    C.c[C.n++] = 1.0;
-    for (int l = 0; l < S; ++l) {
-        C.in[C.n] = P->adj_lam[l];
-        C.c[C.n++] = 1.0;
-    }
-    C.out = lam;
-    launch_comb(P, C, st);
     PHX_CK(hipGetLastError());
     return PHX_OK;
 }

@@ -169,11 +169,18 @@ __host__ __device__ inline Q4 cell_residual(const DevBlock& B, const AdjBlock& O
     return R;
 }
 
+// Residual seed read from an array in the state layout (Seed of face_adj).
+struct PtrSeed {
+    const double* p;
+    __host__ __device__ Q4 operator()(const DevBlock&, long long cell) const { return ld(p + 4 * cell); }
+};
+
 // Adjoint of face f: seed lambda_F = sum over the (up to two) cells of the face of
 // kappa_cell * dR_cell/dF, then the flux VJP in the face frame.  Writes left/right adjoints.
-template <bool CART, int FLUX>
+// kappa(B, cell) returns the residual seed of a cell.
+template <bool CART, int FLUX, class Seed>
 __host__ __device__ inline void face_adj(const DevBlock* blocks, const DevBlock& B, const double* U, const double* rb,
-                                         const AdjBlock& O, const double* kappa, long long f, double* out,
+                                         const AdjBlock& O, const Seed& kappa, long long f, double* out,
                                          const Phys& ph) {
     const FaceRef F = face_ref(B, f);
     Q4 lam = q4zero();
@@ -181,20 +188,20 @@ __host__ __device__ inline void face_adj(const DevBlock* blocks, const DevBlock&
     if (F.xdir) {
         if (F.j > 0) {  // east face of cell (k, jf-1): -LE/A
             dev::cell_L<CART>(B, F.k, F.j - 1, LW, LE, LS, LN, A);
-            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j - 1)), -LE / A));
+            acc(lam, scale(kappa(B, cell_index(O, B, F.k, F.j - 1)), -LE / A));
         }
         if (F.j < B.nx) {  // west face of cell (k, jf): +LW/A
             dev::cell_L<CART>(B, F.k, F.j, LW, LE, LS, LN, A);
-            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j)), LW / A));
+            acc(lam, scale(kappa(B, cell_index(O, B, F.k, F.j)), LW / A));
         }
     } else {
         if (F.k > 0) {  // north face of cell (kf-1, j): -LN/A
             dev::cell_L<CART>(B, F.k - 1, F.j, LW, LE, LS, LN, A);
-            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k - 1, F.j)), -LN / A));
+            acc(lam, scale(kappa(B, cell_index(O, B, F.k - 1, F.j)), -LN / A));
         }
         if (F.k < B.ny) {  // south face of cell (kf, j): +LS/A
             dev::cell_L<CART>(B, F.k, F.j, LW, LE, LS, LN, A);
-            acc(lam, scale(ld(kappa + 4 * cell_index(O, B, F.k, F.j)), LS / A));
+            acc(lam, scale(kappa(B, cell_index(O, B, F.k, F.j)), LS / A));
         }
     }
     Q4 bL = q4zero(), bR = q4zero();

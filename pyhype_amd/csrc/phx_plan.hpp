@@ -109,8 +109,7 @@ struct phx_plan {
     double* adj_rb = nullptr;     // limited face states E, W, N, S of every cell (16 doubles)
     double* adj_face = nullptr;   // adjoints of the left/right states of every face (8 doubles)
     double* adj_slot = nullptr;   // reconstruction adjoints: cell, W, E, S, N (20 doubles per cell)
-    double* adj_kappa = nullptr;  // residual seed of one stage
-    double* adj_lam[PHX_MAX_STAGES] = {};  // stage-state adjoints of one step
+    double* adj_lam[PHX_MAX_STAGES] = {};  // stage-state adjoints of one step (stages 1..S-1)
 };
 
 namespace phx {

@@ -93,9 +93,9 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
         for (long long f = 0; f < nf; ++f) {
             double* out = fadj.data() + adj::FA * (P.offs[b].face + f);
             if (B.kind == PHX_GEOM_CARTESIAN)
-                adj::face_adj<true, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], kappa, f, out, ph);
+                adj::face_adj<true, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], adj::PtrSeed{kappa}, f, out, ph);
             else
-                adj::face_adj<false, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], kappa, f, out, ph);
+                adj::face_adj<false, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], adj::PtrSeed{kappa}, f, out, ph);
         }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
