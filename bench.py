@@ -228,17 +228,17 @@ def main():
         desc["unrealizable_cases"] = failed
         desc["live_cells"] = live
 
-    # stage-kernel duration with HIP events on the launch stream
+    # stage-kernel duration with HIP events on the launch stream: the stage programs of `reps`
+    # steps launched back to back between two events (the average includes the short launch gaps)
     reps = max(3, min(50, int(2000 // max(1, K))))
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(S * reps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for r in range(reps):
         for s in range(S):
-            e = evs[r * S + s]
-            e[0].record()
             plan.stage(s)
-            e[1].record()
+    e1.record()
     torch.cuda.synchronize()
-    stage_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / len(evs)
+    stage_ms = e0.elapsed_time(e1) / (reps * S)
     b_cell_step = alg_bytes_per_cell_step(table)
     bytes_per_launch = cells * b_cell_step / S
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9

@@ -60,7 +60,7 @@ def _time_steps(plan, steps: int) -> float:
 
 
 def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: Sequence[int] = (2, 3),
-               split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (4, 8, 16, 32),
+               split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (1, 2, 4, 8, 16, 32),
                split_waves: Sequence[int] = (3, 4), shared_rows: Sequence[int] = (1, 2, 3, 4, 6)):
     """The default search space: fused shapes, shared-halo shapes (rows per wave), split shapes."""
     out = [{"variant": 0, "rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "rows_b": 0}
