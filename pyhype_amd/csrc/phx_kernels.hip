@@ -869,20 +869,41 @@ void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
     }
 }
 
+// Buffers of one stage-program launch: the caller's state U (read as U0 / stage-0 input, written by
+// the last stage) and the stage-state buffer the stage reads (s_read) or writes (s_write).
+struct BufMap {
+    double *u_read, *u_write, *s_read, *s_write;
+};
+
+double* map_buf(phx_plan* P, const BufMap& M, int id, bool write) {
+    if (id == BUF_USER) return write ? M.u_write : M.u_read;
+    if (id == BUF_S0 || id == BUF_S1) return write ? M.s_write : M.s_read;
+    return P->acc[id - BUF_ACC0];
+}
+
+void launch_prog_stage_mapped(phx_plan* P, size_t s, const BufMap& M, hipStream_t st) {
+    const StageProg& g = P->progs[s];
+    StageArgs A = base_args(P);
+    A.Uin = map_buf(P, M, g.in, false);
+    A.nout = (int)g.outs.size();
+    for (int m = 0; m < A.nout; ++m) {
+        A.dst[m] = map_buf(P, M, g.outs[m].dst, true);
+        A.src[m] = map_buf(P, M, g.outs[m].src, false);
+        A.a[m] = g.outs[m].a;
+    }
+    A.state_out = g.state_out;
+    A.skip_dead = 1;
+    if (g.state_out >= 0) A.dtmin = P->dtmin;
+    launch_stage(P, A, st);
+}
+
 void launch_prog_stage(phx_plan* P, double* U, size_t s, hipStream_t st) {
-        const StageProg& g = P->progs[s];
-        StageArgs A = base_args(P);
-        A.Uin = buf_ptr(P, U, g.in);
-        A.nout = (int)g.outs.size();
-        for (int m = 0; m < A.nout; ++m) {
-            A.dst[m] = buf_ptr(P, U, g.outs[m].dst);
-            A.src[m] = buf_ptr(P, U, g.outs[m].src);
-            A.a[m] = g.outs[m].a;
-        }
-        A.state_out = g.state_out;
-        A.skip_dead = 1;
-        if (g.state_out >= 0) A.dtmin = P->dtmin;
-        launch_stage(P, A, st);
+    const StageProg& g = P->progs[s];
+    int sw = BUF_S0;
+    for (const StageOut& o : g.outs)
+        if (o.dst == BUF_S0 || o.dst == BUF_S1) sw = o.dst;
+    const BufMap M{U, U, buf_ptr(P, U, g.in == BUF_USER ? BUF_S0 : g.in), buf_ptr(P, U, sw)};
+    launch_prog_stage_mapped(P, s, M, st);
 }
 
 void launch_step(phx_plan* P, double* U, hipStream_t st, const double* frozen) {
@@ -1117,15 +1138,24 @@ int phx_run_record(phx_plan* P, double* U, int32_t nsteps, const double* dt_sche
     hipStream_t st = (hipStream_t)stream;
     const size_t len = (size_t)P->state_len;
     const int S = (int)P->progs.size();
+    if (nsteps == 0) return PHX_OK;
+    // The records are the working buffers: step n reads its input from record (n, 0), writes the
+    // state of stage s+1 into record (n, s+1) and its result into record (n+1, 0) (U after the last
+    // step), so only the first input is copied.
+    PHX_CK(hipMemcpyAsync(stages, U, len * sizeof(double), hipMemcpyDeviceToDevice, st));
     for (int n = 0; n < nsteps; ++n) {
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
                            P->dt, P->step, P->bad, dt_sched + (size_t)n * P->ncases, 1);
+        double* rec = stages + (size_t)n * S * len;
+        double* next = (n + 1 < nsteps) ? rec + (size_t)S * len : U;
         for (int s = 0; s < S; ++s) {
-            double* dst = stages + ((size_t)n * S + s) * len;
-            PHX_CK(hipMemcpyAsync(dst, buf_ptr(P, U, P->progs[s].in), len * sizeof(double), hipMemcpyDeviceToDevice, st));
-            launch_prog_stage(P, U, (size_t)s, st);
+            BufMap M;
+            M.u_read = rec;
+            M.u_write = next;
+            M.s_read = rec + (size_t)s * len;
+            M.s_write = (s + 1 < S) ? rec + (size_t)(s + 1) * len : next;  // one-stage tables write S0
+            launch_prog_stage_mapped(P, (size_t)s, M, st);
         }
-        if (P->copy_back) PHX_CK(hipMemcpyAsync(U, P->sbuf[0], len * sizeof(double), hipMemcpyDeviceToDevice, st));
     }
     PHX_CK(hipGetLastError());
     return PHX_OK;
