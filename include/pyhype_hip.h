@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PHX_ABI_VERSION 1
+#define PHX_ABI_VERSION 2
 
 /* error codes */
 #define PHX_OK 0
@@ -46,6 +46,11 @@ extern "C" {
 /* flux functions (flux/factory.py:30-47) */
 #define PHX_FLUX_ROE 0
 #define PHX_FLUX_HLLL 1
+
+/* reconstruction variables (SolverConfig.reconstruction_type; fvm/base.py:255-303,
+ * blocks/quad_block.py:120-134,225-240) */
+#define PHX_RECON_CONSERVATIVE 0
+#define PHX_RECON_PRIMITIVE 1
 
 /* limiters (limiters/factory.py:28-39, limiters/limiters.py:25-62) */
 #define PHX_LIMITER_VENKATAKRISHNAN 0
@@ -105,6 +110,7 @@ typedef struct {
     const double* t_final;    /* per case, already scaled by a_inf (solvers/base.py:73) */
     int32_t rows_per_item;    /* rows of one work item; 0 = default */
     int32_t grid_waves;       /* waves of the persistent grid; 0 = one per work item */
+    int32_t reconstruction;   /* PHX_RECON_* (SolverConfig.reconstruction_type) */
 } phx_plan_desc;
 
 typedef struct phx_plan phx_plan;

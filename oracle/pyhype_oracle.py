@@ -445,6 +445,7 @@ class Block:
     U: np.ndarray = None
     ghost: Dict[str, np.ndarray] = field(default_factory=dict)
     limiter_fn: object = None          # limiter function (None: Venkatakrishnan)
+    prim: bool = False                 # reconstruction_type is PrimitiveState
 
 
 def boundary_theta(g: Geometry, d: str):
@@ -563,12 +564,14 @@ def limiter_phi(U, ghost, hot: Dict[str, np.ndarray], limiter_fn=venkatakrishnan
     return np.where(phi < 0, 0, phi)
 
 
-def _outer_state(b: Block, d: str, limited_boundary: np.ndarray):
+def _outer_state(b: Block, d: str, limited_boundary: np.ndarray, ghost=None):
     """MUSCL._get_boundary_flux_states (fvm/base.py:305-325) + the BC applied to it
-    (fvm/base.py:352-362,421-431)."""
+    (fvm/base.py:352-362,421-431).  `ghost`: the reconstruction block's ghost states (default: the
+    block's conservative ghosts)."""
+    ghost = b.ghost if ghost is None else ghost
     if b.bc[d] is None:
         # ghost block fvm: phi = 0, gradient = 0 -> ghost cell average (+0)
-        return b.ghost[d].copy() + 0.0
+        return ghost[d].copy() + 0.0
     st = limited_boundary.copy()
     if b.bc[d] in ("Reflection", "Slipwall"):
         reflect(st, boundary_theta(b.geom, d))
@@ -578,16 +581,27 @@ def _outer_state(b: Block, d: str, limited_boundary: np.ndarray):
 def residual(b: Block, flux_fn):
     """FiniteVolumeMethod.dUdt (fvm/base.py:108-165) for SecondOrderMUSCL with Conservative
     reconstruction (MUSCL.evaluate_flux, fvm/base.py:327-500)."""
-    U = b.U.copy()          # recon_block.from_block (blocks/quad_block.py:120-134)
+    # recon_block.from_block (blocks/quad_block.py:120-134): the interior and ghost states in the
+    # reconstruction type -- a copy (ConservativeState) or ConservativeConverter.to_primitive
+    # (PrimitiveState; concrete_defs.py:85-100)
+    if b.prim:
+        U = cons_to_prim(b.U)
+        ghost = {d: cons_to_prim(b.ghost[d]) for d in DIRS}
+    else:
+        U = b.U.copy()
+        ghost = b.ghost
+    # MUSCL._get_left_right_riemann_states (fvm/base.py:255-303): primitive face states go to the
+    # flux function as they are
+    to_flux = (lambda a: a) if b.prim else cons_to_prim
     g = b.geom
-    gx, gy = green_gauss(U, b.ghost, g)
+    gx, gy = green_gauss(U, ghost, g)
     hot = {d: high_order_term(gx, gy, g, d) for d in DIRS}
-    phi = limiter_phi(U, b.ghost, hot, b.limiter_fn or venkatakrishnan)
+    phi = limiter_phi(U, ghost, hot, b.limiter_fn or venkatakrishnan)
     face = {d: U + phi * hot[d] for d in DIRS}   # limited_solution_at_quadrature_point
     bnd = {"E": face["E"][:, -1:], "W": face["W"][:, :1], "N": face["N"][-1:], "S": face["S"][:1]}
     # ---- east-west sweep (fvm/base.py:327-390)
-    outE = _outer_state(b, "E", bnd["E"])
-    outW = _outer_state(b, "W", bnd["W"])
+    outE = _outer_state(b, "E", bnd["E"], ghost)
+    outW = _outer_state(b, "W", bnd["W"], ghost)
     fE = face["E"].copy()
     fW = face["W"].copy()
     if not g.is_cartesian:
@@ -595,8 +609,8 @@ def residual(b: Block, flux_fn):
         rotate(g.face["W"].theta, fW)
         rotate(boundary_theta(g, "E"), outE)
         rotate(boundary_theta(g, "W"), outW)
-    left = cons_to_prim(np.concatenate((outW, fE), axis=1))
-    right = cons_to_prim(np.concatenate((fW, outE), axis=1))
+    left = to_flux(np.concatenate((outW, fE), axis=1))
+    right = to_flux(np.concatenate((fW, outE), axis=1))
     Fx = flux_fn(left, right)
     flux_E = Fx[:, 1:].copy()
     flux_W = Fx[:, :-1].copy()
@@ -604,8 +618,8 @@ def residual(b: Block, flux_fn):
         unrotate(g.face["E"].theta, flux_E)
         unrotate(g.face["W"].theta, flux_W)
     # ---- north-south sweep (fvm/base.py:392-486)
-    outN = _outer_state(b, "N", bnd["N"])
-    outS = _outer_state(b, "S", bnd["S"])
+    outN = _outer_state(b, "N", bnd["N"], ghost)
+    outS = _outer_state(b, "S", bnd["S"], ghost)
     fN = face["N"].copy()
     fS = face["S"].copy()
     if g.is_cartesian:
@@ -615,8 +629,8 @@ def residual(b: Block, flux_fn):
         rotate(g.face["S"].theta, fS)
         rotate(boundary_theta(g, "N"), outN)
         rotate(boundary_theta(g, "S"), outS)
-    left = cons_to_prim(np.concatenate((outS.transpose(1, 0, 2), fN.transpose(1, 0, 2)), axis=1))
-    right = cons_to_prim(np.concatenate((fS.transpose(1, 0, 2), outN.transpose(1, 0, 2)), axis=1))
+    left = to_flux(np.concatenate((outS.transpose(1, 0, 2), fN.transpose(1, 0, 2)), axis=1))
+    right = to_flux(np.concatenate((fS.transpose(1, 0, 2), outN.transpose(1, 0, 2)), axis=1))
     Fy = flux_fn(left, right).transpose(1, 0, 2)
     flux_N = Fy[1:].copy()
     flux_S = Fy[:-1].copy()
@@ -700,7 +714,7 @@ class Simulation:
 
     def __init__(self, mesh: Dict[int, dict], nx: int, ny: int, *, flux: str = "Roe",
                  integrator="RK4", CFL: float = 0.8, t_final: float = 1e30, a_inf: float = 1.0,
-                 limiter: str = "Venkatakrishnan"):
+                 limiter: str = "Venkatakrishnan", reconstruction: str = "Conservative"):
         self.flux_fn = {"Roe": flux_roe, "HLLL": flux_hlll}[flux]
         self.limiter_fn = LIMITERS[limiter]
         self.a = BUTCHER[integrator] if isinstance(integrator, str) else integrator
@@ -714,7 +728,8 @@ class Simulation:
             vert = {k: d[k] for k in ("SW", "SE", "NW", "NE")}
             g = block_geometry(vert, nx, ny)
             self.blocks[n] = Block(n, g, {s: d["Neighbor" + s] for s in DIRS},
-                                   {s: d["BCType" + s] for s in DIRS}, limiter_fn=self.limiter_fn)
+                                   {s: d["BCType" + s] for s in DIRS}, limiter_fn=self.limiter_fn,
+                                   prim=(reconstruction == "Primitive"))
 
     def set_state(self, states: Dict[int, np.ndarray]):
         for n, b in self.blocks.items():

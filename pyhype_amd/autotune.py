@@ -34,6 +34,8 @@ def signature(plan) -> str:
         if torch.cuda.is_available() else "none"
     lim = getattr(plan, "limiter", "Venkatakrishnan")
     flux = plan.flux if lim == "Venkatakrishnan" else f"{plan.flux}/{lim}"
+    if getattr(plan, "primitive", False):
+        flux += "/prim"
     return f"{kinds}|{flux}|{len(plan.table)}|{shp}|{dev}"
 
 

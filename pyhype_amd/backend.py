@@ -149,7 +149,7 @@ def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layo
 class DevicePlan:
     def __init__(self, cases: Sequence[CaseSpec], *, flux: str, table: List[List[float]], fluid, cfl: float,
                  rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False,
-                 limiter: str = "Venkatakrishnan"):
+                 limiter: str = "Venkatakrishnan", primitive: bool = False):
         torch = _torch()
         self.torch = torch
         self.lib = nat.load()
@@ -161,6 +161,7 @@ class DevicePlan:
         if limiter not in nat.LIMITERS:
             raise ValueError("MUSCL: Slope limiter type not specified.")
         self.limiter = limiter
+        self.primitive = bool(primitive)
         self.table = table
         S = len(table)
         a = np.zeros((S, S))
@@ -182,6 +183,7 @@ class DevicePlan:
         pd.ncases = len(self.cases)
         pd.flux = flux_id
         pd.limiter = nat.LIMITERS[limiter]
+        pd.reconstruction = nat.RECON_PRIMITIVE if primitive else nat.RECON_CONSERVATIVE
         pd.nstages = S
         pd.butcher_a = self._a.ctypes.data_as(C.POINTER(C.c_double))
         pd.gamma = g

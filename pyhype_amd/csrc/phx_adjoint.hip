@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                    const double* U, double* rb, int lim) {
+                                                    const double* U, double* rb, Phys ph) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
@@ -86,12 +86,12 @@ __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, cons
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = rb + phx::adj::RB * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out, lim);
-    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out, lim);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out, ph);
+    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out, ph);
 }
 
 __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                   const double* U, const double* fadj, double* slot, int lim) {
+                                                   const double* U, const double* fadj, double* slot, Phys ph) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
@@ -99,20 +99,21 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out, lim);
-    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, lim);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out, ph);
+    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, ph);
 }
 
 // lamU = (dR/dU)^T kappa + extra terms (pointwise; extra may include lamU itself)
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
-                                                     const double* slot, const double* fadj, Terms extra, double* lamU) {
+                                                     const double* slot, const double* fadj, const double* U,
+                                                     Phys ph, Terms extra, double* lamU) {
     const AdjTile t = tiles[blockIdx.x];
     if ((int)threadIdx.x >= t.count) return;
     const DevBlock& B = blocks[t.block];
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
-    Q4 lam = (B.kind == PHX_GEOM_CARTESIAN) ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj)
-                                            : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj);
+    Q4 lam = (B.kind == PHX_GEOM_CARTESIAN) ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj, U, ph)
+                                            : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj, U, ph);
     if (extra.n > 0) phx::adj::acc(lam, eval_terms(extra, B, B.off / 4 + (long long)k * B.nx + j));
     phx::adj::st(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
 }
@@ -177,8 +178,7 @@ void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const
                          hipStream_t st) {
     const dim3 blk(TILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
-    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
-                       P->ph.limiter);
+    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
     if (P->flux == PHX_FLUX_ROE)
         hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
@@ -186,9 +186,9 @@ void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const
         hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
     hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
-                       P->adj_slot, P->ph.limiter);
+                       P->adj_slot, P->ph);
     hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
-                       extra, lamU);
+                       U, P->ph, extra, lamU);
 }
 
 

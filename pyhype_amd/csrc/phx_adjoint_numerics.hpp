@@ -372,6 +372,22 @@ __host__ __device__ inline void face_flux_vjp(const Q4& qL, const Q4& qR, const 
     bqR = cons_to_prim_vjp(qR, bR, ph);
 }
 
+// face_flux_vjp on face states in the reconstruction variables (face_flux_recon): with primitive
+// reconstruction the adjoints are those of the primitive states.
+template <int FLUX>
+__host__ __device__ inline void face_flux_vjp_recon(const Q4& qL, const Q4& qR, const Q4& lamF, const Phys& ph,
+                                                    Q4& bqL, Q4& bqR) {
+    if (!ph.prim) {
+        face_flux_vjp<FLUX>(qL, qR, lamF, ph, bqL, bqR);
+        return;
+    }
+    PrimBar bL, bR;
+    if (FLUX == 0) roe_flux_vjp(as_prim(qL), as_prim(qR), lamF, ph, bL, bR);
+    else hlll_flux_vjp(as_prim(qL), as_prim(qR), lamF, ph, bL, bR);
+    bqL.v[0] = bL.r; bqL.v[1] = bL.u; bqL.v[2] = bL.v; bqL.v[3] = bL.p;
+    bqR.v[0] = bR.r; bqR.v[1] = bR.u; bqR.v[2] = bR.v; bqR.v[3] = bR.p;
+}
+
 // Adjoint of reconstruct() for one cell: given the adjoints of the four limited face states,
 // the adjoints of the cell and of its four neighbour values (each a 4-vector).
 struct ReconBar {

@@ -66,14 +66,14 @@ __host__ __device__ inline bool is_zero(const Q4& a) {
 // ---------------------------------------------------------------------------------------------
 template <bool CART>
 __host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
-                                           double* rb, int lim) {
-    const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
-    const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
-    const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
-    const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
-    const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
+                                           double* rb, const Phys& ph) {
+    const Q4 Uc = dev::fetch_r<CART, true>(blocks, B, U, k, j, ph);
+    const Q4 Uw = dev::fetch_r<CART, true>(blocks, B, U, k, j - 1, ph);
+    const Q4 Ue = dev::fetch_r<CART, true>(blocks, B, U, k, j + 1, ph);
+    const Q4 Us = dev::fetch_r<CART, true>(blocks, B, U, k - 1, j, ph);
+    const Q4 Un = dev::fetch_r<CART, true>(blocks, B, U, k + 1, j, ph);
     const CellGeo g = dev::cell_geo<CART>(B, k, j);
-    const Recon r = reconstruct<true>(Uc, Uw, Ue, Us, Un, g, lim);
+    const Recon r = reconstruct<true>(Uc, Uw, Ue, Us, Un, g, ph.limiter);
     st(rb + RB_E, r.E);
     st(rb + RB_W, r.W);
     st(rb + RB_N, r.N);
@@ -108,19 +108,20 @@ __host__ __device__ inline FaceRef face_ref(const DevBlock& B, long long f) {
 
 template <bool CART>
 __host__ __device__ inline void face_states(const DevBlock* blocks, const DevBlock& B, const double* U,
-                                            const double* rb, const AdjBlock& O, const FaceRef& F, Q4& L, Q4& R) {
+                                            const double* rb, const AdjBlock& O, const FaceRef& F, Q4& L, Q4& R,
+                                            const Phys& ph) {
     const int nx = B.nx, ny = B.ny;
     if (F.xdir) {
         const int k = F.k, jf = F.j;
         if (jf == 0) {
             const Q4 own = ld(rb + RB * cell_index(O, B, k, 0) + RB_W);
-            L = dev::outer_state<CART>(B, PHX_W, own, dev::fetch<CART>(blocks, B, U, k, -1), k, 0);
+            L = dev::outer_state<CART>(B, PHX_W, own, dev::fetch_r<CART, true>(blocks, B, U, k, -1, ph), k, 0);
         } else {
             L = ld(rb + RB * cell_index(O, B, k, jf - 1) + RB_E);
         }
         if (jf == nx) {
             const Q4 own = ld(rb + RB * cell_index(O, B, k, nx - 1) + RB_E);
-            R = dev::outer_state<CART>(B, PHX_E, own, dev::fetch<CART>(blocks, B, U, k, nx), k, nx);
+            R = dev::outer_state<CART>(B, PHX_E, own, dev::fetch_r<CART, true>(blocks, B, U, k, nx, ph), k, nx);
         } else {
             R = ld(rb + RB * cell_index(O, B, k, jf) + RB_W);
         }
@@ -128,13 +129,13 @@ __host__ __device__ inline void face_states(const DevBlock* blocks, const DevBlo
         const int kf = F.k, j = F.j;
         if (kf == 0) {
             const Q4 own = ld(rb + RB * cell_index(O, B, 0, j) + RB_S);
-            L = dev::outer_state<CART>(B, PHX_S, own, dev::fetch<CART>(blocks, B, U, -1, j), 0, j);
+            L = dev::outer_state<CART>(B, PHX_S, own, dev::fetch_r<CART, true>(blocks, B, U, -1, j, ph), 0, j);
         } else {
             L = ld(rb + RB * cell_index(O, B, kf - 1, j) + RB_N);
         }
         if (kf == ny) {
             const Q4 own = ld(rb + RB * cell_index(O, B, ny - 1, j) + RB_N);
-            R = dev::outer_state<CART>(B, PHX_N, own, dev::fetch<CART>(blocks, B, U, ny, j), ny - 1, j);
+            R = dev::outer_state<CART>(B, PHX_N, own, dev::fetch_r<CART, true>(blocks, B, U, ny, j, ph), ny - 1, j);
         } else {
             R = ld(rb + RB * cell_index(O, B, kf, j) + RB_S);
         }
@@ -147,8 +148,9 @@ __host__ __device__ inline Q4 face_flux_global(const DevBlock* blocks, const Dev
                                                const double* rb, const AdjBlock& O, long long f, const Phys& ph) {
     const FaceRef F = face_ref(B, f);
     Q4 L, R;
-    face_states<CART>(blocks, B, U, rb, O, F, L, R);
-    return F.xdir ? dev::xflux<CART, FLUX>(B, L, R, F.k, F.j, ph) : dev::yflux<CART, FLUX>(B, L, R, F.k, F.j, ph);
+    face_states<CART>(blocks, B, U, rb, O, F, L, R, ph);
+    return F.xdir ? dev::xflux<CART, FLUX, true>(B, L, R, F.k, F.j, ph)
+                  : dev::yflux<CART, FLUX, true>(B, L, R, F.k, F.j, ph);
 }
 
 // Residual of cell (k, j) from the global-frame face fluxes (fvm/base.py:141-190; same association
@@ -207,13 +209,13 @@ __host__ __device__ inline void face_adj(const DevBlock* blocks, const DevBlock&
     Q4 bL = q4zero(), bR = q4zero();
     if (!is_zero(lam)) {
         Q4 L, R;
-        face_states<CART>(blocks, B, U, rb, O, F, L, R);
+        face_states<CART>(blocks, B, U, rb, O, F, L, R, ph);
         if (CART && F.xdir) {
-            face_flux_vjp<FLUX>(L, R, lam, ph, bL, bR);
+            face_flux_vjp_recon<FLUX>(L, R, lam, ph, bL, bR);
         } else if (CART) {
             // F = unrotate90(flux(rotate90 L, rotate90 R)); transposes: rotate90^T = unrotate90
             Q4 bl, br;
-            face_flux_vjp<FLUX>(rotate90(L), rotate90(R), rotate90(lam), ph, bl, br);
+            face_flux_vjp_recon<FLUX>(rotate90(L), rotate90(R), rotate90(lam), ph, bl, br);
             bL = unrotate90(bl);
             bR = unrotate90(br);
         } else {
@@ -223,7 +225,7 @@ __host__ __device__ inline void face_adj(const DevBlock* blocks, const DevBlock&
             else { c = G.H(dev::F_c, F.k, F.j); s = G.H(dev::F_s, F.k, F.j); }
             // F = unrotate(flux(rotate L, rotate R)); rotate^T = unrotate
             Q4 bl, br;
-            face_flux_vjp<FLUX>(rotate(L, c, s), rotate(R, c, s), rotate(lam, c, s), ph, bl, br);
+            face_flux_vjp_recon<FLUX>(rotate(L, c, s), rotate(R, c, s), rotate(lam, c, s), ph, bl, br);
             bL = unrotate(bl, c, s);
             bR = unrotate(br, c, s);
         }
@@ -251,7 +253,8 @@ __host__ __device__ inline Q4 outer_own_adj(const DevBlock& B, int side, const Q
 // variables are independent; a variable whose four face adjoints are zero contributes zeros).
 template <bool CART>
 __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock& B, const double* U,
-                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot, int lim) {
+                                         const AdjBlock& O, const double* fadj, int k, int j, double* slot,
+                                         const Phys& ph) {
     const int nx = B.nx, ny = B.ny;
     Q4 bE = ld(fadj + FA * xface_index(O, B, k, j + 1));          // left state of the east face
     Q4 bW = ld(fadj + FA * xface_index(O, B, k, j) + 4);          // right state of the west face
@@ -263,12 +266,13 @@ __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock&
     if (k == 0) acc(bS, outer_own_adj<CART>(B, PHX_S, ld(fadj + FA * yface_index(O, B, 0, j)), 0, j));
     for (int q = 0; q < SLOT; ++q) slot[q] = 0.0;
     if (is_zero(bE) && is_zero(bW) && is_zero(bN) && is_zero(bS)) return;
-    const Q4 Uc = dev::fetch<CART>(blocks, B, U, k, j);
-    const Q4 Uw = dev::fetch<CART>(blocks, B, U, k, j - 1);
-    const Q4 Ue = dev::fetch<CART>(blocks, B, U, k, j + 1);
-    const Q4 Us = dev::fetch<CART>(blocks, B, U, k - 1, j);
-    const Q4 Un = dev::fetch<CART>(blocks, B, U, k + 1, j);
+    const Q4 Uc = dev::fetch_r<CART, true>(blocks, B, U, k, j, ph);
+    const Q4 Uw = dev::fetch_r<CART, true>(blocks, B, U, k, j - 1, ph);
+    const Q4 Ue = dev::fetch_r<CART, true>(blocks, B, U, k, j + 1, ph);
+    const Q4 Us = dev::fetch_r<CART, true>(blocks, B, U, k - 1, j, ph);
+    const Q4 Un = dev::fetch_r<CART, true>(blocks, B, U, k + 1, j, ph);
     const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    const int lim = ph.limiter;
     for (int v = 0; v < 4; ++v) {
         const double bq[4] = {bE.v[v], bW.v[v], bN.v[v], bS.v[v]};
         if (bq[0] == 0.0 && bq[1] == 0.0 && bq[2] == 0.0 && bq[3] == 0.0) continue;
@@ -282,79 +286,91 @@ __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock&
     }
 }
 
-// Adjoint of a ghost value of `side` that is a function of the block's own edge cell (wall:
-// reflection; open and outlet: copy).  Neighbour ghosts are gathered from the neighbour block.
+// Adjoint, with respect to the edge cell's conservative state q, of the ghost value of a wall side,
+// R(q) reflected in conservative form and then taken to the reconstruction variables: b is the
+// adjoint of that ghost value.  Reflection is symmetric, so its transpose is itself.
 template <bool CART>
-__host__ __device__ inline Q4 ghost_own_adj(const DevBlock& B, int side, const Q4& b, int k, int j) {
-    const int st_ = B.side[side];
-    if (st_ == PHX_SIDE_WALL) {
-        double c, s;
-        dev::wall_cs<CART>(B, side, k, j, c, s);
-        return reflect(b, c, s);
-    }
-    if (st_ == PHX_SIDE_OPEN || st_ == PHX_SIDE_OUTLET) return b;
-    return q4zero();
+__host__ __device__ inline Q4 wall_ghost_adj(const DevBlock& B, int side, const Q4& q, const Q4& b, int k, int j,
+                                             const Phys& ph) {
+    double c, s;
+    dev::wall_cs<CART>(B, side, k, j, c, s);
+    if (!ph.prim) return reflect(b, c, s);
+    const PrimBar pb{b.v[0], b.v[1], b.v[2], b.v[3]};
+    return reflect(cons_to_prim_vjp(reflect(q, c, s), pb, ph), c, s);
 }
 
-// lambda_U of cell (k, j): every path by which U(k, j) enters the residual.
+// lambda_U of cell (k, j): every path by which U(k, j) enters the residual.  Paths through the
+// cell's own value in the reconstruction variables (its stencil uses, neighbour blocks' ghosts and
+// outer states, open and outlet ghosts, which copy it) are summed first and taken back through the
+// conversion to primitive variables once; a wall ghost is the reflected state converted, and is
+// taken back on its own.
 template <bool CART>
 __host__ __device__ inline Q4 gather_adj(const DevBlock* blocks, const AdjBlock* offs, int b, int k, int j,
-                                         const double* slot, const double* fadj) {
+                                         const double* slot, const double* fadj, const double* U, const Phys& ph) {
     const DevBlock& B = blocks[b];
     const AdjBlock& O = offs[b];
     const int nx = B.nx, ny = B.ny;
-    Q4 lam = ld(slot + SLOT * cell_index(O, B, k, j) + SL_C);
+    const Q4 q = ld(U + B.off + 4LL * ((long long)k * nx + j));
+    Q4 own = ld(slot + SLOT * cell_index(O, B, k, j) + SL_C);
+    Q4 walls = q4zero();
     // stencil neighbour of cells of the same block
-    if (j > 0) acc(lam, ld(slot + SLOT * cell_index(O, B, k, j - 1) + SL_E));
-    if (j < nx - 1) acc(lam, ld(slot + SLOT * cell_index(O, B, k, j + 1) + SL_W));
-    if (k > 0) acc(lam, ld(slot + SLOT * cell_index(O, B, k - 1, j) + SL_N));
-    if (k < ny - 1) acc(lam, ld(slot + SLOT * cell_index(O, B, k + 1, j) + SL_S));
-    // west edge
+    if (j > 0) acc(own, ld(slot + SLOT * cell_index(O, B, k, j - 1) + SL_E));
+    if (j < nx - 1) acc(own, ld(slot + SLOT * cell_index(O, B, k, j + 1) + SL_W));
+    if (k > 0) acc(own, ld(slot + SLOT * cell_index(O, B, k - 1, j) + SL_N));
+    if (k < ny - 1) acc(own, ld(slot + SLOT * cell_index(O, B, k + 1, j) + SL_S));
+    // block edges: a neighbour's ghost (its stencil and its outer state), or this block's own ghost
+    auto edge = [&](int side, const Q4& ghost_slot, int gk, int gj, const Q4& open_outer) {
+        const int st_ = B.side[side];
+        if (st_ == PHX_SIDE_WALL) acc(walls, wall_ghost_adj<CART>(B, side, q, ghost_slot, gk, gj, ph));
+        else if (st_ == PHX_SIDE_OUTLET) acc(own, ghost_slot);
+        else if (st_ == PHX_SIDE_OPEN) { acc(own, ghost_slot); acc(own, open_outer); }
+    };
     if (j == 0) {
         if (B.side[PHX_W] == PHX_SIDE_NEIGHBOR) {
             // this cell is the east ghost of the west neighbour: its stencil and its east outer state
             const int n = B.nbr[PHX_W];
             const DevBlock& N = blocks[n];
-            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, k, N.nx - 1) + SL_E));
-            acc(lam, ld(fadj + FA * xface_index(offs[n], N, k, N.nx) + 4));
+            acc(own, ld(slot + SLOT * cell_index(offs[n], N, k, N.nx - 1) + SL_E));
+            acc(own, ld(fadj + FA * xface_index(offs[n], N, k, N.nx) + 4));
         } else {
-            acc(lam, ghost_own_adj<CART>(B, PHX_W, ld(slot + SLOT * cell_index(O, B, k, 0) + SL_W), k, -1));
-            if (B.side[PHX_W] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * xface_index(O, B, k, 0)));
+            edge(PHX_W, ld(slot + SLOT * cell_index(O, B, k, 0) + SL_W), k, -1, ld(fadj + FA * xface_index(O, B, k, 0)));
         }
     }
     if (j == nx - 1) {
         if (B.side[PHX_E] == PHX_SIDE_NEIGHBOR) {
             const int n = B.nbr[PHX_E];
             const DevBlock& N = blocks[n];
-            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, k, 0) + SL_W));
-            acc(lam, ld(fadj + FA * xface_index(offs[n], N, k, 0)));
+            acc(own, ld(slot + SLOT * cell_index(offs[n], N, k, 0) + SL_W));
+            acc(own, ld(fadj + FA * xface_index(offs[n], N, k, 0)));
         } else {
-            acc(lam, ghost_own_adj<CART>(B, PHX_E, ld(slot + SLOT * cell_index(O, B, k, nx - 1) + SL_E), k, nx));
-            if (B.side[PHX_E] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * xface_index(O, B, k, nx) + 4));
+            edge(PHX_E, ld(slot + SLOT * cell_index(O, B, k, nx - 1) + SL_E), k, nx,
+                 ld(fadj + FA * xface_index(O, B, k, nx) + 4));
         }
     }
     if (k == 0) {
         if (B.side[PHX_S] == PHX_SIDE_NEIGHBOR) {
             const int n = B.nbr[PHX_S];
             const DevBlock& N = blocks[n];
-            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, N.ny - 1, j) + SL_N));
-            acc(lam, ld(fadj + FA * yface_index(offs[n], N, N.ny, j) + 4));
+            acc(own, ld(slot + SLOT * cell_index(offs[n], N, N.ny - 1, j) + SL_N));
+            acc(own, ld(fadj + FA * yface_index(offs[n], N, N.ny, j) + 4));
         } else {
-            acc(lam, ghost_own_adj<CART>(B, PHX_S, ld(slot + SLOT * cell_index(O, B, 0, j) + SL_S), -1, j));
-            if (B.side[PHX_S] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * yface_index(O, B, 0, j)));
+            edge(PHX_S, ld(slot + SLOT * cell_index(O, B, 0, j) + SL_S), -1, j, ld(fadj + FA * yface_index(O, B, 0, j)));
         }
     }
     if (k == ny - 1) {
         if (B.side[PHX_N] == PHX_SIDE_NEIGHBOR) {
             const int n = B.nbr[PHX_N];
             const DevBlock& N = blocks[n];
-            acc(lam, ld(slot + SLOT * cell_index(offs[n], N, 0, j) + SL_S));
-            acc(lam, ld(fadj + FA * yface_index(offs[n], N, 0, j)));
+            acc(own, ld(slot + SLOT * cell_index(offs[n], N, 0, j) + SL_S));
+            acc(own, ld(fadj + FA * yface_index(offs[n], N, 0, j)));
         } else {
-            acc(lam, ghost_own_adj<CART>(B, PHX_N, ld(slot + SLOT * cell_index(O, B, ny - 1, j) + SL_N), ny, j));
-            if (B.side[PHX_N] == PHX_SIDE_OPEN) acc(lam, ld(fadj + FA * yface_index(O, B, ny, j) + 4));
+            edge(PHX_N, ld(slot + SLOT * cell_index(O, B, ny - 1, j) + SL_N), ny, j,
+                 ld(fadj + FA * yface_index(O, B, ny, j) + 4));
         }
     }
+    Q4 lam = own;
+    if (ph.prim) lam = cons_to_prim_vjp(q, PrimBar{own.v[0], own.v[1], own.v[2], own.v[3]}, ph);
+    acc(lam, walls);
     return lam;
 }
 

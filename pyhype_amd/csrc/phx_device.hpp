@@ -118,6 +118,17 @@ __host__ __device__ inline Q4 fetch(const DevBlock* __restrict__ blocks, const D
     return q;
 }
 
+// fetch() in the reconstruction variables: with GEN and PrimitiveState reconstruction the value
+// (for walls the reflected conservative ghost, as the reference's ghost blocks hold it) converted by
+// ConservativeConverter.to_primitive (recon_block.from_block, blocks/quad_block.py:120-134).
+template <bool CART, bool GEN>
+__host__ __device__ inline Q4 fetch_r(const DevBlock* __restrict__ blocks, const DevBlock& B,
+                                      const double* __restrict__ U, int k, int j, const Phys& ph) {
+    const Q4 q = fetch<CART>(blocks, B, U, k, j);
+    if (GEN && ph.prim) return phx::prim_q4(phx::cons_to_prim(q, ph));
+    return q;
+}
+
 // Outer state of a boundary face: MUSCL._get_boundary_flux_states (fvm/base.py:305-325) and the
 // BC applied to it (fvm/base.py:352-362,421-431).  `own` is the block's limited face value,
 // `ghost` the ghost cell value (the ghost block's limited value with phi = 0, grad = 0).
@@ -132,24 +143,30 @@ __host__ __device__ inline Q4 outer_state(const DevBlock& B, int side, const Q4&
     return phx::reflect(own, c, s);
 }
 
-template <bool CART, int FLUX>
+// Face flux of face states in the reconstruction variables; GEN = false: conservative states.
+template <int FLUX, bool GEN>
+__host__ __device__ inline Q4 flux_of(const Q4& L, const Q4& R, const Phys& ph) {
+    return GEN ? phx::face_flux_recon<FLUX>(L, R, ph) : phx::face_flux<FLUX>(L, R, ph);
+}
+
+template <bool CART, int FLUX, bool GEN = false>
 __host__ __device__ inline Q4 xflux(const DevBlock& B, const Q4& L, const Q4& R, int k, int jf, const Phys& ph) {
-    if (CART) return phx::face_flux<FLUX>(L, R, ph);
+    if (CART) return flux_of<FLUX, GEN>(L, R, ph);
     const GenG G{B.geom, B.nx, B.ny};
     const double c = G.V(F_c, k, jf), s = G.V(F_s, k, jf);
-    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    const Q4 F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
     return phx::unrotate(F, c, s);
 }
 
-template <bool CART, int FLUX>
+template <bool CART, int FLUX, bool GEN = false>
 __host__ __device__ inline Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R, int kf, int j, const Phys& ph) {
     if (CART) {
-        const Q4 F = phx::face_flux<FLUX>(phx::rotate90(L), phx::rotate90(R), ph);
+        const Q4 F = flux_of<FLUX, GEN>(phx::rotate90(L), phx::rotate90(R), ph);
         return phx::unrotate90(F);
     }
     const GenG G{B.geom, B.nx, B.ny};
     const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
-    const Q4 F = phx::face_flux<FLUX>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    const Q4 F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
     return phx::unrotate(F, c, s);
 }
 

@@ -144,8 +144,8 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
     const int kr1 = min(it.r1, ny);  // row ny is a virtual row closing the north boundary face
 
     Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
-    if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, kr0 - 1, j);
-    if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, kr0, j);
+    if (inner) Um = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0 - 1, j, ph);
+    if (valid) Uc = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0, j, ph);
 
     Q4 QNprev = phx::q4zero(), part = phx::q4zero();
     double prevLN = 0.0, prevA = 1.0;
@@ -155,8 +155,8 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
     for (int k = kr0; k <= kr1; ++k) {
         const bool real = (k < ny);
         if (real) {
-            if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
-            else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+            if (inner) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+            else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
         }
         const Q4 Uw = shfl_up1(Uc);
         const Q4 Ue = shfl_down1(Uc);
@@ -173,7 +173,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
         if (xrow && xface) {
             const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
             const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
-            FW = xflux<CART, FLUX>(B, L, R, k, j, ph);
+            FW = xflux<CART, FLUX, GEN>(B, L, R, k, j, ph);
         }
         const Q4 FE = shfl_down1(FW);  // flux of face j+1 (east face of cell j)
 
@@ -181,7 +181,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
             // face between rows k-1 and k (south face of row k, north face of row k-1)
             const Q4 L = (k == 0) ? outer_state<CART>(B, PHX_S, r.S, Um, k, j) : QNprev;
             const Q4 R = (k == ny) ? outer_state<CART>(B, PHX_N, QNprev, Uc, k - 1, j) : r.S;
-            const Q4 F = yflux<CART, FLUX>(B, L, R, k, j, ph);
+            const Q4 F = yflux<CART, FLUX, GEN>(B, L, R, k, j, ph);
             if (k - 1 >= it.r0) {
                 // residual of row k-1: 0.5*(W - E + S - N)/A with I = L*(2*F) (fvm/base.py:141-190)
                 const int kk = k - 1;
@@ -278,13 +278,13 @@ __device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, 
         const int kr0 = bottom ? max(a - 1, 0) : a;
         const int kr1 = top ? min(it.r1, ny) : b - 1;
         Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
-        if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, kr0 - 1, j);
-        if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, kr0, j);
+        if (inner) Um = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0 - 1, j, ph);
+        if (valid) Uc = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0, j, ph);
         for (int k = kr0; k <= kr1; ++k) {
             const bool real = (k < ny);
             if (real) {
-                if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
-                else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+                if (inner) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+                else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
             }
             const Q4 Uw = shfl_up1(Uc);
             const Q4 Ue = shfl_down1(Uc);
@@ -301,7 +301,7 @@ __device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, 
             if (xrow && xface) {
                 const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
                 const Q4 R2 = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
-                FW = xflux<CART, FLUX>(B, L, R2, k, j, ph);
+                FW = xflux<CART, FLUX, GEN>(B, L, R2, k, j, ph);
             }
             const Q4 FE = shfl_down1(FW);
             if (k >= a && owner) {
@@ -310,7 +310,7 @@ __device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, 
                 if (!defer_S) {
                     const Q4 L = (k == 0) ? outer_state<CART>(B, PHX_S, r.S, Um, k, j) : QNprev;
                     const Q4 R2 = (k == ny) ? outer_state<CART>(B, PHX_N, QNprev, Uc, k - 1, j) : r.S;
-                    F = yflux<CART, FLUX>(B, L, R2, k, j, ph);
+                    F = yflux<CART, FLUX, GEN>(B, L, R2, k, j, ph);
                     if (k - 1 >= a) {
                         if (k - 1 == a && !bottom) {
                             first_FN = F;
@@ -351,7 +351,7 @@ __device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, 
     Q4 FS_first = phx::q4zero();
     if (active && !bottom && owner) {
         // face between a-1 (last row of wave w-1) and a
-        FS_first = yflux<CART, FLUX>(B, lds_N[(w - 1) * WAVE + lane], first_S, a, j, ph);
+        FS_first = yflux<CART, FLUX, GEN>(B, lds_N[(w - 1) * WAVE + lane], first_S, a, j, ph);
         lds_F[w * WAVE + lane] = FS_first;
     }
     __syncthreads();
@@ -417,11 +417,11 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
     const Phys ph = A.ph;
 
     Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
-    if (inner) Um = fetch<CART>(A.blocks, B, A.Uin, it.r0 - 1, j);
-    if (valid) Uc = fetch<CART>(A.blocks, B, A.Uin, it.r0, j);
+    if (inner) Um = fetch_r<CART, GEN>(A.blocks, B, A.Uin, it.r0 - 1, j, ph);
+    if (valid) Uc = fetch_r<CART, GEN>(A.blocks, B, A.Uin, it.r0, j, ph);
     for (int k = it.r0; k < it.r1; ++k) {
-        if (inner) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
-        else if (gcol && k + 1 < it.r1) Up = fetch<CART>(A.blocks, B, A.Uin, k + 1, j);
+        if (inner) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+        else if (gcol && k + 1 < it.r1) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
         const Q4 Uw = shfl_up1(Uc);
         const Q4 Ue = shfl_down1(Uc);
         phx::Recon r;
@@ -436,7 +436,7 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
         if (xface) {
             const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
             const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
-            FW = xflux<CART, FLUX>(B, L, R, k, j, ph);
+            FW = xflux<CART, FLUX, GEN>(B, L, R, k, j, ph);
         }
         const Q4 FE = shfl_down1(FW);
         if (owner) {
@@ -457,7 +457,7 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
 
 // y-faces of face rows [r0, r1) of a strip of 64 columns (face f between rows f-1 and f; f = 0 and
 // f = ny take the outer states of the S and N boundaries): one flux per lane and face, stored.
-template <bool CART, int FLUX>
+template <bool CART, int FLUX, bool GEN>
 __device__ void yface_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
@@ -468,11 +468,11 @@ __device__ void yface_item(const StageArgs& A, const WorkItem it) {
     const double* sn = A.sn + 2 * B.off;
     for (int f = it.r0; f < it.r1; ++f) {
         Q4 L, R;
-        if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch<CART>(A.blocks, B, A.Uin, -1, j), 0, j);
+        if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch_r<CART, GEN>(A.blocks, B, A.Uin, -1, j, ph), 0, j);
         else L = ld4(sn + 8 * ((long)(f - 1) * nx + j));
-        if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch<CART>(A.blocks, B, A.Uin, ny, j), ny - 1, j);
+        if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch_r<CART, GEN>(A.blocks, B, A.Uin, ny, j, ph), ny - 1, j);
         else R = ld4(sn + 8 * ((long)f * nx + j) + 4);
-        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), yflux<CART, FLUX>(B, L, R, f, j, ph));
+        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), yflux<CART, FLUX, GEN>(B, L, R, f, j, ph));
     }
 }
 
@@ -513,11 +513,11 @@ __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, A.items[i]);
 }
 
-template <bool CART, int FLUX>
+template <bool CART, int FLUX, bool GEN>
 __global__ void __launch_bounds__(WAVE) k_yface(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) yface_item<CART, FLUX>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) yface_item<CART, FLUX, GEN>(A, A.items[i]);
 }
 
 template <bool CART>
@@ -786,7 +786,7 @@ struct k_recon_x_tag {
 struct k_yface_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_yface<C, F>), g, dim3(WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_yface<C, F, G>), g, dim3(WAVE), 0, st, A);
     }
 };
 struct k_update_tag {
@@ -805,7 +805,8 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
     A.nitems = nitems;
     const dim3 g(grid);
     const bool roe = P->flux == PHX_FLUX_ROE;
-    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN;
+    // generic instantiation: another limiter or primitive-variable reconstruction
+    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN || P->ph.prim;
 #define PHX_LAUNCH_M(C, F)                                                                       \
     do {                                                                                         \
         if (gen) T::template launch<C, F, 2, true>(g, st, A);                                    \
@@ -952,6 +953,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         return fail(PHX_E_INVALID, "Flux function type is not available.");
     if (D->limiter < PHX_LIMITER_VENKATAKRISHNAN || D->limiter > PHX_LIMITER_VAN_LEER)
         return fail(PHX_E_INVALID, "MUSCL: Slope limiter type not specified.");
+    if (D->reconstruction != PHX_RECON_CONSERVATIVE && D->reconstruction != PHX_RECON_PRIMITIVE)
+        return fail(PHX_E_INVALID, "unknown reconstruction type");
     if (D->nstages < 1 || D->nstages > PHX_MAX_STAGES || !D->butcher_a)
         return fail(PHX_E_INVALID, "invalid Butcher table");
     if (!D->t_final) return fail(PHX_E_INVALID, "t_final is null");
@@ -1005,7 +1008,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     P->nblocks = D->nblocks;
     P->ncases = D->ncases;
     P->flux = D->flux;
-    P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl, D->limiter};
+    P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl, D->limiter,
+                 D->reconstruction == PHX_RECON_PRIMITIVE ? 1 : 0};
     P->desc.assign(D->blocks, D->blocks + D->nblocks);
     P->state_len = total;
     P->rows_per_item = D->rows_per_item > 0 ? D->rows_per_item : 32;

@@ -21,6 +21,7 @@ struct Q4 {
 struct Phys {
     double gamma, gm1, one_over_gm1, g_over_gm1, cfl;
     int limiter;  // PHX_LIMITER_* (0 = Venkatakrishnan)
+    int prim;     // reconstruction in primitive variables (SolverConfig.reconstruction_type)
 };
 
 struct Prim {
@@ -212,6 +213,22 @@ __host__ __device__ inline Q4 face_flux(const Q4& qL, const Q4& qR, const Phys& 
     const Prim R = cons_to_prim(qR, ph);
     if (FLUX == 0) return roe_flux(L, R, ph);
     return hlll_flux(L, R, ph);
+}
+
+__host__ __device__ inline Prim as_prim(const Q4& q) { return Prim{q.v[0], q.v[1], q.v[2], q.v[3]}; }
+__host__ __device__ inline Q4 prim_q4(const Prim& w) {
+    Q4 q;
+    q.v[0] = w.r; q.v[1] = w.u; q.v[2] = w.v; q.v[3] = w.p;
+    return q;
+}
+
+// face_flux on face states in the reconstruction variables: with PrimitiveState reconstruction the
+// states are primitive already and go to the flux as they are (fvm/base.py:286-289).
+template <int FLUX>
+__host__ __device__ inline Q4 face_flux_recon(const Q4& qL, const Q4& qR, const Phys& ph) {
+    if (!ph.prim) return face_flux<FLUX>(qL, qR, ph);
+    if (FLUX == 0) return roe_flux(as_prim(qL), as_prim(qR), ph);
+    return hlll_flux(as_prim(qL), as_prim(qR), ph);
 }
 
 // Geometry of one cell as seen by the reconstruction: GG face weights x/yl = L*n (per face),

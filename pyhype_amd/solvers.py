@@ -17,7 +17,7 @@ import numpy as np
 
 from .backend import CaseSpec, DevicePlan
 from .mesh import MeshGenerator
-from .states import ConservativeState
+from .states import ConservativeState, PrimitiveState
 from .time_marching import butcher_table
 
 _log = logging.getLogger("pyhype_amd")
@@ -66,8 +66,8 @@ def validate_config(config) -> List[List[float]]:
         raise ValueError(f"Flux function type {config.fvm_flux_function_type} is not available.")
     if config.fvm_slope_limiter_type not in ("VanLeer", "VanAlbada", "Venkatakrishnan", "BarthJespersen"):
         raise ValueError("MUSCL: Slope limiter type not specified.")
-    if config.reconstruction_type is not ConservativeState:
-        raise ValueError("only ConservativeState reconstruction is implemented")
+    if config.reconstruction_type not in (ConservativeState, PrimitiveState):
+        raise ValueError("reconstruction_type must be ConservativeState or PrimitiveState")
     if config.interface_interpolation != "arithmetic_average":
         raise ValueError("Interface Interpolation method is not defined.")
     return butcher_table(config.time_integrator, config)
@@ -94,7 +94,8 @@ class Euler2DBatch:
         self.table = validate_config(c0)
         for c in configs[1:]:
             validate_config(c)
-            for attr in ("fvm_flux_function_type", "fvm_slope_limiter_type", "time_integrator", "CFL"):
+            for attr in ("fvm_flux_function_type", "fvm_slope_limiter_type", "reconstruction_type", "time_integrator",
+                         "CFL"):
                 if getattr(c, attr) != getattr(c0, attr):
                     raise ValueError(f"batched cases must share {attr}")
             if type(c.fluid) is not type(c0.fluid):
@@ -104,7 +105,8 @@ class Euler2DBatch:
         specs = [CaseSpec(mesh_dict(m), c.nx, c.ny, c.t_final * c.fluid.far_field.a) for c, m in zip(configs, meshes)]
         self.plan = DevicePlan(specs, flux=c0.fvm_flux_function_type, table=self.table, fluid=c0.fluid,
                                cfl=c0.CFL, rows_per_item=rows_per_item, grid_waves=grid_waves,
-                               force_general=force_general, limiter=c0.fvm_slope_limiter_type)
+                               force_general=force_general, limiter=c0.fvm_slope_limiter_type,
+                               primitive=c0.reconstruction_type is PrimitiveState)
         self.tuned = False
         if rows_per_item == 0:
             from . import autotune

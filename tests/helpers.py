@@ -13,7 +13,9 @@ def oracle_for(config, mesh, states0):
     sim = O.Simulation(mesh_dict(mesh), config.nx, config.ny, flux=config.fvm_flux_function_type,
                        integrator=butcher_table(config.time_integrator, config), CFL=config.CFL,
                        t_final=config.t_final, a_inf=config.fluid.far_field.a,
-                       limiter=config.fvm_slope_limiter_type)
+                       limiter=config.fvm_slope_limiter_type,
+                       reconstruction="Primitive" if config.reconstruction_type.__name__ == "PrimitiveState"
+                       else "Conservative")
     sim.set_state({n: np.array(a) for n, a in states0.items()})
     return sim
 

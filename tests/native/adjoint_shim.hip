@@ -15,8 +15,8 @@ using phx::host::AdjBlock;
 
 namespace {
 
-// p: gamma, gm1, 1/gm1, gamma/gm1, CFL, limiter id
-Phys make_phys(const double* p) { return Phys{p[0], p[1], p[2], p[3], p[4], (int)p[5]}; }
+// p: gamma, gm1, 1/gm1, gamma/gm1, CFL, limiter id, primitive reconstruction (0/1)
+Phys make_phys(const double* p) { return Phys{p[0], p[1], p[2], p[3], p[4], (int)p[5], (int)p[6]}; }
 Q4 q4(const double* p) { Q4 q; for (int k = 0; k < 4; ++k) q.v[k] = p[k]; return q; }
 void put(double* p, const Q4& q) { for (int k = 0; k < 4; ++k) p[k] = q.v[k]; }
 
@@ -48,15 +48,15 @@ HostPlan make_plan(const phx_block_desc* d, int nb) {
     return P;
 }
 
-void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, int lim) {
+void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, const Phys& ph) {
     rb.assign((size_t)P.cells * adj::RB, 0.0);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 double* out = rb.data() + adj::RB * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true>(P.blocks.data(), B, U, k, j, out, lim);
-                else adj::recon_cell<false>(P.blocks.data(), B, U, k, j, out, lim);
+                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true>(P.blocks.data(), B, U, k, j, out, ph);
+                else adj::recon_cell<false>(P.blocks.data(), B, U, k, j, out, ph);
             }
     }
 }
@@ -64,7 +64,7 @@ void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, int 
 template <int FLUX>
 void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
     std::vector<double> rb, flux((size_t)P.faces * 4);
-    recon_all(P, U, rb, ph.limiter);
+    recon_all(P, U, rb, ph);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -86,7 +86,7 @@ void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
 template <int FLUX>
 void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, double* lamU, const Phys& ph) {
     std::vector<double> rb, fadj((size_t)P.faces * adj::FA), slot((size_t)P.cells * adj::SLOT, 0.0);
-    recon_all(P, U, rb, ph.limiter);
+    recon_all(P, U, rb, ph);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -104,9 +104,9 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
             for (int j = 0; j < B.nx; ++j) {
                 double* out = slot.data() + adj::SLOT * adj::cell_index(P.offs[b], B, k, j);
                 if (B.kind == PHX_GEOM_CARTESIAN)
-                    adj::cell_adj<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph.limiter);
+                    adj::cell_adj<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
                 else
-                    adj::cell_adj<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph.limiter);
+                    adj::cell_adj<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
             }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
@@ -114,8 +114,8 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 const Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
-                                   ? adj::gather_adj<true>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data())
-                                   : adj::gather_adj<false>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data());
+                                   ? adj::gather_adj<true>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data(), U, ph)
+                                   : adj::gather_adj<false>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data(), U, ph);
                 put(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
             }
     }

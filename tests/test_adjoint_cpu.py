@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 from oracle import pyhype_oracle as O
+from pyhype_amd import PrimitiveState
 from pyhype_amd import workloads as W
 from pyhype_amd.backend import CaseSpec, layout_cases
 from pyhype_amd.solvers import mesh_dict
@@ -31,12 +32,18 @@ LIMITER_IDS = {"Venkatakrishnan": 0, "BarthJespersen": 1, "VanAlbada": 2, "VanLe
 def _phys(cfg):
     f = cfg.fluid
     return np.array([f.gamma(), f.gm1(), f.one_over_gm1(), f.g_over_gm1(), cfg.CFL,
-                     LIMITER_IDS[cfg.fvm_slope_limiter_type]])
+                     LIMITER_IDS[cfg.fvm_slope_limiter_type], cfg.reconstruction_type is PrimitiveState])
 
 
 def _with_limiter(case, limiter):
     (cfg, mesh), general = case
     cfg.fvm_slope_limiter_type = limiter
+    return (cfg, mesh), general
+
+
+def _primitive(case):
+    (cfg, mesh), general = case
+    cfg.reconstruction_type = PrimitiveState
     return (cfg, mesh), general
 
 
@@ -50,6 +57,11 @@ CASES = {
     "single_barth": lambda: _with_limiter((W.explosion_single(10), False), "BarthJespersen"),
     "multiblock_vanalbada": lambda: _with_limiter((W.explosion_multiblock(6, 2, 4), False), "VanAlbada"),
     "oblique_vanleer": lambda: _with_limiter((W.oblique_shock(8, 3), False), "VanLeer"),
+    "single_prim": lambda: _primitive((W.explosion_single(10), False)),
+    "multiblock_prim": lambda: _primitive((W.explosion_multiblock(6, 2, 4), False)),
+    "oblique_hlll_prim": lambda: _primitive((W.oblique_shock(8, 3), False)),
+    "skewed_prim_vanalbada": lambda: _primitive(_with_limiter(((W.explosion_single(8)[0], skewed_mesh()), False),
+                                                              "VanAlbada")),
 }
 
 
@@ -103,7 +115,8 @@ def shim(adjoint_shim):
 
 
 @pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll", "skewed", "single_barth",
-                                  "multiblock_vanalbada", "oblique_vanleer"])
+                                  "multiblock_vanalbada", "oblique_vanleer", "single_prim", "multiblock_prim",
+                                  "oblique_hlll_prim", "skewed_prim_vanalbada"])
 def test_elementwise_residual_equals_oracle(shim, name):
     """(Forcing the general path on a Cartesian mesh changes the arithmetic, as it would in the
     reference, so only the meshes' own geometry kinds are compared.)"""
@@ -133,7 +146,7 @@ def _cons(rng, n, mach):
 def test_face_flux_vjp_matches_central_differences(shim, flux, mach):
     rng = np.random.default_rng(10 + flux + int(10 * mach))
     n = 400
-    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8, 0])
+    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8, 0, 0])
     qL, qR = _cons(rng, n, mach), _cons(rng, n, mach)
     qR = qL + 0.3 * (qR - qL)  # nearby states, the usual regime of a face
     lam = rng.standard_normal((n, 4))

@@ -320,3 +320,29 @@ def test_shared_halo_variant_batched_cases_and_limiter():
     for x, y in zip(a.states(), b.states()):
         assert bitwise_equal_blocks(x, y)
     assert a.failed == b.failed
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("case", ["multiblock", "oblique"])
+def test_primitive_reconstruction_matches_oracle(case, variant):
+    """reconstruction_type = PrimitiveState (fvm/base.py:286-289, blocks/quad_block.py:225-240):
+    residual and steps bit for bit on every stage variant."""
+    from pyhype_amd import PrimitiveState
+
+    if case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(14, 2, 4)
+    else:
+        cfg, mesh = W.oblique_shock(18, 3)
+    cfg.reconstruction_type = PrimitiveState
+    s = _solver(cfg, mesh, rows_per_item=2)
+    if variant:
+        s.plan.set_launch(2, 0, 3 if variant == 1 else 2, variant=variant, rows_b=4)
+    st = _perturbed(s.states()[0], seed=17)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(5)
+    sim.run(5)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
