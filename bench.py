@@ -249,9 +249,10 @@ def main():
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
-    split = getattr(plan, "launch", {}).get("variant", 0) == 1
-    kernel = ("k_recon_x + k_yface + k_update (split stage: one RK stage over all blocks)" if split
-              else "k_stage (one RK stage over all blocks)")
+    variant = plan.launch_info().get("variant", 0)
+    kernel = {1: "k_recon_x + k_yface + k_update (split stage: one RK stage over all blocks)",
+              2: "k_stage_wg (shared-halo stage: one RK stage over all blocks)"}.get(
+                  variant, "k_stage (one RK stage over all blocks)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": kernel, "stage_ms": round(stage_ms, 5),
