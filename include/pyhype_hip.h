@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define PHX_ABI_VERSION 2
+#define PHX_ABI_VERSION 3
 
 /* error codes */
 #define PHX_OK 0
@@ -64,6 +64,9 @@ extern "C" {
 
 #define PHX_MAX_STAGES 8
 
+/* Gauss points per face (SolverConfig.fvm_num_quadrature_points; mesh/quadratures.py:33-37) */
+#define PHX_MAX_QUADRATURE_POINTS 3
+
 /*
  * Packed geometry, all values in FP64, produced on the host by pyhype_amd.geometry from the
  * reference's own formulas (mesh/quad_mesh.py:71-212, mesh/base.py:29-78,
@@ -81,6 +84,11 @@ extern "C" {
  *       cells j-1 and j; the west face of a cell uses -xl, -yl)
  *   horizontal faces (ny+1) x nx, 7 arrays: L xl yl xq yq c s (north orientation)
  *   cells ny x nx, 5 arrays: A xc yc dx dy
+ * With nq = quadrature_points > 1 either layout is followed by the Gauss-point offsets of every
+ * cell, 8*nq arrays of ny x nx: for face F in E, W, N, S, point i < nq (rule order), then
+ * component x, y: xq_{F,i} - xc, yq_{F,i} - yc, with the points of
+ * QuadraturePointData._transform (mesh/quadratures.py:56-95); the midpoint entries above are
+ * then unused by the reconstruction.
  */
 typedef struct {
     int32_t nx, ny;
@@ -111,6 +119,8 @@ typedef struct {
     int32_t rows_per_item;    /* rows of one work item; 0 = default */
     int32_t grid_waves;       /* waves of the persistent grid; 0 = one per work item */
     int32_t reconstruction;   /* PHX_RECON_* (SolverConfig.reconstruction_type) */
+    int32_t quadrature_points; /* Gauss points per face, 1..PHX_MAX_QUADRATURE_POINTS; > 1 runs
+                                  the split stage (phx_set_variant 1) and no adjoint */
 } phx_plan_desc;
 
 typedef struct phx_plan phx_plan;
@@ -173,8 +183,9 @@ int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host,
 
 /* Reverse-mode product of the residual: lamU = (dR/dU)^T lamR at state U (discrete adjoint of
  * fvm/base.py:108-190 with the active limiter/flux branches; deterministic, no atomics).  Requires
- * mutual neighbour links.  The first adjoint call allocates the plan's adjoint work arrays
- * (about 60 doubles per cell).  No counterpart in the reference. */
+ * mutual neighbour links and the 1-point Gauss rule (PHX_E_INVALID otherwise).  The first adjoint
+ * call allocates the plan's adjoint work arrays (about 60 doubles per cell).  No counterpart in
+ * the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
 
 /* Reverse-mode product of one Runge-Kutta step with a frozen time step: on entry lam holds the
@@ -196,8 +207,10 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * reconstruction + x-face kernel, a y-face kernel and a residual + update kernel, which evaluates
  * every reconstruction and face flux exactly once (best for small meshes, where short fused items
  * spend most of their work on halo rows); rows_b = rows per work item of the update kernel
- * (0 = keep).  Both variants give identical results.  Rebuilds the work lists; the split variant
- * allocates about 16 doubles per cell.  No counterpart in the reference. */
+ * (0 = keep).  All variants give identical results.  Rebuilds the work lists; the split variant
+ * allocates about 8*nq + 8 doubles per cell.  A plan with more than one Gauss point per face runs
+ * the split variant only (variants 0 and 2 return PHX_E_INVALID).  No counterpart in the
+ * reference. */
 int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 
 /* Number of work items and the persistent grid used for the Cartesian and general kernels. */

@@ -116,6 +116,15 @@ class Face:
     qy: np.ndarray
 
 
+# Gauss rules on [-1, 1] as (point, weight) in the reference's order and with its expressions
+# (mesh/quadratures.py:33-37): the weights sum to 2; integrate_flux multiplies by L and _dUdt_JIT by 0.5.
+QUADS = {
+    1: ((0, 2),),
+    2: ((-1 / np.sqrt(3), 1), (1 / np.sqrt(3), 1)),
+    3: ((-np.sqrt(3 / 5), 5 / 9), (0, 8 / 9), (np.sqrt(3 / 5), 5 / 9)),
+}
+
+
 @dataclass
 class Geometry:
     nx: int
@@ -130,6 +139,8 @@ class Geometry:
     face: Dict[str, Face]
     is_cartesian: bool
     vertices: dict
+    # Gauss points per face: [(x, y, w)] in rule order (QuadraturePointData, mesh/quadratures.py:60-95)
+    qpts: Dict[str, list] = None
 
 
 def _cellface(hx, hy, lx, ly, vertical: bool, direction: int):
@@ -154,13 +165,14 @@ def _face_length_jit(x1, x2, y1, y2):
     return np.sqrt(dx * dx + dy * dy)
 
 
-def _qp(p1, p2):
-    """QuadraturePointData._transform with the 1-point rule p=0 (mesh/quadratures.py:33,56-58)."""
-    return 0.5 * ((p2 - p1) * 0 + (p2 + p1))
+def _qp(p1, p2, p=0):
+    """QuadraturePointData._transform (mesh/quadratures.py:56-58) at the rule point p."""
+    return 0.5 * ((p2 - p1) * p + (p2 + p1))
 
 
-def block_geometry(vert: dict, nx: int, ny: int) -> Geometry:
-    """QuadMesh.create_mesh (mesh/quad_mesh.py:71-135)."""
+def block_geometry(vert: dict, nx: int, ny: int, nq: int = 1) -> Geometry:
+    """QuadMesh.create_mesh (mesh/quad_mesh.py:71-135) and the nq-point Gauss points of every face
+    (QuadraturePointData._make_quadrature, mesh/quadratures.py:60-95)."""
     SW, SE, NW, NE = vert["SW"], vert["SE"], vert["NW"], vert["NE"]
     Ex = np.linspace(SE[0], NE[0], ny + 1)
     Ey = np.linspace(SE[1], NE[1], ny + 1)
@@ -211,7 +223,12 @@ def block_geometry(vert: dict, nx: int, ny: int) -> Geometry:
     dx = faces["E"].mx - faces["W"].mx
     dy = faces["N"].my - faces["S"].my
     cart = (NE[1] == NW[1]) and (SE[1] == SW[1]) and (SE[0] == NE[0]) and (SW[0] == NW[0])
-    return Geometry(nx, ny, x, y, xc, yc, A, dx, dy, faces, bool(cart), vert)
+    # (p1, p2) vertex pairs of each face: E (NE, SE), W (NW, SW), N (NE, NW), S (SE, SW)
+    ends = {"E": ((xne, yne), (xse, yse)), "W": ((xnw, ynw), (xsw, ysw)),
+            "N": ((xne, yne), (xnw, ynw)), "S": ((xse, yse), (xsw, ysw))}
+    qpts = {d: [(_qp(a[0], b[0], p), _qp(a[1], b[1], p), w) for p, w in QUADS[nq]]
+            for d, (a, b) in ends.items()}
+    return Geometry(nx, ny, x, y, xc, yc, A, dx, dy, faces, bool(cart), vert, qpts)
 
 
 # --------------------------------------------------------------------------------------
@@ -501,10 +518,12 @@ def green_gauss(U, ghost, g: Geometry):
     return gx, gy
 
 
-def high_order_term(gx, gy, g: Geometry, d: str):
-    """FirstOrderGradients.get_high_order_term_JIT (blocks/base.py:261-289)."""
-    ddx = g.face[d].qx - g.xc
-    ddy = g.face[d].qy - g.yc
+def high_order_term(gx, gy, g: Geometry, d: str, i: int = 0):
+    """FirstOrderGradients.get_high_order_term_JIT (blocks/base.py:261-289) at Gauss point i of
+    face d."""
+    qx, qy, _ = g.qpts[d][i]
+    ddx = qx - g.xc
+    ddy = qy - g.yc
     return gx * ddx[..., None] + gy * ddy[..., None]
 
 
@@ -546,9 +565,11 @@ def compute_slope(dmax, dmin, d):
     return s
 
 
-def limiter_phi(U, ghost, hot: Dict[str, np.ndarray], limiter_fn=venkatakrishnan):
+def limiter_phi(U, ghost, hot: Dict[str, object], limiter_fn=venkatakrishnan):
     """SlopeLimiter._get_slope / _limit (limiters/base.py:47-108,147-187) with the limiter
-    function of the configured limiter (default Venkatakrishnan)."""
+    function of the configured limiter (default Venkatakrishnan).  hot[d]: the high-order term of
+    face d, or a list of them, one per Gauss point; phi is the minimum over the faces' points in
+    the order E, W, N, S (each face's points in rule order)."""
     EW = np.concatenate((ghost["W"], U, ghost["E"]), axis=1)
     NS = np.concatenate((ghost["S"], U, ghost["N"]), axis=0)
     vals = (U, EW[:, :-2], EW[:, 2:], NS[:-2, :], NS[2:, :])
@@ -556,10 +577,11 @@ def limiter_phi(U, ghost, hot: Dict[str, np.ndarray], limiter_fn=venkatakrishnan
     dmin = np.minimum.reduce(vals) - U
     phis = []
     for d in DIRS:
-        q = U + hot[d]          # unlimited solution at the quadrature point
-        dd = q - U
-        with np.errstate(divide="ignore", invalid="ignore"):
-            phis.append(limiter_fn(compute_slope(dmax, dmin, dd)))
+        for h in (hot[d] if isinstance(hot[d], (list, tuple)) else (hot[d],)):
+            q = U + h           # unlimited solution at the quadrature point
+            dd = q - U
+            with np.errstate(divide="ignore", invalid="ignore"):
+                phis.append(limiter_fn(compute_slope(dmax, dmin, dd)))
     phi = np.minimum.reduce(tuple(phis))
     return np.where(phi < 0, 0, phi)
 
@@ -579,8 +601,9 @@ def _outer_state(b: Block, d: str, limited_boundary: np.ndarray, ghost=None):
 
 
 def residual(b: Block, flux_fn):
-    """FiniteVolumeMethod.dUdt (fvm/base.py:108-165) for SecondOrderMUSCL with Conservative
-    reconstruction (MUSCL.evaluate_flux, fvm/base.py:327-500)."""
+    """FiniteVolumeMethod.dUdt (fvm/base.py:108-165) for SecondOrderMUSCL (MUSCL.evaluate_flux,
+    fvm/base.py:327-500) with the block geometry's Gauss rule: one flux evaluation per face and
+    Gauss point, the points of a face paired in rule order (fvm/base.py:349-356,411-418)."""
     # recon_block.from_block (blocks/quad_block.py:120-134): the interior and ghost states in the
     # reconstruction type -- a copy (ConservativeState) or ConservativeConverter.to_primitive
     # (PrimitiveState; concrete_defs.py:85-100)
@@ -594,55 +617,65 @@ def residual(b: Block, flux_fn):
     # flux function as they are
     to_flux = (lambda a: a) if b.prim else cons_to_prim
     g = b.geom
+    nq = len(g.qpts["E"])
     gx, gy = green_gauss(U, ghost, g)
-    hot = {d: high_order_term(gx, gy, g, d) for d in DIRS}
+    hot = {d: [high_order_term(gx, gy, g, d, i) for i in range(nq)] for d in DIRS}
     phi = limiter_phi(U, ghost, hot, b.limiter_fn or venkatakrishnan)
-    face = {d: U + phi * hot[d] for d in DIRS}   # limited_solution_at_quadrature_point
-    bnd = {"E": face["E"][:, -1:], "W": face["W"][:, :1], "N": face["N"][-1:], "S": face["S"][:1]}
-    # ---- east-west sweep (fvm/base.py:327-390)
-    outE = _outer_state(b, "E", bnd["E"], ghost)
-    outW = _outer_state(b, "W", bnd["W"], ghost)
-    fE = face["E"].copy()
-    fW = face["W"].copy()
-    if not g.is_cartesian:
-        rotate(g.face["E"].theta, fE)
-        rotate(g.face["W"].theta, fW)
-        rotate(boundary_theta(g, "E"), outE)
-        rotate(boundary_theta(g, "W"), outW)
-    left = to_flux(np.concatenate((outW, fE), axis=1))
-    right = to_flux(np.concatenate((fW, outE), axis=1))
-    Fx = flux_fn(left, right)
-    flux_E = Fx[:, 1:].copy()
-    flux_W = Fx[:, :-1].copy()
-    if not g.is_cartesian:
-        unrotate(g.face["E"].theta, flux_E)
-        unrotate(g.face["W"].theta, flux_W)
-    # ---- north-south sweep (fvm/base.py:392-486)
-    outN = _outer_state(b, "N", bnd["N"], ghost)
-    outS = _outer_state(b, "S", bnd["S"], ghost)
-    fN = face["N"].copy()
-    fS = face["S"].copy()
-    if g.is_cartesian:
-        rotate90(fN, fS, outN, outS)
-    else:
-        rotate(g.face["N"].theta, fN)
-        rotate(g.face["S"].theta, fS)
-        rotate(boundary_theta(g, "N"), outN)
-        rotate(boundary_theta(g, "S"), outS)
-    left = to_flux(np.concatenate((outS.transpose(1, 0, 2), fN.transpose(1, 0, 2)), axis=1))
-    right = to_flux(np.concatenate((fS.transpose(1, 0, 2), outN.transpose(1, 0, 2)), axis=1))
-    Fy = flux_fn(left, right).transpose(1, 0, 2)
-    flux_N = Fy[1:].copy()
-    flux_S = Fy[:-1].copy()
-    if g.is_cartesian:
-        unrotate90(flux_N, flux_S)
-    else:
-        unrotate(g.face["N"].theta, flux_N)
-        unrotate(g.face["S"].theta, flux_S)
-    # ---- integrate + dUdt (fvm/base.py:118-139,141-190)
+    fluxes = {d: [] for d in DIRS}
+    for i in range(nq):
+        # limited_solution_at_quadrature_point at Gauss point i of every face
+        face = {d: U + phi * hot[d][i] for d in DIRS}
+        bnd = {"E": face["E"][:, -1:], "W": face["W"][:, :1], "N": face["N"][-1:], "S": face["S"][:1]}
+        # ---- east-west sweep (fvm/base.py:327-390)
+        outE = _outer_state(b, "E", bnd["E"], ghost)
+        outW = _outer_state(b, "W", bnd["W"], ghost)
+        fE = face["E"].copy()
+        fW = face["W"].copy()
+        if not g.is_cartesian:
+            rotate(g.face["E"].theta, fE)
+            rotate(g.face["W"].theta, fW)
+            rotate(boundary_theta(g, "E"), outE)
+            rotate(boundary_theta(g, "W"), outW)
+        left = to_flux(np.concatenate((outW, fE), axis=1))
+        right = to_flux(np.concatenate((fW, outE), axis=1))
+        Fx = flux_fn(left, right)
+        flux_E = Fx[:, 1:].copy()
+        flux_W = Fx[:, :-1].copy()
+        if not g.is_cartesian:
+            unrotate(g.face["E"].theta, flux_E)
+            unrotate(g.face["W"].theta, flux_W)
+        # ---- north-south sweep (fvm/base.py:392-486)
+        outN = _outer_state(b, "N", bnd["N"], ghost)
+        outS = _outer_state(b, "S", bnd["S"], ghost)
+        fN = face["N"].copy()
+        fS = face["S"].copy()
+        if g.is_cartesian:
+            rotate90(fN, fS, outN, outS)
+        else:
+            rotate(g.face["N"].theta, fN)
+            rotate(g.face["S"].theta, fS)
+            rotate(boundary_theta(g, "N"), outN)
+            rotate(boundary_theta(g, "S"), outS)
+        left = to_flux(np.concatenate((outS.transpose(1, 0, 2), fN.transpose(1, 0, 2)), axis=1))
+        right = to_flux(np.concatenate((fS.transpose(1, 0, 2), outN.transpose(1, 0, 2)), axis=1))
+        Fy = flux_fn(left, right).transpose(1, 0, 2)
+        flux_N = Fy[1:].copy()
+        flux_S = Fy[:-1].copy()
+        if g.is_cartesian:
+            unrotate90(flux_N, flux_S)
+        else:
+            unrotate(g.face["N"].theta, flux_N)
+            unrotate(g.face["S"].theta, flux_S)
+        for d, fl in (("E", flux_E), ("W", flux_W), ("N", flux_N), ("S", flux_S)):
+            fluxes[d].append(fl)
+    # ---- integrate_flux: L * sum(w_i * F_i), the sum starting at 0 (fvm/base.py:168-190);
+    # dUdt (fvm/base.py:118-139,141-165)
     I = {}
-    for d, fl in (("E", flux_E), ("W", flux_W), ("N", flux_N), ("S", flux_S)):
-        I[d] = g.face[d].L[..., None] * (0 + 2 * fl)
+    for d in DIRS:
+        acc = 0
+        for (_, _, w), fl in zip(g.qpts[d], fluxes[d]):
+            acc = acc + w * fl
+        I[d] = g.face[d].L[..., None] * acc
     return 0.5 * (I["W"] - I["E"] + I["S"] - I["N"]) / g.A[..., None]
 
 
@@ -714,7 +747,8 @@ class Simulation:
 
     def __init__(self, mesh: Dict[int, dict], nx: int, ny: int, *, flux: str = "Roe",
                  integrator="RK4", CFL: float = 0.8, t_final: float = 1e30, a_inf: float = 1.0,
-                 limiter: str = "Venkatakrishnan", reconstruction: str = "Conservative"):
+                 limiter: str = "Venkatakrishnan", reconstruction: str = "Conservative",
+                 quadrature_points: int = 1):
         self.flux_fn = {"Roe": flux_roe, "HLLL": flux_hlll}[flux]
         self.limiter_fn = LIMITERS[limiter]
         self.a = BUTCHER[integrator] if isinstance(integrator, str) else integrator
@@ -726,7 +760,7 @@ class Simulation:
         self.blocks: Dict[int, Block] = {}
         for n, d in mesh.items():
             vert = {k: d[k] for k in ("SW", "SE", "NW", "NE")}
-            g = block_geometry(vert, nx, ny)
+            g = block_geometry(vert, nx, ny, quadrature_points)
             self.blocks[n] = Block(n, g, {s: d["Neighbor" + s] for s in DIRS},
                                    {s: d["BCType" + s] for s in DIRS}, limiter_fn=self.limiter_fn,
                                    prim=(reconstruction == "Primitive"))

@@ -9,7 +9,7 @@ from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libpyhype_hip.so"
 
-PHX_ABI_VERSION = 2
+PHX_ABI_VERSION = 3
 PHX_OK, PHX_E_INVALID, PHX_E_HIP, PHX_E_UNREALIZABLE, PHX_E_STATE = 0, -1, -2, -3, -4
 E, W, N, S = 0, 1, 2, 3
 SIDE_NEIGHBOR, SIDE_OPEN, SIDE_WALL, SIDE_OUTLET = 0, 1, 2, 3
@@ -46,6 +46,7 @@ class PlanDesc(C.Structure):
         ("one_over_gm1", C.c_double), ("g_over_gm1", C.c_double), ("cfl", C.c_double),
         ("t_final", C.POINTER(C.c_double)), ("rows_per_item", C.c_int32), ("grid_waves", C.c_int32),
         ("reconstruction", C.c_int32),
+        ("quadrature_points", C.c_int32),
     ]
 
 

@@ -36,6 +36,8 @@ def signature(plan) -> str:
     flux = plan.flux if lim == "Venkatakrishnan" else f"{plan.flux}/{lim}"
     if getattr(plan, "primitive", False):
         flux += "/prim"
+    if getattr(plan, "quadrature_points", 1) > 1:
+        flux += f"/q{plan.quadrature_points}"
     return f"{kinds}|{flux}|{len(plan.table)}|{shp}|{dev}"
 
 
@@ -85,7 +87,10 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
     apply and cache the best.  Returns the best shape, its ms/step and all trials."""
     import numpy as np
 
-    shapes = candidates() if shapes is None else shapes
+    if shapes is None:
+        shapes = candidates()
+        if getattr(plan, "quadrature_points", 1) > 1:   # Gauss faces run the split stage only
+            shapes = [c for c in shapes if c["variant"] == 1]
     U_saved = plan.U.clone()
     _, step0, t0, _ = plan.status()
     trials = []

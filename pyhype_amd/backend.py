@@ -99,10 +99,11 @@ class Layout:
         self.geoms, self.descs = geoms, descs
 
 
-def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layout:
+def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False, quadrature_points: int = 1) -> Layout:
     """State layout and ``phx_block_desc`` array of a list of cases (host only, no GPU needed).
     Blocks of a case are numbered in mesh-dictionary order; each block's state is its reference
-    ``(ny, nx, 4)`` array, blocks concatenated."""
+    ``(ny, nx, 4)`` array, blocks concatenated.  With ``quadrature_points`` > 1 each block's packed
+    geometry ends with its Gauss-point offsets."""
     blocks: List[BlockEntry] = []
     case_blocks: List[Dict[int, BlockEntry]] = []
     offset = 0
@@ -133,6 +134,8 @@ def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layo
         e.cartesian = packed is not None
         if packed is None:
             packed = e.geom.pack_general()
+        if quadrature_points > 1:
+            packed = np.concatenate([packed, e.geom.pack_quadrature(quadrature_points)])
         packed = np.ascontiguousarray(packed, dtype=np.float64)
         geoms.append(packed)
         dsc.nx, dsc.ny, dsc.case_id = e.nx, e.ny, e.case
@@ -149,7 +152,7 @@ def layout_cases(cases: Sequence[CaseSpec], force_general: bool = False) -> Layo
 class DevicePlan:
     def __init__(self, cases: Sequence[CaseSpec], *, flux: str, table: List[List[float]], fluid, cfl: float,
                  rows_per_item: int = 0, grid_waves: int = 0, force_general: bool = False,
-                 limiter: str = "Venkatakrishnan", primitive: bool = False):
+                 limiter: str = "Venkatakrishnan", primitive: bool = False, quadrature_points: int = 1):
         torch = _torch()
         self.torch = torch
         self.lib = nat.load()
@@ -162,6 +165,9 @@ class DevicePlan:
             raise ValueError("MUSCL: Slope limiter type not specified.")
         self.limiter = limiter
         self.primitive = bool(primitive)
+        if quadrature_points not in (1, 2, 3):
+            raise KeyError(quadrature_points)   # _QUADS[...] lookup (mesh/quadratures.py:71)
+        self.quadrature_points = int(quadrature_points)
         self.table = table
         S = len(table)
         a = np.zeros((S, S))
@@ -171,7 +177,7 @@ class DevicePlan:
             a[s, : s + 1] = row
         self._a = np.ascontiguousarray(a)
         self.nstages = S
-        lay = layout_cases(self.cases, force_general=force_general)
+        lay = layout_cases(self.cases, force_general=force_general, quadrature_points=self.quadrature_points)
         self.blocks, self.case_blocks, self.state_len = lay.blocks, lay.case_blocks, lay.state_len
         self._geoms, descs = lay.geoms, lay.descs
         self._tfinal = np.array([c.t_final for c in self.cases], dtype=np.float64)
@@ -184,6 +190,7 @@ class DevicePlan:
         pd.flux = flux_id
         pd.limiter = nat.LIMITERS[limiter]
         pd.reconstruction = nat.RECON_PRIMITIVE if primitive else nat.RECON_CONSERVATIVE
+        pd.quadrature_points = self.quadrature_points
         pd.nstages = S
         pd.butcher_a = self._a.ctypes.data_as(C.POINTER(C.c_double))
         pd.gamma = g
@@ -201,6 +208,8 @@ class DevicePlan:
         self.U = torch.zeros(self.state_len, dtype=torch.float64, device="cuda")
         self.nsteps_done = 0
         self.launch = {"rows_per_item": rows_per_item or 32, "grid_waves": grid_waves, "waves_per_simd": 2}
+        if self.quadrature_points > 1:   # the plan runs the split stage (8 rows per update item)
+            self.launch.update({"variant": 1, "rows_b": 8})
 
     # ------------------------------------------------------------------ lifetime
     def close(self):

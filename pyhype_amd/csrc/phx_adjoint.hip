@@ -129,8 +129,10 @@ int upload(const std::vector<T>& h, T** d) {
 
 const int OPP[4] = {PHX_W, PHX_E, PHX_S, PHX_N};
 
-// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual.
+// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual and the
+// 1-point Gauss rule (the element operations transpose one midpoint flux per face).
 int adj_setup(phx_plan* P) {
+    if (P->ph.nq > 1) return fail(PHX_E_INVALID, "adjoint: only the 1-point Gauss rule is differentiated");
     if (P->adj_ready) return PHX_OK;
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];

@@ -24,6 +24,7 @@ struct DevBlock {
     long long off;
     long long yoff;  // offset (doubles) of the block's (ny+1) x nx x 4 y-face fluxes in the split stage
     const double* geom;
+    const double* qoff;  // Gauss-point offsets (8*nq arrays of ny x nx) when nq > 1, else null
     double wall[8];
 };
 
@@ -168,6 +169,25 @@ __host__ __device__ inline Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R,
     const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
     const Q4 F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
     return phx::unrotate(F, c, s);
+}
+
+// Gauss-point offsets of cell (k, j): plane ((F*nq + i)*2 + c) of B.qoff for face F, point i,
+// component c (layout in include/pyhype_hip.h); points i >= nq are zero.
+__host__ __device__ inline phx::QGeo cell_qgeo(const DevBlock& B, int k, int j, int nq) {
+    phx::QGeo q;
+    const long nc = (long)B.nx * B.ny, c = (long)k * B.nx + j;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int i = 0; i < phx::QMAX; ++i) {
+            q.dx[f][i] = 0.0;
+            q.dy[f][i] = 0.0;
+            if (i < nq) {
+                q.dx[f][i] = B.qoff[(long)((f * nq + i) * 2) * nc + c];
+                q.dy[f][i] = B.qoff[(long)((f * nq + i) * 2 + 1) * nc + c];
+            }
+        }
+    return q;
 }
 
 template <bool CART>

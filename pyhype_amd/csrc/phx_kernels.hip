@@ -402,6 +402,115 @@ __global__ void __launch_bounds__(WGW * WAVE, MINW) k_stage_wg(StageArgs A) {
 constexpr int STRIP_A = WAVE - 2 * HALO;
 constexpr int STRIP_B = WAVE;      // k_yface, k_update: no column halo
 
+// ---- nq-point Gauss faces (split stage only; GEN instantiation, nq = A.ph.nq in 2..QMAX) ----
+// Face F's sum S_F = sum_i w_i*F_i starting from 0 (integrate_flux, fvm/base.py:168-190); per cell
+// the split buffers hold p1 = LW*S_W - LE*S_E, the nq N states then the nq S states
+// (8*nq doubles at 2*nq*off + 8*nq*cell) and per y-face S_F.
+__device__ __forceinline__ void acc_w(Q4& acc, double w, const Q4& F) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc.v[c] = acc.v[c] + w * F.v[c];
+}
+
+template <bool CART, int FLUX>
+__device__ void recon_x_item_q(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int nx = B.nx;
+    const int j = it.j0 - HALO + threadIdx.x % WAVE;
+    const bool valid = (j >= -1) && (j <= nx);
+    const bool inner = (j >= 0) && (j < nx);
+    const bool gcol = (j == -1) || (j == nx);
+    const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP_A);
+    const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP_A, nx));
+    const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP_A);
+    const Phys ph = A.ph;
+    const int nq = ph.nq;
+
+    Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
+    if (inner) Um = fetch_r<CART, true>(A.blocks, B, A.Uin, it.r0 - 1, j, ph);
+    if (valid) Uc = fetch_r<CART, true>(A.blocks, B, A.Uin, it.r0, j, ph);
+    for (int k = it.r0; k < it.r1; ++k) {
+        if (inner) Up = fetch_r<CART, true>(A.blocks, B, A.Uin, k + 1, j, ph);
+        else if (gcol && k + 1 < it.r1) Up = fetch_r<CART, true>(A.blocks, B, A.Uin, k + 1, j, ph);
+        const Q4 Uw = shfl_up1(Uc);
+        const Q4 Ue = shfl_down1(Uc);
+        phx::ReconQ r;
+        r.u = r.gx = r.gy = r.phi = phx::q4zero();
+        phx::QGeo qg;
+        for (int f = 0; f < 4; ++f)
+            for (int i = 0; i < phx::QMAX; ++i) qg.dx[f][i] = qg.dy[f][i] = 0.0;
+        if (recon) {
+            const phx::CellGeo g = cell_geo<CART>(B, k, j);
+            qg = cell_qgeo(B, k, j, nq);
+            r = phx::reconstruct_q(Uc, Uw, Ue, Um, Up, g, qg, nq, ph.limiter);
+        }
+        Q4 SW = phx::q4zero();  // sum over the W face's points of w_i * F_i
+#pragma unroll
+        for (int i = 0; i < phx::QMAX; ++i) {
+            if (i >= nq) break;
+            const Q4 qsend = (j == -1) ? Uc : phx::qstate(r, qg, PHX_E, i);
+            const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 at point i
+            if (xface) {
+                const Q4 Wi = phx::qstate(r, qg, PHX_W, i);
+                const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, Wi, QEl, k, j) : QEl;
+                const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : Wi;
+                acc_w(SW, phx::gauss_w(nq, i), xflux<CART, FLUX, true>(B, L, R, k, j, ph));
+            }
+        }
+        const Q4 SE = shfl_down1(SW);
+        if (owner) {
+            double LW, LE, LS, LN, Ac;
+            cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+            Q4 part;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) part.v[c] = LW * SW.v[c] - LE * SE.v[c];
+            const long cell = (long)k * nx + j;
+            st4(A.p1 + B.off + 4 * cell, part);
+            double* sn = A.sn + 2 * nq * B.off + 8 * nq * cell;
+#pragma unroll
+            for (int i = 0; i < phx::QMAX; ++i) {
+                if (i >= nq) break;
+                st4(sn + 4 * i, phx::qstate(r, qg, PHX_N, i));
+                st4(sn + 4 * (nq + i), phx::qstate(r, qg, PHX_S, i));
+            }
+        }
+        Um = Uc;
+        Uc = Up;
+    }
+}
+
+template <bool CART, int FLUX>
+__device__ void yface_item_q(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int nx = B.nx, ny = B.ny;
+    const int j = it.j0 + (int)(threadIdx.x % WAVE);
+    if (j >= nx) return;
+    const Phys ph = A.ph;
+    const int nq = ph.nq;
+    const double* sn = A.sn + 2 * nq * B.off;
+    for (int f = it.r0; f < it.r1; ++f) {
+        // N states of row f-1 (f > 0) and S states of row f (f < ny)
+        const long below = f > 0 ? 8 * nq * ((long)(f - 1) * nx + j) : 0;
+        const long above = f < ny ? 8 * nq * ((long)f * nx + j) + 4 * nq : 0;
+        Q4 gS = phx::q4zero(), gN = phx::q4zero();
+        if (f == 0) gS = fetch_r<CART, true>(A.blocks, B, A.Uin, -1, j, ph);
+        if (f == ny) gN = fetch_r<CART, true>(A.blocks, B, A.Uin, ny, j, ph);
+        Q4 S = phx::q4zero();
+#pragma unroll
+        for (int i = 0; i < phx::QMAX; ++i) {
+            if (i >= nq) break;
+            Q4 L, R;
+            if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + above + 4 * i), gS, 0, j);
+            else L = ld4(sn + below + 4 * i);
+            if (f == ny) R = outer_state<CART>(B, PHX_N, L, gN, ny - 1, j);
+            else R = ld4(sn + above + 4 * i);
+            acc_w(S, phx::gauss_w(nq, i), yflux<CART, FLUX, true>(B, L, R, f, j, ph));
+        }
+        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), S);
+    }
+}
+
 template <bool CART, int FLUX, bool GEN>
 __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
@@ -476,7 +585,7 @@ __device__ void yface_item(const StageArgs& A, const WorkItem it) {
     }
 }
 
-// residual ((x-part + LS*(2 F_S)) - LN*(2 F_N))/A of rows [r0, r1) of a 64-column strip, then every
+// residual ((x-part + LS*(2 F_S)) - LN*(2 F_N))/A (Gauss sums S_F in place of 2 F with nq > 1) of rows [r0, r1) of a 64-column strip, then every
 // Butcher output, the CFL quantity and realizability (as the fused kernel).
 template <bool CART>
 __device__ void update_item(const StageArgs& A, const WorkItem it) {
@@ -497,8 +606,14 @@ __device__ void update_item(const StageArgs& A, const WorkItem it) {
             double LW, LE, LS, LN, Ac;
             cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
             Q4 Res;
+            if (A.ph.nq > 1) {  // y-face entries are the Gauss sums already
 #pragma unroll
-            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+                for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * ((p.v[c] + LS * FS.v[c]) - LN * FN.v[c])) / Ac;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+            }
             emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
             FS = FN;
         }
@@ -518,6 +633,22 @@ __global__ void __launch_bounds__(WAVE) k_yface(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) yface_item<CART, FLUX, GEN>(A, A.items[i]);
+}
+
+// Gauss faces (nq > 1): kernels of their own, so that the register budget of the 1-point kernels is
+// not set by the nq-point path
+template <bool CART, int FLUX>
+__global__ void __launch_bounds__(WAVE, 2) k_recon_x_q(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) recon_x_item_q<CART, FLUX>(A, A.items[i]);
+}
+
+template <bool CART, int FLUX>
+__global__ void __launch_bounds__(WAVE) k_yface_q(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) yface_item_q<CART, FLUX>(A, A.items[i]);
 }
 
 template <bool CART>
@@ -789,6 +920,18 @@ struct k_yface_tag {
         hipLaunchKernelGGL((k_yface<C, F, G>), g, dim3(WAVE), 0, st, A);
     }
 };
+struct k_recon_x_q_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_recon_x_q<C, F>), g, dim3(WAVE), 0, st, A);
+    }
+};
+struct k_yface_q_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yface_q<C, F>), g, dim3(WAVE), 0, st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -805,8 +948,8 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
     A.nitems = nitems;
     const dim3 g(grid);
     const bool roe = P->flux == PHX_FLUX_ROE;
-    // generic instantiation: another limiter or primitive-variable reconstruction
-    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN || P->ph.prim;
+    // generic instantiation: another limiter, primitive-variable reconstruction or Gauss faces
+    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN || P->ph.prim || P->ph.nq > 1;
 #define PHX_LAUNCH_M(C, F)                                                                       \
     do {                                                                                         \
         if (gen) T::template launch<C, F, 2, true>(g, st, A);                                    \
@@ -828,15 +971,26 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
     if (P->variant == 1) {
         A.sn = P->sn;
         A.p1 = P->p1;
+        const bool gauss = P->ph.nq > 1;
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty())
-                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                           P->waves_per_simd, st);
+            if (!P->items[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                                 P->grid[kind], 2, st);
+                else
+                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                               P->grid[kind], P->waves_per_simd, st);
+            }
         A.yf = P->yf;
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_f[kind].empty())
-                launch_kind<k_yface_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(), P->grid_f[kind],
-                                         2, st);
+            if (!P->items_f[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_yface_q_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                               P->grid_f[kind], 2, st);
+                else
+                    launch_kind<k_yface_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                             P->grid_f[kind], 2, st);
+            }
         for (int kind = 0; kind < 2; ++kind)
             if (!P->items_b[kind].empty())
                 launch_kind<k_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
@@ -955,6 +1109,9 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         return fail(PHX_E_INVALID, "MUSCL: Slope limiter type not specified.");
     if (D->reconstruction != PHX_RECON_CONSERVATIVE && D->reconstruction != PHX_RECON_PRIMITIVE)
         return fail(PHX_E_INVALID, "unknown reconstruction type");
+    if (D->quadrature_points < 1 || D->quadrature_points > PHX_MAX_QUADRATURE_POINTS)
+        return fail(PHX_E_INVALID, "quadrature_points must be 1, 2 or 3");
+    const int nq = D->quadrature_points;
     if (D->nstages < 1 || D->nstages > PHX_MAX_STAGES || !D->butcher_a)
         return fail(PHX_E_INVALID, "invalid Butcher table");
     if (!D->t_final) return fail(PHX_E_INVALID, "t_final is null");
@@ -970,6 +1127,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         if (d.geom_kind == PHX_GEOM_CARTESIAN) glen = 8 + 11LL * d.nx + 11LL * d.ny + (int64_t)d.nx * d.ny;
         else if (d.geom_kind == PHX_GEOM_GENERAL) glen = 7LL * d.ny * (d.nx + 1) + 7LL * (d.ny + 1) * d.nx + 5LL * d.ny * d.nx;
         else return fail(PHX_E_INVALID, where + "unknown geometry kind");
+        if (nq > 1) glen += 8LL * nq * d.nx * d.ny;  // Gauss-point offsets
         if (d.geom_len != glen) return fail(PHX_E_INVALID, where + "geometry length mismatch");
         for (int s = 0; s < 4; ++s) {
             const int st = d.side_type[s];
@@ -1009,7 +1167,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     P->ncases = D->ncases;
     P->flux = D->flux;
     P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl, D->limiter,
-                 D->reconstruction == PHX_RECON_PRIMITIVE ? 1 : 0};
+                 D->reconstruction == PHX_RECON_PRIMITIVE ? 1 : 0, nq};
     P->desc.assign(D->blocks, D->blocks + D->nblocks);
     P->state_len = total;
     P->rows_per_item = D->rows_per_item > 0 ? D->rows_per_item : 32;
@@ -1038,6 +1196,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         h.yoff = P->yf_len;
         P->yf_len += 4LL * (d.ny + 1) * d.nx;
         h.geom = P->dgeom + goff[b];
+        h.qoff = nq > 1 ? h.geom + (d.geom_len - 8LL * nq * d.nx * d.ny) : nullptr;
         if (d.geom_kind == PHX_GEOM_CARTESIAN)
             for (int q = 0; q < 8; ++q) h.wall[q] = d.geom[q];
         ghost_off[b] = gacc;
@@ -1054,6 +1213,11 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     if (rc) return bail(rc);
     rc = build_items(P);
     if (rc) return bail(rc);
+    if (nq > 1) {  // Gauss faces run the split stage
+        P->rows_b = 8;
+        rc = phx_set_variant(P, 1, 0);
+        if (rc) return bail(rc);
+    }
 
     const size_t sbytes = (size_t)std::max<int64_t>(total, 4) * sizeof(double);
     PHX_CKP(hipMalloc(&P->sbuf[0], sbytes));
@@ -1229,10 +1393,12 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
     if (!P || variant < 0 || variant > 2 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (P->ph.nq > 1 && variant != 1)
+        return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stage only (variant 1)");
     PHX_CK(hipDeviceSynchronize());
     if (variant == 1 && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
-        PHX_CK(hipMalloc(&P->sn, 2 * L * sizeof(double)));
+        PHX_CK(hipMalloc(&P->sn, 2 * (size_t)std::max(P->ph.nq, 1) * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
         PHX_CK(hipMalloc(&P->yf, (size_t)std::max<long long>(P->yf_len, 4) * sizeof(double)));
     }

@@ -22,6 +22,7 @@ struct Phys {
     double gamma, gm1, one_over_gm1, g_over_gm1, cfl;
     int limiter;  // PHX_LIMITER_* (0 = Venkatakrishnan)
     int prim;     // reconstruction in primitive variables (SolverConfig.reconstruction_type)
+    int nq;       // Gauss points per face (SolverConfig.fvm_num_quadrature_points); 0 or 1: midpoint
 };
 
 struct Prim {
@@ -330,6 +331,80 @@ __host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4
         reconstruct_var<GEN>(U.v[k], Uw.v[k], Ue.v[k], Us.v[k], Un.v[k], g, lim, r.E.v[k], r.W.v[k], r.N.v[k],
                              r.S.v[k]);
     return r;
+}
+
+// ---------------------------------------------------------------------------------------------
+// nq-point Gauss faces (mesh/quadratures.py:33-95, fvm/base.py:168-190)
+// ---------------------------------------------------------------------------------------------
+constexpr int QMAX = 3;
+
+// Offsets of the Gauss points of one cell from its centroid, faces E, W, N, S, points in rule order.
+struct QGeo {
+    double dx[4][QMAX], dy[4][QMAX];
+};
+
+// Reconstruction of one cell with nq points per face: the cell value (reconstruction variables),
+// the Green-Gauss gradient and the limiter value; the limited value at a point is u + phi*hot.
+struct ReconQ {
+    Q4 u, gx, gy, phi;
+};
+
+// Weight of point i of the nq-point rule (_QUAD_2 = {-1/sqrt(3): 1, 1/sqrt(3): 1},
+// _QUAD_3 = {-sqrt(3/5): 5/9, 0: 8/9, sqrt(3/5): 5/9}).
+__host__ __device__ inline double gauss_w(int nq, int i) {
+    if (nq == 3) return i == 1 ? 8.0 / 9.0 : 5.0 / 9.0;
+    return nq == 2 ? 1.0 : 2.0;
+}
+
+// One variable of reconstruct_var with nq points per face: interface averages, Green-Gauss, and
+// phi = max(0, min over E points, W points, N points, S points of the limiter of each point's slope)
+// (limiters/base.py:47-187: np.minimum.reduce over the faces' point lists in that order).
+__host__ __device__ inline void reconstruct_q_var(double u, double uw, double ue, double us, double un,
+                                                  const CellGeo& g, const QGeo& q, int nq, int lim, double& gx,
+                                                  double& gy, double& phi) {
+    const double e = 0.5 * (u + ue);
+    const double w = 0.5 * (uw + u);
+    const double n = 0.5 * (u + un);
+    const double s = 0.5 * (us + u);
+    gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+    gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+    const double mx = fmax(fmax(fmax(fmax(u, uw), ue), us), un);
+    const double mn = fmin(fmin(fmin(fmin(u, uw), ue), us), un);
+    const double dmax = mx - u;
+    const double dmin = mn - u;
+    double p = 0.0;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int i = 0; i < QMAX; ++i) {
+            if (i >= nq) continue;
+            const double h = gx * q.dx[f][i] + gy * q.dy[f][i];
+            const double pf = limit_face<true>(dmax, dmin, (u + h) - u, lim);
+            p = (f == 0 && i == 0) ? pf : np_minimum(p, pf);
+        }
+    phi = (p < 0.0) ? 0.0 : p;
+}
+
+__host__ __device__ inline ReconQ reconstruct_q(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us, const Q4& Un,
+                                                const CellGeo& g, const QGeo& q, int nq, int lim) {
+    ReconQ r;
+    r.u = U;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        reconstruct_q_var(U.v[k], Uw.v[k], Ue.v[k], Us.v[k], Un.v[k], g, q, nq, lim, r.gx.v[k], r.gy.v[k], r.phi.v[k]);
+    return r;
+}
+
+// Limited value at point i of face f: U + phi*(gx*dxq + gy*dyq)
+// (SecondOrderMUSCL.limited_solution_at_quadrature_point, fvm/SecondOrderMUSCL.py:106-126).
+__host__ __device__ inline Q4 qstate(const ReconQ& r, const QGeo& q, int f, int i) {
+    Q4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double h = r.gx.v[k] * q.dx[f][i] + r.gy.v[k] * q.dy[f][i];
+        o.v[k] = r.u.v[k] + r.phi.v[k] * h;
+    }
+    return o;
 }
 
 // Ideal-gas CFL quantity min(dx/(|u|+a), dy/(|v|+a)) of one cell (blocks/quad_block.py:433-436;

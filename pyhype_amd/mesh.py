@@ -23,6 +23,14 @@ import numpy as np
 
 SIDES = ("E", "W", "N", "S")
 
+# Gauss rules on [-1, 1] as (point, weight) pairs in the reference's order, with the reference's
+# expressions (mesh/quadratures.py:33-37).
+QUADS = {
+    1: ((0, 2),),
+    2: ((-1 / np.sqrt(3), 1), (1 / np.sqrt(3), 1)),
+    3: ((-np.sqrt(3 / 5), 5 / 9), (0, 8 / 9), (np.sqrt(3 / 5), 5 / 9)),
+}
+
 
 class MeshGenerator:
     def __init__(self):
@@ -294,6 +302,23 @@ class BlockGeometry:
                 raise ValueError("block faces are not consistently shared between neighbouring cells")
         C = np.stack([self.A, self.xc, self.yc, self.dx, self.dy])
         return np.concatenate([V.ravel(), H.ravel(), C.ravel()]).astype(np.float64)
+
+    def pack_quadrature(self, nq: int) -> np.ndarray:
+        """Gauss-point offsets from the centroid (include/pyhype_hip.h, after the packed layout):
+        for face F in E, W, N, S and point i of the nq-point rule, xq - xc then yq - yc per cell,
+        with QuadraturePointData._transform 0.5*((p2 - p1)*p + (p2 + p1)) of the face's vertex pair
+        (p1, p2) = E (NE, SE), W (NW, SW), N (NE, NW), S (SE, SW) (mesh/quadratures.py:56-95)."""
+        X, Y = self.node_x, self.node_y
+        ne, nw, se, sw = (X[1:, 1:], Y[1:, 1:]), (X[1:, :-1], Y[1:, :-1]), (X[:-1, 1:], Y[:-1, 1:]), (X[:-1, :-1], Y[:-1, :-1])
+        ends = {"E": (ne, se), "W": (nw, sw), "N": (ne, nw), "S": (se, sw)}
+        out = []
+        for d in SIDES:
+            p1, p2 = ends[d]
+            for p, _ in QUADS[nq]:
+                qx = 0.5 * ((p2[0] - p1[0]) * p + (p2[0] + p1[0]))
+                qy = 0.5 * ((p2[1] - p1[1]) * p + (p2[1] + p1[1]))
+                out += [(qx - self.xc).ravel(), (qy - self.yc).ravel()]
+        return np.concatenate(out).astype(np.float64)
 
     def wall_cos_sin(self, d: str):
         sn, cs = libm_sincos(self.boundary_theta(d))

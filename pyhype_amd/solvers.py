@@ -58,8 +58,9 @@ def validate_config(config) -> List[List[float]]:
         raise ValueError(f"No MUSCL finite volume method has been specialized with order {config.fvm_spatial_order}")
     if config.nghost != 1:
         raise ValueError("Number of ghost cells must be equal to 1 for this method.")
-    if config.fvm_num_quadrature_points != 1:
-        raise ValueError("only the 1-point Gauss rule is implemented")
+    if config.fvm_num_quadrature_points not in (1, 2, 3):
+        # the _QUADS lookup of QuadraturePointData (mesh/quadratures.py:33-37,71)
+        raise KeyError(config.fvm_num_quadrature_points)
     if config.fvm_gradient_type != "GreenGauss":
         raise ValueError(f"Gradient type {config.fvm_gradient_type} is not available.")
     if config.fvm_flux_function_type not in ("Roe", "HLLL"):
@@ -95,7 +96,7 @@ class Euler2DBatch:
         for c in configs[1:]:
             validate_config(c)
             for attr in ("fvm_flux_function_type", "fvm_slope_limiter_type", "reconstruction_type", "time_integrator",
-                         "CFL"):
+                         "CFL", "fvm_num_quadrature_points"):
                 if getattr(c, attr) != getattr(c0, attr):
                     raise ValueError(f"batched cases must share {attr}")
             if type(c.fluid) is not type(c0.fluid):
@@ -106,7 +107,8 @@ class Euler2DBatch:
         self.plan = DevicePlan(specs, flux=c0.fvm_flux_function_type, table=self.table, fluid=c0.fluid,
                                cfl=c0.CFL, rows_per_item=rows_per_item, grid_waves=grid_waves,
                                force_general=force_general, limiter=c0.fvm_slope_limiter_type,
-                               primitive=c0.reconstruction_type is PrimitiveState)
+                               primitive=c0.reconstruction_type is PrimitiveState,
+                               quadrature_points=c0.fvm_num_quadrature_points)
         self.tuned = False
         if rows_per_item == 0:
             from . import autotune

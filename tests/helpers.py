@@ -15,7 +15,7 @@ def oracle_for(config, mesh, states0):
                        t_final=config.t_final, a_inf=config.fluid.far_field.a,
                        limiter=config.fvm_slope_limiter_type,
                        reconstruction="Primitive" if config.reconstruction_type.__name__ == "PrimitiveState"
-                       else "Conservative")
+                       else "Conservative", quadrature_points=config.fvm_num_quadrature_points)
     sim.set_state({n: np.array(a) for n, a in states0.items()})
     return sim
 

@@ -346,3 +346,69 @@ def test_primitive_reconstruction_matches_oracle(case, variant):
     s.run_steps(5)
     sim.run(5)
     assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+@pytest.mark.parametrize("nq", [2, 3])
+@pytest.mark.parametrize("case,rows,rows_b,waves", [
+    ("single", 2, 8, 3), ("multiblock", 1, 4, 3), ("oblique", 3, 5, 4), ("skewed", 2, 3, 3),
+    ("wide_multiblock", 2, 16, 4)])
+def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq):
+    """fvm_num_quadrature_points = 2 and 3 (mesh/quadratures.py:33-95, fvm/base.py:168-190): one
+    flux per face and Gauss point, limiter over every point; the plan runs the split stage.
+    Residual and steps bit for bit, Cartesian and rotated-frame blocks, walls, outlets and
+    neighbours, interior strip edges."""
+    if case == "single":
+        cfg, mesh = W.explosion_single(22)
+    elif case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(12, 2, 3)
+    elif case == "oblique":
+        cfg, mesh = W.oblique_shock(16, 3)
+    elif case == "wide_multiblock":   # two column strips per block
+        cfg, mesh = W.explosion_multiblock(66, 2, 1)
+    else:
+        cfg, mesh = W.explosion_single(16)[0], skewed_mesh()
+    cfg.fvm_num_quadrature_points = nq
+    s = _solver(cfg, mesh, rows_per_item=rows)
+    assert s.plan.launch_info()["variant"] == 1
+    s.plan.set_launch(rows, 0, waves, variant=1, rows_b=rows_b)
+    st = _perturbed(s.states()[0], seed=21)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(4)
+    sim.run(4)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+def test_gauss_faces_with_primitive_reconstruction_and_van_leer():
+    from pyhype_amd import PrimitiveState
+
+    cfg, mesh = W.explosion_multiblock(12, 2, 2)
+    cfg.fvm_num_quadrature_points = 2
+    cfg.fvm_slope_limiter_type = "VanLeer"
+    cfg.reconstruction_type = PrimitiveState
+    s = _solver(cfg, mesh, rows_per_item=2)
+    st = _perturbed(s.states()[0], seed=23)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(4)
+    sim.run(4)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+def test_gauss_faces_refuse_fused_variants_and_adjoint():
+    cfg, mesh = W.explosion_single(12)
+    cfg.fvm_num_quadrature_points = 3
+    s = _solver(cfg, mesh, rows_per_item=2)
+    for v in (0, 2):
+        with pytest.raises(ValueError, match="Gauss point"):
+            s.plan.set_launch(2, 0, 2, variant=v)
+    lam = s.plan.torch.zeros_like(s.plan.U)
+    with pytest.raises(ValueError, match="1-point Gauss"):
+        s.plan.residual_vjp(s.plan.U, lam)
+    s.run_steps(2)   # the plan still runs after the refusals

@@ -325,3 +325,152 @@ def test_ghost_fill_neighbours_and_walls():
     np.testing.assert_array_equal(b0.ghost["W"], st[0][:, :1])     # outlet: own column
     g = b1.ghost["E"]
     assert np.array_equal(g[..., 1], -st[1][:, -1:, 1]) and np.array_equal(g[..., 3], st[1][:, -1:, 3])
+
+
+# ------------------------------------------------------------------ Gauss faces (2 and 3 points)
+@pytest.mark.parametrize("nq", [1, 2, 3])
+def test_gauss_rules_integrate_polynomials(nq):
+    """An nq-point Gauss-Legendre rule integrates x^k over [-1, 1] exactly for k <= 2nq - 1 and
+    not for k = 2nq; its weights sum to 2."""
+    pts = [(float(p), float(w)) for p, w in O.QUADS[nq]]
+    for k in range(2 * nq + 1):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        quad = sum(w * p ** k for p, w in pts)
+        if k <= 2 * nq - 1:
+            assert abs(quad - exact) < 1e-15
+        else:
+            assert abs(quad - exact) > 1e-3
+    assert [p for p, _ in pts] == sorted(p for p, _ in pts)
+
+
+def test_gauss_points_lie_on_faces_in_rule_order():
+    """Points of a face lie on the segment between its two vertices, at the mapped Gauss abscissae,
+    the first point nearest the first vertex (E: NE, W: NW, N: NE, S: SE); the points of a face
+    shared by two cells coincide (E of cell j with W of cell j+1, N of row k with S of row k+1)."""
+    vert = {"SW": [0.0, 0.0], "SE": [4.0, 1.0], "NW": [1.0, 3.0], "NE": [5.0, 4.0]}
+    g = O.block_geometry(vert, 6, 5, 3)
+    X, Y = g.xn, g.yn
+    first = {"E": (X[1:, 1:], Y[1:, 1:]), "W": (X[1:, :-1], Y[1:, :-1]), "N": (X[1:, 1:], Y[1:, 1:]),
+             "S": (X[:-1, 1:], Y[:-1, 1:])}
+    second = {"E": (X[:-1, 1:], Y[:-1, 1:]), "W": (X[:-1, :-1], Y[:-1, :-1]), "N": (X[1:, :-1], Y[1:, :-1]),
+              "S": (X[:-1, :-1], Y[:-1, :-1])}
+    for d in O.DIRS:
+        (ax, ay), (bx, by) = first[d], second[d]
+        for (qx, qy, _), t in zip(g.qpts[d], (-math.sqrt(0.6), 0.0, math.sqrt(0.6))):
+            s = 0.5 * (t + 1.0)     # fraction of the way from the first to the second vertex
+            np.testing.assert_allclose(qx, ax + s * (bx - ax), atol=1e-14)
+            np.testing.assert_allclose(qy, ay + s * (by - ay), atol=1e-14)
+    for (ex, ey, _), (wx, wy, _) in zip(g.qpts["E"], g.qpts["W"]):
+        assert np.array_equal(ex[:, :-1], wx[:, 1:]) and np.array_equal(ey[:, :-1], wy[:, 1:])
+    for (nx_, ny_, _), (sx, sy, _) in zip(g.qpts["N"], g.qpts["S"]):
+        assert np.array_equal(nx_[:-1], sx[1:]) and np.array_equal(ny_[:-1], sy[1:])
+
+
+def _linear_field(x, y):
+    U = np.empty(x.shape + (4,))
+    U[..., 0] = 1.0 + 0.3 * x + 0.2 * y
+    U[..., 1] = 0.4 + 0.25 * x - 0.1 * y
+    U[..., 2] = -0.2 + 0.15 * x + 0.3 * y
+    U[..., 3] = 2.5 + 0.5 * x + 0.35 * y
+    return U
+
+
+def _exact_residual(g, m=10):
+    """-(1/A) times the boundary integral of F(U).n of the linear field, m-point Gauss-Legendre
+    per edge (numpy's leggauss): the divergence the scheme approximates."""
+    p, w = np.polynomial.legendre.leggauss(m)
+    X, Y = g.xn, g.yn
+    ne, nw, se, sw = (X[1:, 1:], Y[1:, 1:]), (X[1:, :-1], Y[1:, :-1]), (X[:-1, 1:], Y[:-1, 1:]), (X[:-1, :-1], Y[:-1, :-1])
+    tot = 0.0
+    for a, b in ((sw, se), (se, ne), (ne, nw), (nw, sw)):      # counter-clockwise, normal (dy, -dx)
+        dx, dy = b[0] - a[0], b[1] - a[1]
+        for pi, wi in zip(p, w):
+            t = 0.5 * (pi + 1.0)
+            U = _linear_field(a[0] + t * dx, a[1] + t * dy)
+            r = U[..., 0]
+            u, v = U[..., 1] / r, U[..., 2] / r
+            pr = 0.4 * (U[..., 3] - 0.5 * r * (u * u + v * v))
+            Fx = np.stack([r * u, r * u * u + pr, r * u * v, u * (U[..., 3] + pr)], -1)
+            Fy = np.stack([r * v, r * u * v, r * v * v + pr, v * (U[..., 3] + pr)], -1)
+            tot = tot + 0.5 * wi * (Fx * dy[..., None] - Fy * dx[..., None])
+    return -tot / g.A[..., None]
+
+
+def _gauss_error(n, nq, vert):
+    d = {"nBLK": 0, **vert}
+    for s in O.DIRS:
+        d["Neighbor" + s] = None
+        d["BCType" + s] = "OutletDirichlet"
+    sim = O.Simulation({0: d}, n, n, limiter="BarthJespersen", quadrature_points=nq)
+    b = sim.blocks[0]
+    sim.set_state({0: _linear_field(b.geom.xc, b.geom.yc)})
+    err = np.abs(O.residual(b, O.flux_roe) - _exact_residual(b.geom))
+    return err[2:-2, 2:-2].max()
+
+
+@pytest.mark.parametrize("vert", [{"SW": [0, 0], "SE": [1, 0], "NW": [0, 1], "NE": [1, 1]},
+                                  {"SW": [0, 0], "SE": [1, 0.2], "NW": [0.3, 1], "NE": [1.3, 1.2]}],
+                         ids=["rectangle", "parallelogram"])
+def test_gauss_faces_converge_at_their_order(vert):
+    """On a linear field the reconstruction is exact away from the boundary (Green-Gauss is exact on
+    these meshes and BarthJespersen is inactive), the Roe flux of equal states is the physical flux,
+    so the residual error is the face quadrature's alone: order 2 for the midpoint rule, 4 for
+    2 points, round-off for 3 points at these sizes.  A weight permuted or a point placed on the
+    wrong face fails these bounds."""
+    e1 = [_gauss_error(n, 1, vert) for n in (8, 16)]
+    e2 = [_gauss_error(n, 2, vert) for n in (8, 16)]
+    e3 = _gauss_error(8, 3, vert)
+    assert math.log2(e1[0] / e1[1]) > 1.4
+    assert math.log2(e2[0] / e2[1]) > 3.2
+    assert e2[0] < 1e-3 * e1[0]
+    assert e3 < 1e-11
+
+
+@pytest.mark.parametrize("nq", [2, 3])
+def test_gauss_limiter_bounds_every_point(nq):
+    """BarthJespersen with phi = min over all Gauss points of all faces keeps the limited value at
+    every point inside the 5-point stencil's [min, max] (a limiter that looked at one point per face
+    would let the others overshoot on rough data)."""
+    rng = np.random.default_rng(7)
+    vert = {"SW": [0.0, 0.0], "SE": [4.0, 1.0], "NW": [1.0, 3.0], "NE": [5.0, 4.0]}
+    g = O.block_geometry(vert, 9, 7, nq)
+    U = 1.0 + rng.random((7, 9, 4))
+    b = O.Block(0, g, {s: None for s in O.DIRS}, {s: "OutletDirichlet" for s in O.DIRS}, U=U)
+    O.apply_boundary_condition({0: b})
+    gx, gy = O.green_gauss(U, b.ghost, g)
+    hot = {d: [O.high_order_term(gx, gy, g, d, i) for i in range(nq)] for d in O.DIRS}
+    phi = O.limiter_phi(U, b.ghost, hot, O.barth_jespersen)
+    EW = np.concatenate((b.ghost["W"], U, b.ghost["E"]), axis=1)
+    NS = np.concatenate((b.ghost["S"], U, b.ghost["N"]), axis=0)
+    vals = (U, EW[:, :-2], EW[:, 2:], NS[:-2], NS[2:])
+    hi, lo = np.maximum.reduce(vals), np.minimum.reduce(vals)
+    limited_somewhere = False
+    for d in O.DIRS:
+        for h in hot[d]:
+            q = U + phi * h
+            assert np.all(q <= hi + 1e-13) and np.all(q >= lo - 1e-13)
+            limited_somewhere |= bool(np.any((U + h > hi + 1e-9) | (U + h < lo - 1e-9)))
+    assert limited_somewhere
+
+
+@pytest.mark.parametrize("nq", [2, 3])
+def test_gauss_face_states_pair_by_point(nq):
+    """On a linear field the two limited states that meet at a Gauss point of an interior face are
+    equal (both are the field's value there): point i of a cell's E face meets point i of its east
+    neighbour's W face, likewise N with S.  Pairing point i with another point would differ by
+    O(h * gradient)."""
+    vert = {"SW": [0, 0], "SE": [1, 0.2], "NW": [0.3, 1], "NE": [1.3, 1.2]}
+    g = O.block_geometry(vert, 10, 10, nq)
+    U = _linear_field(g.xc, g.yc)
+    b = O.Block(0, g, {s: None for s in O.DIRS}, {s: "OutletDirichlet" for s in O.DIRS}, U=U)
+    O.apply_boundary_condition({0: b})
+    gx, gy = O.green_gauss(U, b.ghost, g)
+    hot = {d: [O.high_order_term(gx, gy, g, d, i) for i in range(nq)] for d in O.DIRS}
+    phi = O.limiter_phi(U, b.ghost, hot, O.barth_jespersen)
+    st = {d: [U + phi * h for h in hot[d]] for d in O.DIRS}
+    inner = np.s_[2:-2, 2:-3]
+    for i in range(nq):
+        assert np.max(np.abs(st["E"][i][:, :-1] - st["W"][i][:, 1:])[inner]) < 1e-13
+        assert np.max(np.abs(st["N"][i][:-1] - st["S"][i][1:])[2:-3, 2:-2]) < 1e-13
+        if nq == 2:
+            assert np.max(np.abs(st["E"][i][:, :-1] - st["W"][1 - i][:, 1:])[inner]) > 1e-3
