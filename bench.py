@@ -240,7 +240,8 @@ def main():
     torch.cuda.synchronize()
     stage_ms = e0.elapsed_time(e1) / (reps * S)
     b_cell_step = alg_bytes_per_cell_step(table)
-    bytes_per_launch = cells * b_cell_step / S
+    # the stage launches above skip the blocks of frozen cases: credit the live cells only
+    bytes_per_launch = live * b_cell_step / S
     achieved = bytes_per_launch / (stage_ms * 1e-3) / 1e9
     traffic = None
     tfile = PROFILES / f"traffic_{args.workload}.json"

@@ -44,15 +44,26 @@ __host__ __device__ inline void st4(double* p, const Q4& q) {
     v[1] = make_double2(q.v[2], q.v[3]);
 }
 
+// Element i of an array in device global memory.  The geometry pointers come from the block
+// descriptors in memory and carry no address space, so device code states it: the compiler then
+// emits global loads instead of flat ones (which also wait on the LDS/scalar counter).
+__host__ __device__ inline double gld(const double* p, long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(1))) double*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
 // ---------------------------------------------------------------------------------------------
 // geometry accessors (layouts documented in include/pyhype_hip.h)
 // ---------------------------------------------------------------------------------------------
 struct CartG {
     const double* g;
     int nx, ny;
-    __host__ __device__ double col(int q, int j) const { return g[8 + q * nx + j]; }
-    __host__ __device__ double row(int q, int k) const { return g[8 + 11 * nx + q * ny + k]; }
-    __host__ __device__ double A(int k, int j) const { return g[8 + 11 * nx + 11 * ny + (long)k * nx + j]; }
+    __host__ __device__ double col(int q, int j) const { return gld(g, 8 + q * nx + j); }
+    __host__ __device__ double row(int q, int k) const { return gld(g, 8 + 11 * nx + q * ny + k); }
+    __host__ __device__ double A(int k, int j) const { return gld(g, 8 + 11 * nx + 11 * ny + (long)k * nx + j); }
 };
 enum { C_xlN, C_xlS, C_ylN, C_ylS, C_LN, C_LS, C_dxE, C_dxW, C_dxN, C_dxS, C_dx };
 enum { R_xlE, R_xlW, R_ylE, R_ylW, R_LE, R_LW, R_dyE, R_dyW, R_dyN, R_dyS, R_dy };
@@ -62,10 +73,10 @@ struct GenG {
     int nx, ny;
     __host__ __device__ long nV() const { return (long)ny * (nx + 1); }
     __host__ __device__ long nH() const { return (long)(ny + 1) * nx; }
-    __host__ __device__ double V(int q, int k, int jf) const { return g[q * nV() + (long)k * (nx + 1) + jf]; }
-    __host__ __device__ double H(int q, int kf, int j) const { return g[7 * nV() + q * nH() + (long)kf * nx + j]; }
+    __host__ __device__ double V(int q, int k, int jf) const { return gld(g, q * nV() + (long)k * (nx + 1) + jf); }
+    __host__ __device__ double H(int q, int kf, int j) const { return gld(g, 7 * nV() + q * nH() + (long)kf * nx + j); }
     __host__ __device__ double C(int q, int k, int j) const {
-        return g[7 * nV() + 7 * nH() + q * (long)ny * nx + (long)k * nx + j];
+        return gld(g, 7 * nV() + 7 * nH() + q * (long)ny * nx + (long)k * nx + j);
     }
 };
 enum { F_L, F_xl, F_yl, F_xq, F_yq, F_c, F_s };
@@ -183,8 +194,8 @@ __host__ __device__ inline phx::QGeo cell_qgeo(const DevBlock& B, int k, int j, 
             q.dx[f][i] = 0.0;
             q.dy[f][i] = 0.0;
             if (i < nq) {
-                q.dx[f][i] = B.qoff[(long)((f * nq + i) * 2) * nc + c];
-                q.dy[f][i] = B.qoff[(long)((f * nq + i) * 2 + 1) * nc + c];
+                q.dx[f][i] = gld(B.qoff, (long)((f * nq + i) * 2) * nc + c);
+                q.dy[f][i] = gld(B.qoff, (long)((f * nq + i) * 2 + 1) * nc + c);
             }
         }
     return q;

@@ -80,7 +80,9 @@ __device__ __forceinline__ void emit_row(const StageArgs& A, const DevBlock& B, 
                                          double& lane_dt, bool& lane_bad) {
     const long idx = B.off + 4 * ((long)kk * B.nx + j);
     if (A.Rout) st4(A.Rout + idx, Res);
-#pragma unroll
+    // not unrolled: the outputs' pointers and coefficients are read from the kernel arguments per
+    // output instead of occupying scalar registers for all MAX_OUT slots (which spills them)
+#pragma unroll 1
     for (int m = 0; m < MAX_OUT; ++m) {
         if (m >= A.nout) break;
         const Q4 s = ld4(A.src[m] + idx);
