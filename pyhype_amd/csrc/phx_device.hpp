@@ -55,6 +55,18 @@ __host__ __device__ inline double gld(const double* p, long i) {
 #endif
 }
 
+// Element i of a descriptor table that no kernel writes while it runs (block descriptors, work
+// lists, per-case dt).  Device code reads it through the constant address space, so wave-uniform
+// reads become scalar loads instead of vector loads that wait behind the state prefetch.
+template <class T>
+__host__ __device__ inline T ld_const(const T* p, long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+    return p[i];
+#endif
+}
+
 // ---------------------------------------------------------------------------------------------
 // geometry accessors (layouts documented in include/pyhype_hip.h)
 // ---------------------------------------------------------------------------------------------
@@ -98,6 +110,30 @@ __host__ __device__ inline void wall_cs(const DevBlock& B, int side, int k, int 
     }
 }
 
+// Ghost value beyond side SIDE of block B at (k, j): neighbour cell, or own boundary cell
+// (reflected for walls).  SIDE is a compile-time constant so that the descriptor fields read here
+// (side kind, neighbour block) are wave-uniform scalar loads.
+template <bool CART, int SIDE>
+__host__ __device__ inline Q4 ghost_value(const DevBlock* __restrict__ blocks, const DevBlock& B,
+                                          const double* __restrict__ U, int k, int j) {
+    const int st = B.side[SIDE];
+    if (st == PHX_SIDE_NEIGHBOR) {
+        const DevBlock N = ld_const(blocks, B.nbr[SIDE]);
+        const int nk = SIDE == PHX_N ? 0 : (SIDE == PHX_S ? N.ny - 1 : k);
+        const int nj = SIDE == PHX_E ? 0 : (SIDE == PHX_W ? N.nx - 1 : j);
+        return ld4(U + N.off + 4 * ((long)nk * N.nx + nj));
+    }
+    const int kk = SIDE == PHX_N ? B.ny - 1 : (SIDE == PHX_S ? 0 : k);
+    const int jj = SIDE == PHX_E ? B.nx - 1 : (SIDE == PHX_W ? 0 : j);
+    Q4 q = ld4(U + B.off + 4 * ((long)kk * B.nx + jj));
+    if (st == PHX_SIDE_WALL) {
+        double c, s;
+        wall_cs<CART>(B, SIDE, k, j, c, s);
+        q = phx::reflect(q, c, s);
+    }
+    return q;
+}
+
 // Value of cell (k, j) of block B, where (k, j) may lie one cell outside the block on one side
 // (never a corner): Blocks.apply_boundary_condition / GhostBlock.send_boundary_data
 // (blocks/base.py:448-471, blocks/ghost.py:187-216) followed by the ghost's BC (ghost.py:243-278).
@@ -106,28 +142,10 @@ __host__ __device__ inline Q4 fetch(const DevBlock* __restrict__ blocks, const D
                     int k, int j) {
     const int nx = B.nx, ny = B.ny;
     if (k >= 0 && k < ny && j >= 0 && j < nx) return ld4(U + B.off + 4 * ((long)k * nx + j));
-    int side, kk = k, jj = j;
-    if (j < 0) { side = PHX_W; jj = 0; }
-    else if (j >= nx) { side = PHX_E; jj = nx - 1; }
-    else if (k < 0) { side = PHX_S; kk = 0; }
-    else { side = PHX_N; kk = ny - 1; }
-    const int st = B.side[side];
-    if (st == PHX_SIDE_NEIGHBOR) {
-        const DevBlock& N = blocks[B.nbr[side]];
-        int nk = k, nj = j;
-        if (side == PHX_E) nj = 0;
-        else if (side == PHX_W) nj = N.nx - 1;
-        else if (side == PHX_N) nk = 0;
-        else nk = N.ny - 1;
-        return ld4(U + N.off + 4 * ((long)nk * N.nx + nj));
-    }
-    Q4 q = ld4(U + B.off + 4 * ((long)kk * nx + jj));
-    if (st == PHX_SIDE_WALL) {
-        double c, s;
-        wall_cs<CART>(B, side, k, j, c, s);
-        q = phx::reflect(q, c, s);
-    }
-    return q;
+    if (j < 0) return ghost_value<CART, PHX_W>(blocks, B, U, k, j);
+    if (j >= nx) return ghost_value<CART, PHX_E>(blocks, B, U, k, j);
+    if (k < 0) return ghost_value<CART, PHX_S>(blocks, B, U, k, j);
+    return ghost_value<CART, PHX_N>(blocks, B, U, k, j);
 }
 
 // fetch() in the reconstruction variables: with GEN and PrimitiveState reconstruction the value

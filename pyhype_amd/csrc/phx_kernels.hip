@@ -80,28 +80,38 @@ __device__ __forceinline__ void emit_row(const StageArgs& A, const DevBlock& B, 
                                          double& lane_dt, bool& lane_bad) {
     const long idx = B.off + 4 * ((long)kk * B.nx + j);
     if (A.Rout) st4(A.Rout + idx, Res);
-    // not unrolled: the outputs' pointers and coefficients are read from the kernel arguments per
-    // output instead of occupying scalar registers for all MAX_OUT slots (which spills them)
+    // Outputs in chunks of EMIT_CHUNK: the chunk's sources are loaded together before its stores (an
+    // output's source is never a buffer an earlier output of the stage writes), and the chunk loop
+    // is not unrolled, so the outputs' pointers and coefficients are read from the kernel arguments
+    // per chunk instead of occupying scalar registers for all MAX_OUT slots.
+    constexpr int EMIT_CHUNK = 3;
 #pragma unroll 1
-    for (int m = 0; m < MAX_OUT; ++m) {
-        if (m >= A.nout) break;
-        const Q4 s = ld4(A.src[m] + idx);
-        Q4 o;
-        const double am = A.a[m];
-        if (am == 0.0) {
-            o = s;
-        } else {
-            const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
+    for (int m0 = 0; m0 < A.nout; m0 += EMIT_CHUNK) {
+        Q4 s[EMIT_CHUNK];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) o.v[c] = s.v[c] + cdt * Res.v[c];
-        }
-        st4(A.dst[m] + idx, o);
-        if (m == A.state_out && A.dtmin != nullptr) {
-            double dx, dy;
-            cell_dxdy<CART>(B, kk, j, dx, dy);
-            const double t = phx::cell_dt(o, dx, dy, A.ph);
-            if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
-            else lane_dt = fmin(lane_dt, t);
+        for (int q = 0; q < EMIT_CHUNK; ++q)
+            if (m0 + q < A.nout) s[q] = ld4(A.src[m0 + q] + idx);
+#pragma unroll
+        for (int q = 0; q < EMIT_CHUNK; ++q) {
+            const int m = m0 + q;
+            if (m >= A.nout) break;
+            Q4 o;
+            const double am = A.a[m];
+            if (am == 0.0) {
+                o = s[q];
+            } else {
+                const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) o.v[c] = s[q].v[c] + cdt * Res.v[c];
+            }
+            st4(A.dst[m] + idx, o);
+            if (m == A.state_out && A.dtmin != nullptr) {
+                double dx, dy;
+                cell_dxdy<CART>(B, kk, j, dx, dy);
+                const double t = phx::cell_dt(o, dx, dy, A.ph);
+                if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
+                else lane_dt = fmin(lane_dt, t);
+            }
         }
     }
 }
@@ -139,7 +149,7 @@ __device__ void stage_item(const StageArgs& A, const WorkItem it) {
     const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP);
     const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP, nx));
     const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP);
-    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
     const Phys ph = A.ph;
 
     const int kr0 = max(it.r0 - 1, 0);
@@ -232,7 +242,7 @@ template <bool CART, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX, GEN>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -264,7 +274,7 @@ __device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, 
     const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP);
     const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP, nx));
     const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP);
-    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
     const Phys ph = A.ph;
 
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
@@ -386,7 +396,7 @@ __global__ void __launch_bounds__(WGW * WAVE, MINW) k_stage_wg(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) {
-        stage_wg_item<CART, FLUX, GEN>(A, A.items[i], lds_N, lds_F);
+        stage_wg_item<CART, FLUX, GEN>(A, ld_const(A.items, i), lds_N, lds_F);
         __syncthreads();  // the next item reuses the exchange buffers
     }
 }
@@ -596,7 +606,7 @@ __device__ void update_item(const StageArgs& A, const WorkItem it) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int nx = B.nx;
     const int j = it.j0 + lane;
-    const double dt = (A.dt != nullptr) ? A.dt[B.case_id] : 0.0;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
     bool lane_bad = false;
     if (j < nx) {
@@ -627,14 +637,14 @@ template <bool CART, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
 }
 
 template <bool CART, int FLUX, bool GEN>
 __global__ void __launch_bounds__(WAVE) k_yface(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) yface_item<CART, FLUX, GEN>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) yface_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
 }
 
 // Gauss faces (nq > 1): kernels of their own, so that the register budget of the 1-point kernels is
@@ -643,21 +653,21 @@ template <bool CART, int FLUX>
 __global__ void __launch_bounds__(WAVE, 2) k_recon_x_q(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) recon_x_item_q<CART, FLUX>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) recon_x_item_q<CART, FLUX>(A, ld_const(A.items, i));
 }
 
 template <bool CART, int FLUX>
 __global__ void __launch_bounds__(WAVE) k_yface_q(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) yface_item_q<CART, FLUX>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) yface_item_q<CART, FLUX>(A, ld_const(A.items, i));
 }
 
 template <bool CART>
 __global__ void __launch_bounds__(WAVE) k_update(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) update_item<CART>(A, A.items[i]);
+    for (int i = first; i < hi; i += stride) update_item<CART>(A, ld_const(A.items, i));
 }
 
 // standalone CFL reduction over the owned cells of every work item
@@ -679,7 +689,7 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
     item_range(A.nitems, lo, hi, first, stride);
     const int lane = threadIdx.x & (WAVE - 1);
     for (int i = first; i < hi; i += stride) {
-        const WorkItem it = A.items[i];
+        const WorkItem it = ld_const(A.items, i);
         const DevBlock& B = A.blocks[it.block];
         const int j = it.j0 + lane;
         const bool own = (lane < A.strip) && (j < B.nx);
