@@ -693,7 +693,26 @@ BUTCHER = {
     "RK3SSP": [[1], [1 / 4, 1 / 4], [1 / 6, 1 / 6, 2 / 3]],
     "Ralston3": [[1 / 2], [0, 3 / 4], [2 / 9, 1 / 3, 4 / 9]],
     "RK4": [[0.5], [0, 0.5], [0, 0, 1], [1 / 6, 1 / 3, 1 / 3, 1 / 6]],
+    "Ralston4": [[0.4], [0.29697761, 0.15875964], [0.21810040, -3.05096516, 3.83286476],
+                 [0.17476028, -0.55148066, 1.20553560, 0.17118478]],
+    "DormandPrince5": [[1 / 5], [3 / 40, 9 / 40], [44 / 45, -56 / 15, 32 / 9],
+                       [19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729],
+                       [9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656],
+                       [35 / 384, 0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84]],
 }
+
+
+def generic2(a):
+    """ExplicitRungeKutta.Generic2 (time_marching/explicit_runge_kutta.py:103-115)."""
+    return [[a], [1 - 1 / (2 * a), 1 / (2 * a)]]
+
+
+def generic3(a):
+    """ExplicitRungeKutta.Generic3 (time_marching/explicit_runge_kutta.py:143-168)."""
+    if a in (0, 2 / 3, 1):
+        raise ValueError("Value of alpha parameter is not allowd.")
+    k = (1 - a) / a / (3 * a - 2)
+    return [[a], [1 + k, -k], [0.5 - 1 / (6 * a), 1 / (6 * a * (1 - a)), (2 - 3 * a) / (6 * (1 - a))]]
 
 
 def block_dt(b: Block, CFL: float) -> float:

@@ -412,3 +412,29 @@ def test_gauss_faces_refuse_fused_variants_and_adjoint():
     with pytest.raises(ValueError, match="1-point Gauss"):
         s.plan.residual_vjp(s.plan.U, lam)
     s.run_steps(2)   # the plan still runs after the refusals
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("integrator", ["ExplicitEuler1", "RK2", "Ralston2", "RK3", "RK3SSP", "Ralston3",
+                                        "Ralston4", "DormandPrince5", "Generic3"])
+def test_every_butcher_table_matches_oracle(integrator, variant):
+    """Every table of TimeIntegratorFactory (time_marching/explicit_runge_kutta.py:91-274) in
+    accumulator form on every stage variant: up to 6 outputs per stage (Dormand-Prince), negative
+    coefficients (RK3), zero entries skipped; 3 steps bit for bit."""
+    from pyhype_amd.time_marching import butcher_table
+
+    cfg, mesh = W.explosion_single(16)
+    cfg.time_integrator = integrator
+    cfg.CFL = 0.4
+    if integrator == "Generic3":
+        cfg.alpha = 0.4
+    s = _solver(cfg, mesh, rows_per_item=2)
+    if variant:
+        s.plan.set_launch(2, 0, 3 if variant == 1 else 2, variant=variant, rows_b=3)
+    st = _perturbed(s.states()[0], seed=31)
+    s.set_state([st])
+    sim = oracle_for(cfg, mesh, st)
+    assert sim.a == butcher_table(integrator, cfg)
+    s.run_steps(3)
+    sim.run(3)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())

@@ -185,3 +185,20 @@ def test_config_validation_errors():
     for nq in (2, 3):
         cfg.fvm_num_quadrature_points = nq
         validate_config(cfg)
+
+
+def test_backend_butcher_tables_equal_the_oracle_tables():
+    """Every table the backend accepts is the oracle's table, entry for entry (same doubles)."""
+    from types import SimpleNamespace
+
+    from pyhype_amd.time_marching import TABLES, butcher_table
+
+    for name in TABLES:
+        if name.startswith("Generic"):
+            for alpha in (0.3, 0.4, 1.0) if name == "Generic2" else (0.3, 0.4, 0.8):
+                got = butcher_table(name, SimpleNamespace(alpha=alpha))
+                ref = (O.generic2 if name == "Generic2" else O.generic3)(alpha)
+                assert got == ref and all(type(a) is type(b) for r, q in zip(got, ref) for a, b in zip(r, q))
+        else:
+            assert butcher_table(name) == O.BUTCHER[name], name
+    assert butcher_table("SSPRK2") == O.generic2(1)

@@ -287,21 +287,39 @@ def test_residual_mirror_equivariance_on_smooth_state():
     assert np.max(np.abs(Rt - T(R)) / sc) < 1e-10
 
 
-@pytest.mark.parametrize("name,poly", [
-    ("RK4", lambda z: 1 + z + z ** 2 / 2 + z ** 3 / 6 + z ** 4 / 24),
-    ("SSPRK2", lambda z: 1 + z + z ** 2 / 2),
-    ("RK2", lambda z: 1 + z + z ** 2 / 2),
-    ("RK3SSP", lambda z: 1 + z + z ** 2 / 2 + z ** 3 / 6),
+def _taylor(n, extra=()):
+    return lambda z: sum(z ** k / math.factorial(k) for k in range(n + 1)) + sum(c * z ** k for k, c in extra)
+
+
+@pytest.mark.parametrize("name,poly,rtol", [
+    ("ExplicitEuler1", _taylor(1), 1e-14),
+    ("RK2", _taylor(2), 1e-14),
+    ("SSPRK2", _taylor(2), 1e-14),
+    ("Ralston2", _taylor(2), 1e-14),
+    ("RK3", _taylor(3), 1e-14),
+    ("RK3SSP", _taylor(3), 1e-14),
+    ("Ralston3", _taylor(3), 1e-14),
+    ("RK4", _taylor(4), 1e-14),
+    ("Ralston4", _taylor(4), 1e-7),          # the reference's 8-digit coefficients
+    # the 6-stage 5th-order Dormand-Prince weights: R(z) = exp(z) to z^5, then z^6/600
+    ("DormandPrince5", _taylor(5, [(6, 1 / 600)]), 1e-14),
+    ("Generic2(0.3)", _taylor(2), 1e-14),
+    ("Generic3(0.4)", _taylor(3), 1e-14),
 ])
-def test_rk_stability_polynomial(monkeypatch, name, poly):
-    lam = -0.37
+def test_rk_stability_polynomial(monkeypatch, name, poly, rtol):
+    """A linear residual R(U) = lam*U makes one step multiply U by the method's stability polynomial:
+    exp(z) up to the method's order (plus the known z^6 term of Dormand-Prince), z = lam*dt."""
+    if name.startswith("Generic"):
+        table = (O.generic2 if name[7] == "2" else O.generic3)(float(name[9:-1]))
+    else:
+        table = O.BUTCHER[name]
     monkeypatch.setattr(O, "residual", lambda b, f: lam * b.U)
     monkeypatch.setattr(O, "apply_boundary_condition", lambda blocks: None)
-    g = O.block_geometry(RECT, 3, 2)
-    b = O.Block(0, g, {}, {}, U=np.full((2, 3, 4), 2.0))
-    dt = 0.9
-    O.integrate({0: b}, dt, O.BUTCHER[name], None)
-    np.testing.assert_allclose(b.U, 2.0 * poly(lam * dt), rtol=1e-14)
+    for lam, dt in ((-0.37, 0.9), (-1.3, 0.7), (0.6, 0.5)):
+        g = O.block_geometry(RECT, 3, 2)
+        b = O.Block(0, g, {}, {}, U=np.full((2, 3, 4), 2.0))
+        O.integrate({0: b}, dt, table, None)
+        np.testing.assert_allclose(b.U, 2.0 * poly(lam * dt), rtol=rtol)
 
 
 def test_dt_closed_form_on_uniform_state():
