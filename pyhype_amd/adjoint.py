@@ -6,7 +6,8 @@ that dt does not depend on the state being differentiated), U_0 -> U_n.  The rev
 binomial-free uniform checkpointing: the forward pass keeps U every ``checkpoint_every`` steps; the
 reverse pass re-runs each segment from its checkpoint while recording the stage inputs
 (``phx_run_record``), then applies the step VJP of every step of the segment in reverse order
-(``phx_step_vjp``).  All arithmetic is in the HIP library; torch holds the buffers and chains the
+(``phx_step_vjp``); the re-run of the next segment overlaps the reverse sweep of the current one
+on a second stream.  All arithmetic is in the HIP library; torch holds the buffers and chains the
 result into user objectives.
 
     traj = Trajectory(plan, dt_sched, checkpoint_every=10)
@@ -45,6 +46,8 @@ class Trajectory:
         torch = plan.torch
         m = min(self.every, max(1, self.nsteps))
         self.stages = torch.empty((m, plan.nstages, plan.state_len), dtype=torch.float64, device="cuda")
+        self._stages_b = None   # second record buffer, allocated by the first backward with > 1 segment
+        self._side = None       # stream of the segment recomputation (default priority)
 
     def forward(self, U0):
         """U_n from U_0 (device tensor of the state layout); keeps the checkpoints."""
@@ -58,18 +61,62 @@ class Trajectory:
         return P.U.clone()
 
     def backward(self, lam_T):
-        """Adjoint of U_0 from the adjoint of U_n (the checkpoints of the last forward)."""
+        """Adjoint of U_0 from the adjoint of U_n (the checkpoints of the last forward).
+
+        The recomputation of segment k-1 (``run_record`` from its checkpoint, on a second stream)
+        overlaps the reverse sweep of segment k (``step_vjp``, on the caller's stream): the two
+        use disjoint device buffers (the plan's state and scratch versus its adjoint work
+        arrays), and the records alternate between two buffers, each reused only after the sweep
+        that read it has finished (events)."""
         if len(self.checkpoints) != (self.nsteps + self.every - 1) // self.every:
             raise RuntimeError("Trajectory.backward needs a preceding forward")
         P = self.plan
-        lam = lam_T.detach().to(dtype=P.torch.float64).contiguous().clone()
-        for seg in reversed(range(len(self.checkpoints))):
+        torch = P.torch
+        lam = lam_T.detach().to(dtype=torch.float64).contiguous().clone()
+        segs = list(reversed(range(len(self.checkpoints))))
+
+        def span(seg):
             s0 = seg * self.every
-            m = min(self.every, self.nsteps - s0)
-            P.U.copy_(self.checkpoints[seg])
+            return s0, min(self.every, self.nsteps - s0)
+
+        if len(segs) == 1:
+            s0, m = span(segs[0])
+            P.U.copy_(self.checkpoints[segs[0]])
             P.run_record(self.dt[s0:s0 + m], self.stages[:m])
             for i in reversed(range(m)):
                 P.step_vjp(self.stages[i], self.dt[s0 + i], lam)
+            return lam
+
+        if self._stages_b is None:
+            self._stages_b = torch.empty_like(self.stages)
+        if self._side is None:
+            self._side = torch.cuda.Stream()
+        main, side = torch.cuda.current_stream(), self._side
+        bufs = (self.stages, self._stages_b)
+        ready = (torch.cuda.Event(), torch.cuda.Event())
+        free = (torch.cuda.Event(), torch.cuda.Event())
+
+        def recompute(seg, slot):
+            s0, m = span(seg)
+            with torch.cuda.stream(side):
+                P.U.copy_(self.checkpoints[seg])
+                P.run_record(self.dt[s0:s0 + m], bufs[slot][:m])
+                ready[slot].record(side)
+
+        side.wait_stream(main)          # checkpoints, schedule and state written on the caller's stream
+        recompute(segs[0], 0)
+        for k, seg in enumerate(segs):
+            slot = k % 2
+            if k + 1 < len(segs):
+                if k >= 1:
+                    side.wait_event(free[1 - slot])   # the sweep of segment k-1 read that buffer
+                recompute(segs[k + 1], 1 - slot)
+            main.wait_event(ready[slot])
+            s0, m = span(seg)
+            for i in reversed(range(m)):
+                P.step_vjp(bufs[slot][i], self.dt[s0 + i], lam)
+            free[slot].record(main)
+        main.wait_stream(side)          # the plan's state is the caller's again
         return lam
 
 

@@ -438,3 +438,29 @@ def test_every_butcher_table_matches_oracle(integrator, variant):
     s.run_steps(3)
     sim.run(3)
     assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_batch_mixing_cartesian_and_rotated_frame_blocks(variant):
+    """One plan whose cases mix Cartesian blocks (one block), rotated-frame blocks with neighbours
+    (a 2x2 rectangular mesh, off-by-ulp interior corners) and a skewed two-block mesh: both kernel
+    kinds launch in every stage; each case keeps its own dt and matches its own oracle."""
+    c1, m1 = W.explosion_single(18)
+    c2, m2 = W.explosion_multiblock(12, 2, 2)
+    c3, m3 = W.explosion_single(14)[0], skewed_mesh()
+    cfgs, meshes = [c1, c2, c3], [m1, m2, m3]
+    b = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    kinds = {e.cartesian for e in b.plan.blocks}
+    assert kinds == {True, False}
+    if variant:
+        b.plan.set_launch(2, 0, 3 if variant == 1 else 2, variant=variant, rows_b=4)
+    b.apply_initial_condition()
+    st0 = b.states()
+    b.run_steps(5)
+    got = b.states()
+    assert not b.failed
+    for c in range(3):
+        sim = oracle_for(cfgs[c], meshes[c], st0[c])
+        sim.run(5)
+        assert bitwise_equal_blocks(got[c], sim.states()), f"case {c}"
+        assert b.t[c] == sim.t
