@@ -35,6 +35,7 @@ constexpr int WAVE = 64;
 constexpr int HALO = 2;
 constexpr int STRIP = WAVE - 2 * HALO;  // owned columns per wavefront
 constexpr unsigned long long DT_INF_BITS = 0x7FF0000000000000ull;  // +inf
+constexpr int GRAPH_STEPS = 8;  // steps captured in the multi-step graph of phx_run
 
 struct StageArgs {
     const DevBlock* blocks;
@@ -851,6 +852,8 @@ int upload_items(std::vector<WorkItem> (&items)[2], WorkItem* (&ditems)[2], int 
 // STRIP owned columns per item; the split variant uses STRIP_A for its reconstruction kernel,
 // STRIP_B with `rows` face rows (ny + 1 of them) for the y-face kernel and STRIP_B with rows_b rows
 // for the update kernel.
+void release_graphs(phx_plan* P);
+
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
@@ -884,10 +887,17 @@ int build_items(phx_plan* P) {
     if (rc) return rc;
     rc = upload_items(P->items_f, P->ditems_f, P->grid_f, P->grid_waves);
     if (rc) return rc;
-    if (P->gexec) { (void)hipGraphExecDestroy(P->gexec); P->gexec = nullptr; }
-    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
-    P->graph_U = nullptr;
+    release_graphs(P);
     return PHX_OK;
+}
+
+// Destroys the captured step graphs (they hold the launch list and the state pointer).
+void release_graphs(phx_plan* P) {
+    for (int g = 0; g < 2; ++g) {
+        if (P->gexec[g]) { (void)hipGraphExecDestroy(P->gexec[g]); P->gexec[g] = nullptr; }
+        if (P->graph[g]) { (void)hipGraphDestroy(P->graph[g]); P->graph[g] = nullptr; }
+    }
+    P->graph_U = nullptr;
 }
 
 StageArgs base_args(phx_plan* P) {
@@ -1093,8 +1103,7 @@ int phx_plan_destroy(phx_plan* P) {
     if (!P) return PHX_OK;
     (void)hipDeviceSynchronize();
     phx::host::adjoint_release(P);
-    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
-    if (P->graph) (void)hipGraphDestroy(P->graph);
+    release_graphs(P);
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
@@ -1287,18 +1296,22 @@ int phx_run(phx_plan* P, double* U, int32_t nsteps, int32_t use_graph, void* str
         PHX_CK(hipGetLastError());
         return PHX_OK;
     }
-    if (!P->gexec || P->graph_U != U) {
-        if (P->gexec) { PHX_CK(hipGraphExecDestroy(P->gexec)); P->gexec = nullptr; }
-        if (P->graph) { PHX_CK(hipGraphDestroy(P->graph)); P->graph = nullptr; }
+    if (P->graph_U != U) release_graphs(P);
+    // GRAPH_STEPS steps per replay while enough remain (one graph launch per GRAPH_STEPS steps),
+    // then single steps
+    const int ng = nsteps / GRAPH_STEPS, n1 = nsteps % GRAPH_STEPS;
+    for (int g = 0; g < 2; ++g) {  // both captured on first use, so that later calls only replay
+        if (P->gexec[g]) continue;
         PHX_CK(hipStreamBeginCapture(P->cap_stream, hipStreamCaptureModeThreadLocal));
-        launch_step(P, U, P->cap_stream, nullptr);
-        PHX_CK(hipStreamEndCapture(P->cap_stream, &P->graph));
-        PHX_CK(hipGraphInstantiate(&P->gexec, P->graph, nullptr, nullptr, 0));
-        P->graph_U = U;
+        for (int s = 0; s < (g == 0 ? 1 : GRAPH_STEPS); ++s) launch_step(P, U, P->cap_stream, nullptr);
+        PHX_CK(hipStreamEndCapture(P->cap_stream, &P->graph[g]));
+        PHX_CK(hipGraphInstantiate(&P->gexec[g], P->graph[g], nullptr, nullptr, 0));
     }
+    P->graph_U = U;
     PHX_CK(hipEventRecord(P->ev_a, st));
     PHX_CK(hipStreamWaitEvent(P->cap_stream, P->ev_a, 0));
-    for (int s = 0; s < nsteps; ++s) PHX_CK(hipGraphLaunch(P->gexec, P->cap_stream));
+    for (int s = 0; s < ng; ++s) PHX_CK(hipGraphLaunch(P->gexec[1], P->cap_stream));
+    for (int s = 0; s < n1; ++s) PHX_CK(hipGraphLaunch(P->gexec[0], P->cap_stream));
     PHX_CK(hipEventRecord(P->ev_b, P->cap_stream));
     PHX_CK(hipStreamWaitEvent(st, P->ev_b, 0));
     return PHX_OK;

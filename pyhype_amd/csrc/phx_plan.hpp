@@ -95,8 +95,9 @@ struct phx_plan {
     long long* dghost_off = nullptr;
     hipStream_t cap_stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured steps: [0] one step, [1] GRAPH_STEPS steps (replayed by phx_run)
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     double* graph_U = nullptr;
     bool copy_back = false;  // one-stage tables: result lands in sbuf[0] and is copied to U
 
