@@ -464,3 +464,25 @@ def test_batch_mixing_cartesian_and_rotated_frame_blocks(variant):
         sim.run(5)
         assert bitwise_equal_blocks(got[c], sim.states()), f"case {c}"
         assert b.t[c] == sim.t
+
+
+def test_multi_step_graph_with_failing_case_matches_oracles():
+    """19 graph steps (two replays of the 8-step graph, then three of the one-step graph) on uneven
+    cases, one of which fails at step 1 and is frozen: every other case matches its own oracle bit
+    for bit, time and the last dt included."""
+    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52], density_scaled=True)[0]
+    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28], density_scaled=True)[0]
+    meshes = [W.single_block_mesh() for _ in cfgs]
+    b = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b.apply_initial_condition()
+    st0 = b.states()
+    b.run_steps(19)
+    got = b.states()
+    assert b.failed == {2: 1}
+    _, step, t, dt = b.plan.status()
+    assert step == 19
+    for c in (0, 1):
+        sim = oracle_for(cfgs[c], meshes[c], st0[c])
+        sim.run(19)
+        assert bitwise_equal_blocks(got[c], sim.states()), f"case {c}"
+        assert t[c] == sim.t and dt[c] == sim.dts[-1]
