@@ -120,7 +120,7 @@ typedef struct {
     int32_t grid_waves;       /* waves of the persistent grid; 0 = one per work item */
     int32_t reconstruction;   /* PHX_RECON_* (SolverConfig.reconstruction_type) */
     int32_t quadrature_points; /* Gauss points per face, 1..PHX_MAX_QUADRATURE_POINTS; > 1 runs
-                                  the split stage (phx_set_variant 1) and no adjoint */
+                                  the split stage (phx_set_variant 1) */
 } phx_plan_desc;
 
 typedef struct phx_plan phx_plan;
@@ -183,9 +183,9 @@ int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host,
 
 /* Reverse-mode product of the residual: lamU = (dR/dU)^T lamR at state U (discrete adjoint of
  * fvm/base.py:108-190 with the active limiter/flux branches; deterministic, no atomics).  Requires
- * mutual neighbour links and the 1-point Gauss rule (PHX_E_INVALID otherwise).  The first adjoint
- * call allocates the plan's adjoint work arrays (about 60 doubles per cell).  No counterpart in
- * the reference. */
+ * mutual neighbour links (PHX_E_INVALID otherwise).  The first adjoint call allocates the plan's
+ * adjoint work arrays (about 20 + 4*(nstages-1) + 32*nq doubles per cell, nq Gauss points per
+ * face).  No counterpart in the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
 
 /* Reverse-mode product of one Runge-Kutta step with a frozen time step: on entry lam holds the

@@ -62,7 +62,9 @@ struct TermSeed {
     __device__ Q4 operator()(const DevBlock& B, long long cell) const { return eval_terms(t, B, cell); }
 };
 
-template <int FLUX>
+// NQ: 1 (midpoint faces) or phx::QMAX (nq = ph.nq Gauss points per face); the point records are
+// RB*nq doubles per cell and FA*nq per face
+template <int FLUX, int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                    const double* U, const double* rb, Terms kappa, double* fadj,
                                                    Phys ph) {
@@ -71,12 +73,13 @@ __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const
     const DevBlock& B = blocks[t.block];
     const AdjBlock O = offs[t.block];
     const long long f = (long long)t.start + threadIdx.x;
-    double* out = fadj + phx::adj::FA * (O.face + f);
+    double* out = fadj + (long long)phx::adj::FA * phx::adj::npts<NQ>(ph) * (O.face + f);
     const TermSeed seed{kappa};
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::face_adj<true, FLUX>(blocks, B, U, rb, O, seed, f, out, ph);
-    else phx::adj::face_adj<false, FLUX>(blocks, B, U, rb, O, seed, f, out, ph);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::face_adj<true, FLUX, TermSeed, NQ>(blocks, B, U, rb, O, seed, f, out, ph);
+    else phx::adj::face_adj<false, FLUX, TermSeed, NQ>(blocks, B, U, rb, O, seed, f, out, ph);
 }
 
+template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                     const double* U, double* rb, Phys ph) {
     const AdjTile t = tiles[blockIdx.x];
@@ -85,11 +88,12 @@ __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, cons
     const AdjBlock O = offs[t.block];
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
-    double* out = rb + phx::adj::RB * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true>(blocks, B, U, k, j, out, ph);
-    else phx::adj::recon_cell<false>(blocks, B, U, k, j, out, ph);
+    double* out = rb + (long long)phx::adj::RB * phx::adj::npts<NQ>(ph) * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell<true, NQ>(blocks, B, U, k, j, out, ph);
+    else phx::adj::recon_cell<false, NQ>(blocks, B, U, k, j, out, ph);
 }
 
+template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                    const double* U, const double* fadj, double* slot, Phys ph) {
     const AdjTile t = tiles[blockIdx.x];
@@ -99,11 +103,12 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
-    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true>(blocks, B, U, O, fadj, k, j, out, ph);
-    else phx::adj::cell_adj<false>(blocks, B, U, O, fadj, k, j, out, ph);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj<true, NQ>(blocks, B, U, O, fadj, k, j, out, ph);
+    else phx::adj::cell_adj<false, NQ>(blocks, B, U, O, fadj, k, j, out, ph);
 }
 
 // lamU = (dR/dU)^T kappa + extra terms (pointwise; extra may include lamU itself)
+template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                      const double* slot, const double* fadj, const double* U,
                                                      Phys ph, Terms extra, double* lamU) {
@@ -112,8 +117,9 @@ __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, con
     const DevBlock& B = blocks[t.block];
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
-    Q4 lam = (B.kind == PHX_GEOM_CARTESIAN) ? phx::adj::gather_adj<true>(blocks, offs, t.block, k, j, slot, fadj, U, ph)
-                                            : phx::adj::gather_adj<false>(blocks, offs, t.block, k, j, slot, fadj, U, ph);
+    Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
+                 ? phx::adj::gather_adj<true, NQ>(blocks, offs, t.block, k, j, slot, fadj, U, ph)
+                 : phx::adj::gather_adj<false, NQ>(blocks, offs, t.block, k, j, slot, fadj, U, ph);
     if (extra.n > 0) phx::adj::acc(lam, eval_terms(extra, B, B.off / 4 + (long long)k * B.nx + j));
     phx::adj::st(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
 }
@@ -129,10 +135,8 @@ int upload(const std::vector<T>& h, T** d) {
 
 const int OPP[4] = {PHX_W, PHX_E, PHX_S, PHX_N};
 
-// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual and the
-// 1-point Gauss rule (the element operations transpose one midpoint flux per face).
+// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual.
 int adj_setup(phx_plan* P) {
-    if (P->ph.nq > 1) return fail(PHX_E_INVALID, "adjoint: only the 1-point Gauss rule is differentiated");
     if (P->adj_ready) return PHX_OK;
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
@@ -168,29 +172,37 @@ int adj_setup(phx_plan* P) {
     if ((rc = upload(P->adj_cell_tiles, &P->dadj_cell_tiles))) return rc;
     if ((rc = upload(P->adj_face_tiles, &P->dadj_face_tiles))) return rc;
     const size_t L = (size_t)P->state_len;
-    PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * sizeof(double)));
-    PHX_CK(hipMalloc(&P->adj_face, (size_t)faces * phx::adj::FA * sizeof(double)));
+    const size_t nq = (size_t)std::max(P->ph.nq, 1);
+    PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * nq * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_face, (size_t)faces * phx::adj::FA * nq * sizeof(double)));
     PHX_CK(hipMalloc(&P->adj_slot, (size_t)P->adj_cells * phx::adj::SLOT * sizeof(double)));
     for (int s = 1; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->adj_lam[s], L * sizeof(double)));
     P->adj_ready = true;
     return PHX_OK;
 }
 
-void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
-                         hipStream_t st) {
+template <int NQ>
+void launch_residual_vjp_nq(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
+                            hipStream_t st) {
     const dim3 blk(TILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
-    hipLaunchKernelGGL(k_adj_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
+    hipLaunchKernelGGL(k_adj_recon<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
     if (P->flux == PHX_FLUX_ROE)
-        hipLaunchKernelGGL(k_adj_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
+        hipLaunchKernelGGL((k_adj_face<0, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
     else
-        hipLaunchKernelGGL(k_adj_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
+        hipLaunchKernelGGL((k_adj_face<1, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
-    hipLaunchKernelGGL(k_adj_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
+    hipLaunchKernelGGL(k_adj_cell<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
                        P->adj_slot, P->ph);
-    hipLaunchKernelGGL(k_adj_gather, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot, P->adj_face,
-                       U, P->ph, extra, lamU);
+    hipLaunchKernelGGL(k_adj_gather<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot,
+                       P->adj_face, U, P->ph, extra, lamU);
+}
+
+void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
+                         hipStream_t st) {
+    if (P->ph.nq > 1) launch_residual_vjp_nq<phx::QMAX>(P, U, kappa, extra, lamU, st);
+    else launch_residual_vjp_nq<1>(P, U, kappa, extra, lamU, st);
 }
 
 

@@ -477,6 +477,80 @@ __host__ __device__ inline void reconstruct_vjp_var(double u, double uw, double 
     out[4] = 0.5 * bn + bval[4];
 }
 
+// reconstruct_vjp_var with nq Gauss points per face (reconstruct_q_var): bq[f][i] = adjoint of the
+// limited value at point i of face f (E, W, N, S); phi is the minimum over the 4*nq points in the
+// order E points, W points, N points, S points, and the first minimum is the active one.
+__host__ __device__ inline void reconstruct_vjp_var_q(double u, double uw, double ue, double us, double un,
+                                                      const CellGeo& g, const QGeo& q, int nq,
+                                                      const double bq[4][QMAX], int lim, double out[5]) {
+    const double vals[5] = {u, uw, ue, us, un};
+    const double e = 0.5 * (u + ue), w = 0.5 * (uw + u);
+    const double n = 0.5 * (u + un), s = 0.5 * (us + u);
+    const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+    const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+    int imax = 0, imin = 0;
+    double mx = vals[0], mn = vals[0];
+    for (int t = 1; t < 5; ++t) {
+        if (vals[t] > mx) { mx = vals[t]; imax = t; }
+        if (vals[t] < mn) { mn = vals[t]; imin = t; }
+    }
+    const double dmax = mx - u, dmin = mn - u;
+    double hf[4][QMAX], sf[4][QMAX], df[4][QMAX];
+    int fa = 0, ia = 0;
+    double pa = 0.0;
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < QMAX; ++i) {
+            hf[f][i] = sf[f][i] = df[f][i] = 0.0;
+            if (i >= nq) continue;
+            hf[f][i] = gx * q.dx[f][i] + gy * q.dy[f][i];
+            df[f][i] = (u + hf[f][i]) - u;
+            sf[f][i] = face_slope(dmax, dmin, df[f][i]);
+            const double p = limiter_fn(lim, sf[f][i]);
+            if ((f == 0 && i == 0) || p < pa) { pa = p; fa = f; ia = i; }
+        }
+    const double phi = (pa < 0.0) ? 0.0 : pa;
+    // Q_fi = u + phi h_fi
+    double bu = 0.0, bphi = 0.0, bh[4][QMAX];
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < QMAX; ++i) {
+            bh[f][i] = 0.0;
+            if (i >= nq) continue;
+            bu += bq[f][i];
+            bphi += bq[f][i] * hf[f][i];
+            bh[f][i] = phi * bq[f][i];
+        }
+    double bval[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!(pa < 0.0)) {
+        const double dfs = df[fa][ia];
+        if (dfs > 0.0 || dfs < 0.0) {
+            const double sv = sf[fa][ia];
+            const double bsv = bphi * limiter_prime(lim, sv);
+            const double bnum = bsv / dfs;
+            const double bd = -bsv * sv / dfs;
+            if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }
+            else { bval[imin] += bnum; bu -= bnum; }
+            bh[fa][ia] += bd;
+        }
+    }
+    double bgx = 0.0, bgy = 0.0;
+    for (int f = 0; f < 4; ++f)
+        for (int i = 0; i < QMAX; ++i) {
+            if (i >= nq) continue;
+            bgx += bh[f][i] * q.dx[f][i];
+            bgy += bh[f][i] * q.dy[f][i];
+        }
+    const double be = (bgx * g.xlE + bgy * g.ylE) * g.ainv;
+    const double bw = (bgx * g.xlW + bgy * g.ylW) * g.ainv;
+    const double bn = (bgx * g.xlN + bgy * g.ylN) * g.ainv;
+    const double bs = (bgx * g.xlS + bgy * g.ylS) * g.ainv;
+    bu += 0.5 * (be + bw + bn + bs);
+    out[0] = bu + bval[0];
+    out[1] = 0.5 * bw + bval[1];
+    out[2] = 0.5 * be + bval[2];
+    out[3] = 0.5 * bs + bval[3];
+    out[4] = 0.5 * bn + bval[4];
+}
+
 __host__ __device__ inline ReconBar reconstruct_vjp(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
                                                     const Q4& Un, const CellGeo& g, const Recon& bQ, int lim = 0) {
     ReconBar out;

@@ -15,8 +15,10 @@ using phx::host::AdjBlock;
 
 namespace {
 
-// p: gamma, gm1, 1/gm1, gamma/gm1, CFL, limiter id, primitive reconstruction (0/1)
-Phys make_phys(const double* p) { return Phys{p[0], p[1], p[2], p[3], p[4], (int)p[5], (int)p[6]}; }
+// p: gamma, gm1, 1/gm1, gamma/gm1, CFL, limiter id, primitive reconstruction (0/1), Gauss points
+Phys make_phys(const double* p) {
+    return Phys{p[0], p[1], p[2], p[3], p[4], (int)p[5], (int)p[6], p[7] > 1.0 ? (int)p[7] : 1};
+}
 Q4 q4(const double* p) { Q4 q; for (int k = 0; k < 4; ++k) q.v[k] = p[k]; return q; }
 void put(double* p, const Q4& q) { for (int k = 0; k < 4; ++k) p[k] = q.v[k]; }
 
@@ -26,7 +28,7 @@ struct HostPlan {
     long long cells = 0, faces = 0, len = 0;
 };
 
-HostPlan make_plan(const phx_block_desc* d, int nb) {
+HostPlan make_plan(const phx_block_desc* d, int nb, int nq) {
     HostPlan P;
     P.blocks.resize(nb);
     P.offs.resize(nb);
@@ -37,6 +39,7 @@ HostPlan make_plan(const phx_block_desc* d, int nb) {
         for (int s = 0; s < 4; ++s) { h.side[s] = d[b].side_type[s]; h.nbr[s] = d[b].neighbor[s]; }
         h.off = d[b].state_offset;
         h.geom = d[b].geom;
+        h.qoff = nq > 1 ? d[b].geom + (d[b].geom_len - 8LL * nq * h.nx * h.ny) : nullptr;
         if (h.kind == PHX_GEOM_CARTESIAN)
             for (int q = 0; q < 8; ++q) h.wall[q] = d[b].geom[q];
         P.offs[b].cell = d[b].state_offset / 4;
@@ -48,23 +51,25 @@ HostPlan make_plan(const phx_block_desc* d, int nb) {
     return P;
 }
 
+template <int NQ>
 void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, const Phys& ph) {
-    rb.assign((size_t)P.cells * adj::RB, 0.0);
+    const int nq = adj::npts<NQ>(ph);
+    rb.assign((size_t)P.cells * adj::RB * nq, 0.0);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
-                double* out = rb.data() + adj::RB * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true>(P.blocks.data(), B, U, k, j, out, ph);
-                else adj::recon_cell<false>(P.blocks.data(), B, U, k, j, out, ph);
+                double* out = rb.data() + (long long)adj::RB * nq * adj::cell_index(P.offs[b], B, k, j);
+                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true, NQ>(P.blocks.data(), B, U, k, j, out, ph);
+                else adj::recon_cell<false, NQ>(P.blocks.data(), B, U, k, j, out, ph);
             }
     }
 }
 
-template <int FLUX>
+template <int FLUX, int NQ>
 void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
     std::vector<double> rb, flux((size_t)P.faces * 4);
-    recon_all(P, U, rb, ph);
+    recon_all<NQ>(P, U, rb, ph);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -76,26 +81,31 @@ void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
         }
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
-                const Q4 r = (B.kind == PHX_GEOM_CARTESIAN) ? adj::cell_residual<true>(B, P.offs[b], flux.data(), k, j)
-                                                            : adj::cell_residual<false>(B, P.offs[b], flux.data(), k, j);
+                const int nq = adj::npts<NQ>(ph);
+                const Q4 r = (B.kind == PHX_GEOM_CARTESIAN)
+                                 ? adj::cell_residual<true>(B, P.offs[b], flux.data(), k, j, nq)
+                                 : adj::cell_residual<false>(B, P.offs[b], flux.data(), k, j, nq);
                 put(R + B.off + 4LL * ((long long)k * B.nx + j), r);
             }
     }
 }
 
-template <int FLUX>
+template <int FLUX, int NQ>
 void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, double* lamU, const Phys& ph) {
-    std::vector<double> rb, fadj((size_t)P.faces * adj::FA), slot((size_t)P.cells * adj::SLOT, 0.0);
-    recon_all(P, U, rb, ph);
+    const int nq = adj::npts<NQ>(ph);
+    std::vector<double> rb, fadj((size_t)P.faces * adj::FA * nq), slot((size_t)P.cells * adj::SLOT, 0.0);
+    recon_all<NQ>(P, U, rb, ph);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
         for (long long f = 0; f < nf; ++f) {
-            double* out = fadj.data() + adj::FA * (P.offs[b].face + f);
+            double* out = fadj.data() + (long long)adj::FA * nq * (P.offs[b].face + f);
             if (B.kind == PHX_GEOM_CARTESIAN)
-                adj::face_adj<true, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], adj::PtrSeed{kappa}, f, out, ph);
+                adj::face_adj<true, FLUX, adj::PtrSeed, NQ>(P.blocks.data(), B, U, rb.data(), P.offs[b],
+                                                            adj::PtrSeed{kappa}, f, out, ph);
             else
-                adj::face_adj<false, FLUX>(P.blocks.data(), B, U, rb.data(), P.offs[b], adj::PtrSeed{kappa}, f, out, ph);
+                adj::face_adj<false, FLUX, adj::PtrSeed, NQ>(P.blocks.data(), B, U, rb.data(), P.offs[b],
+                                                             adj::PtrSeed{kappa}, f, out, ph);
         }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
@@ -104,9 +114,9 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
             for (int j = 0; j < B.nx; ++j) {
                 double* out = slot.data() + adj::SLOT * adj::cell_index(P.offs[b], B, k, j);
                 if (B.kind == PHX_GEOM_CARTESIAN)
-                    adj::cell_adj<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
+                    adj::cell_adj<true, NQ>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
                 else
-                    adj::cell_adj<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
+                    adj::cell_adj<false, NQ>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
             }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
@@ -114,8 +124,10 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 const Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
-                                   ? adj::gather_adj<true>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data(), U, ph)
-                                   : adj::gather_adj<false>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(), fadj.data(), U, ph);
+                                   ? adj::gather_adj<true, NQ>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(),
+                                                               fadj.data(), U, ph)
+                                   : adj::gather_adj<false, NQ>(P.blocks.data(), P.offs.data(), (int)b, k, j, slot.data(),
+                                                                fadj.data(), U, ph);
                 put(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
             }
     }
@@ -127,18 +139,30 @@ extern "C" {
 
 // element-wise residual of every block (same layout as phx_residual)
 int shim_residual(const phx_block_desc* d, int nb, int flux, const double* phys, const double* U, double* R) {
-    const HostPlan P = make_plan(d, nb);
-    if (flux == 0) residual_t<0>(P, U, R, make_phys(phys));
-    else residual_t<1>(P, U, R, make_phys(phys));
+    const Phys ph = make_phys(phys);
+    const HostPlan P = make_plan(d, nb, ph.nq);
+    if (ph.nq > 1) {
+        if (flux == 0) residual_t<0, QMAX>(P, U, R, ph);
+        else residual_t<1, QMAX>(P, U, R, ph);
+    } else {
+        if (flux == 0) residual_t<0, 1>(P, U, R, ph);
+        else residual_t<1, 1>(P, U, R, ph);
+    }
     return 0;
 }
 
 // (dR/dU)^T kappa (same layout as phx_residual_vjp)
 int shim_residual_vjp(const phx_block_desc* d, int nb, int flux, const double* phys, const double* U,
                       const double* kappa, double* lamU) {
-    const HostPlan P = make_plan(d, nb);
-    if (flux == 0) residual_vjp_t<0>(P, U, kappa, lamU, make_phys(phys));
-    else residual_vjp_t<1>(P, U, kappa, lamU, make_phys(phys));
+    const Phys ph = make_phys(phys);
+    const HostPlan P = make_plan(d, nb, ph.nq);
+    if (ph.nq > 1) {
+        if (flux == 0) residual_vjp_t<0, QMAX>(P, U, kappa, lamU, ph);
+        else residual_vjp_t<1, QMAX>(P, U, kappa, lamU, ph);
+    } else {
+        if (flux == 0) residual_vjp_t<0, 1>(P, U, kappa, lamU, ph);
+        else residual_vjp_t<1, 1>(P, U, kappa, lamU, ph);
+    }
     return 0;
 }
 

@@ -32,7 +32,8 @@ LIMITER_IDS = {"Venkatakrishnan": 0, "BarthJespersen": 1, "VanAlbada": 2, "VanLe
 def _phys(cfg):
     f = cfg.fluid
     return np.array([f.gamma(), f.gm1(), f.one_over_gm1(), f.g_over_gm1(), cfg.CFL,
-                     LIMITER_IDS[cfg.fvm_slope_limiter_type], cfg.reconstruction_type is PrimitiveState])
+                     LIMITER_IDS[cfg.fvm_slope_limiter_type], cfg.reconstruction_type is PrimitiveState,
+                     cfg.fvm_num_quadrature_points])
 
 
 def _with_limiter(case, limiter):
@@ -44,6 +45,12 @@ def _with_limiter(case, limiter):
 def _primitive(case):
     (cfg, mesh), general = case
     cfg.reconstruction_type = PrimitiveState
+    return (cfg, mesh), general
+
+
+def _gauss(case, nq):
+    (cfg, mesh), general = case
+    cfg.fvm_num_quadrature_points = nq
     return (cfg, mesh), general
 
 
@@ -62,13 +69,19 @@ CASES = {
     "oblique_hlll_prim": lambda: _primitive((W.oblique_shock(8, 3), False)),
     "skewed_prim_vanalbada": lambda: _primitive(_with_limiter(((W.explosion_single(8)[0], skewed_mesh()), False),
                                                               "VanAlbada")),
+    "single_gauss2": lambda: _gauss((W.explosion_single(8), False), 2),
+    "multiblock_gauss3": lambda: _gauss((W.explosion_multiblock(5, 2, 2), False), 3),
+    "oblique_hlll_gauss2": lambda: _gauss((W.oblique_shock(6, 2), False), 2),
+    "skewed_prim_vanleer_gauss2": lambda: _gauss(_primitive(_with_limiter(((W.explosion_single(6)[0], skewed_mesh()),
+                                                                           False), "VanLeer")), 2),
 }
 
 
 def _setup(name):
     (cfg, mesh), general = CASES[name]()
     md = mesh_dict(mesh)
-    lay = layout_cases([CaseSpec(md, cfg.nx, cfg.ny)], force_general=general)
+    lay = layout_cases([CaseSpec(md, cfg.nx, cfg.ny)], force_general=general,
+                       quadrature_points=cfg.fvm_num_quadrature_points)
     flux = 0 if cfg.fvm_flux_function_type == "Roe" else 1
     return cfg, mesh, md, lay, flux
 
@@ -116,7 +129,8 @@ def shim(adjoint_shim):
 
 @pytest.mark.parametrize("name", ["single", "multiblock", "oblique_hlll", "skewed", "single_barth",
                                   "multiblock_vanalbada", "oblique_vanleer", "single_prim", "multiblock_prim",
-                                  "oblique_hlll_prim", "skewed_prim_vanalbada"])
+                                  "oblique_hlll_prim", "skewed_prim_vanalbada", "single_gauss2", "multiblock_gauss3",
+                                  "oblique_hlll_gauss2", "skewed_prim_vanleer_gauss2"])
 def test_elementwise_residual_equals_oracle(shim, name):
     """(Forcing the general path on a Cartesian mesh changes the arithmetic, as it would in the
     reference, so only the meshes' own geometry kinds are compared.)"""
@@ -146,7 +160,7 @@ def _cons(rng, n, mach):
 def test_face_flux_vjp_matches_central_differences(shim, flux, mach):
     rng = np.random.default_rng(10 + flux + int(10 * mach))
     n = 400
-    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8, 0, 0])
+    phys = np.array([1.4, 1.4 - 1, 1 / (1.4 - 1.0), 1.4 / (1.4 - 1.0), 0.8, 0, 0, 1])
     qL, qR = _cons(rng, n, mach), _cons(rng, n, mach)
     qR = qL + 0.3 * (qR - qL)  # nearby states, the usual regime of a face
     lam = rng.standard_normal((n, 4))

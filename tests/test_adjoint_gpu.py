@@ -164,17 +164,19 @@ def test_gradient_matches_central_differences():
     assert kinks <= U0.numel() // 50
 
 
-def test_gradient_matches_differences_of_the_cpu_oracle():
+@pytest.mark.parametrize("nq", [1, 3])
+def test_gradient_matches_differences_of_the_cpu_oracle(nq):
     """The GPU adjoint gradient against a reference gradient obtained by differencing the CPU path
     itself: J evaluated by the NumPy oracle with the same frozen dt schedule (10x10 smoothed
-    explosion, ramp half-width 1, 3 SSP-RK2 steps), central differences on 4 seeded unit
-    directions at h = 1e-7 |U0|."""
+    explosion, ramp half-width 1, 3 SSP-RK2 steps; midpoint and 3-point Gauss faces), central
+    differences on 4 seeded unit directions at h = 1e-7 |U0|."""
     import torch
 
     from tests.helpers import oracle_for
 
     n, steps = 10, 3
     cfg, mesh, target, _ = W.adjoint_problem(n, seed=0, width=1.0)
+    cfg.fvm_num_quadrature_points = nq
     prob = AdjointProblem(Euler2D(cfg, mesh), target, nsteps=steps, checkpoint_every=2)
     J, g = prob.gradient()
     dts = [float(x) for x in prob.dt.cpu().numpy()[:, 0]]
@@ -189,15 +191,23 @@ def test_gradient_matches_differences_of_the_cpu_oracle():
     U0 = prob.U0.cpu().numpy()
     assert abs(J_cpu(U0) - J) <= 1e-13 * abs(J)       # same trajectory (summation order aside)
     rng = np.random.default_rng(1)
-    h = 1e-7 * float(np.linalg.norm(U0))
+    h0 = 1e-7 * float(np.linalg.norm(U0))
     gh = g.cpu().numpy()
     errs = []
     for _ in range(4):
         v = rng.standard_normal(U0.size)
         v /= np.linalg.norm(v)
-        fd = (J_cpu(U0 + h * v) - J_cpu(U0 - h * v)) / (2 * h)
         ad = float(gh @ v)
-        errs.append(abs(fd - ad) / abs(ad))
-    print("relative errors against differences of the CPU path", errs)
-    assert max(errs) < 1e-6, errs
+        # the map is piecewise smooth: a limiter or wave-speed switch at distance d along v spoils
+        # the central difference for h > d only, so the step is refined until it is below d
+        e = []
+        for h in (h0, h0 / 10, h0 / 100):
+            fd = (J_cpu(U0 + h * v) - J_cpu(U0 - h * v)) / (2 * h)
+            e.append(abs(fd - ad) / abs(ad))
+            if e[-1] < 1e-6:
+                break
+        errs.append(e)
+    print("relative errors against differences of the CPU path (per direction, h refined)", errs)
+    assert all(e[-1] < 1e-6 for e in errs), errs
+    assert sum(len(e) > 1 for e in errs) <= 1, errs
     assert torch.isfinite(g).all()
