@@ -479,7 +479,8 @@ __host__ __device__ inline void reconstruct_vjp_var(double u, double uw, double 
 
 // reconstruct_vjp_var with nq Gauss points per face (reconstruct_q_var): bq[f][i] = adjoint of the
 // limited value at point i of face f (E, W, N, S); phi is the minimum over the 4*nq points in the
-// order E points, W points, N points, S points, and the first minimum is the active one.
+// order E points, W points, N points, S points, and the first minimum is the active one.  Only the
+// active point's slope data is kept; the high-order terms are recomputed in the reverse sweep.
 __host__ __device__ inline void reconstruct_vjp_var_q(double u, double uw, double ue, double us, double un,
                                                       const CellGeo& g, const QGeo& q, int nq,
                                                       const double bq[4][QMAX], int lim, double out[5]) {
@@ -495,50 +496,39 @@ __host__ __device__ inline void reconstruct_vjp_var_q(double u, double uw, doubl
         if (vals[t] < mn) { mn = vals[t]; imin = t; }
     }
     const double dmax = mx - u, dmin = mn - u;
-    double hf[4][QMAX], sf[4][QMAX], df[4][QMAX];
-    int fa = 0, ia = 0;
-    double pa = 0.0;
+    double pa = 0.0, da = 0.0, sa = 0.0, xa = 0.0, ya = 0.0;  // active point: phi, d, slope, offset
     for (int f = 0; f < 4; ++f)
         for (int i = 0; i < QMAX; ++i) {
-            hf[f][i] = sf[f][i] = df[f][i] = 0.0;
             if (i >= nq) continue;
-            hf[f][i] = gx * q.dx[f][i] + gy * q.dy[f][i];
-            df[f][i] = (u + hf[f][i]) - u;
-            sf[f][i] = face_slope(dmax, dmin, df[f][i]);
-            const double p = limiter_fn(lim, sf[f][i]);
-            if ((f == 0 && i == 0) || p < pa) { pa = p; fa = f; ia = i; }
+            const double h = gx * q.dx[f][i] + gy * q.dy[f][i];
+            const double d = (u + h) - u;
+            const double sl = face_slope(dmax, dmin, d);
+            const double p = limiter_fn(lim, sl);
+            if ((f == 0 && i == 0) || p < pa) { pa = p; da = d; sa = sl; xa = q.dx[f][i]; ya = q.dy[f][i]; }
         }
     const double phi = (pa < 0.0) ? 0.0 : pa;
     // Q_fi = u + phi h_fi
-    double bu = 0.0, bphi = 0.0, bh[4][QMAX];
+    double bu = 0.0, bphi = 0.0, bgx = 0.0, bgy = 0.0;
     for (int f = 0; f < 4; ++f)
         for (int i = 0; i < QMAX; ++i) {
-            bh[f][i] = 0.0;
             if (i >= nq) continue;
+            const double h = gx * q.dx[f][i] + gy * q.dy[f][i];
             bu += bq[f][i];
-            bphi += bq[f][i] * hf[f][i];
-            bh[f][i] = phi * bq[f][i];
+            bphi += bq[f][i] * h;
+            const double bh = phi * bq[f][i];
+            bgx += bh * q.dx[f][i];
+            bgy += bh * q.dy[f][i];
         }
     double bval[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (!(pa < 0.0)) {
-        const double dfs = df[fa][ia];
-        if (dfs > 0.0 || dfs < 0.0) {
-            const double sv = sf[fa][ia];
-            const double bsv = bphi * limiter_prime(lim, sv);
-            const double bnum = bsv / dfs;
-            const double bd = -bsv * sv / dfs;
-            if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }
-            else { bval[imin] += bnum; bu -= bnum; }
-            bh[fa][ia] += bd;
-        }
+    if (!(pa < 0.0) && (da > 0.0 || da < 0.0)) {
+        const double bsv = bphi * limiter_prime(lim, sa);
+        const double bnum = bsv / da;
+        const double bd = -bsv * sa / da;
+        if (da > 0.0) { bval[imax] += bnum; bu -= bnum; }
+        else { bval[imin] += bnum; bu -= bnum; }
+        bgx += bd * xa;
+        bgy += bd * ya;
     }
-    double bgx = 0.0, bgy = 0.0;
-    for (int f = 0; f < 4; ++f)
-        for (int i = 0; i < QMAX; ++i) {
-            if (i >= nq) continue;
-            bgx += bh[f][i] * q.dx[f][i];
-            bgy += bh[f][i] * q.dy[f][i];
-        }
     const double be = (bgx * g.xlE + bgy * g.ylE) * g.ainv;
     const double bw = (bgx * g.xlW + bgy * g.ylW) * g.ainv;
     const double bn = (bgx * g.xlN + bgy * g.ylN) * g.ainv;

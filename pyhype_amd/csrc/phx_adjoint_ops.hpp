@@ -295,7 +295,8 @@ __host__ __device__ inline Q4 outer_own_adj(const DevBlock& B, int side, const Q
 }
 
 // pass 3 reverse, nq Gauss points per face: adjoints of the 4*nq point states of cell (k, j)
-// (face records of FA*nq doubles) -> reconstruction VJP over all points.
+// (face records of FA*nq doubles) -> reconstruction VJP over all points, one variable at a time
+// (a boundary point's outer-state term mixes the variables, so it is taken per variable again).
 template <bool CART>
 __host__ __device__ inline void cell_adj_q(const DevBlock* blocks, const DevBlock& B, const double* U,
                                            const AdjBlock& O, const double* fadj, int k, int j, double* slot,
@@ -303,26 +304,11 @@ __host__ __device__ inline void cell_adj_q(const DevBlock* blocks, const DevBloc
     const int nx = B.nx, ny = B.ny;
     const int nq = npts<QMAX>(ph);
     const long long fs = (long long)FA * nq;
-    Q4 b[4][QMAX];
-    bool any = false;
-    for (int i = 0; i < QMAX; ++i) {
-        for (int f = 0; f < 4; ++f) b[f][i] = q4zero();
-        if (i >= nq) continue;
-        b[PHX_E][i] = ld(fadj + fs * xface_index(O, B, k, j + 1) + FA * i);       // left state of the east face
-        b[PHX_W][i] = ld(fadj + fs * xface_index(O, B, k, j) + FA * i + 4);       // right state of the west face
-        b[PHX_N][i] = ld(fadj + fs * yface_index(O, B, k + 1, j) + FA * i);       // left state of the north face
-        b[PHX_S][i] = ld(fadj + fs * yface_index(O, B, k, j) + FA * i + 4);       // right state of the south face
-        if (j == nx - 1)
-            acc(b[PHX_E][i], outer_own_adj<CART>(B, PHX_E, ld(fadj + fs * xface_index(O, B, k, nx) + FA * i + 4), k, nx));
-        if (j == 0) acc(b[PHX_W][i], outer_own_adj<CART>(B, PHX_W, ld(fadj + fs * xface_index(O, B, k, 0) + FA * i), k, 0));
-        if (k == ny - 1)
-            acc(b[PHX_N][i],
-                outer_own_adj<CART>(B, PHX_N, ld(fadj + fs * yface_index(O, B, ny, j) + FA * i + 4), ny - 1, j));
-        if (k == 0) acc(b[PHX_S][i], outer_own_adj<CART>(B, PHX_S, ld(fadj + fs * yface_index(O, B, 0, j) + FA * i), 0, j));
-        for (int f = 0; f < 4; ++f) any = any || !is_zero(b[f][i]);
-    }
+    const double* fE = fadj + fs * xface_index(O, B, k, j + 1);      // left states of the east face
+    const double* fW = fadj + fs * xface_index(O, B, k, j) + 4;      // right states of the west face
+    const double* fN = fadj + fs * yface_index(O, B, k + 1, j);      // left states of the north face
+    const double* fS = fadj + fs * yface_index(O, B, k, j) + 4;      // right states of the south face
     for (int q = 0; q < SLOT; ++q) slot[q] = 0.0;
-    if (!any) return;
     const Q4 Uc = dev::fetch_r<CART, true>(blocks, B, U, k, j, ph);
     const Q4 Uw = dev::fetch_r<CART, true>(blocks, B, U, k, j - 1, ph);
     const Q4 Ue = dev::fetch_r<CART, true>(blocks, B, U, k, j + 1, ph);
@@ -333,11 +319,25 @@ __host__ __device__ inline void cell_adj_q(const DevBlock* blocks, const DevBloc
     for (int v = 0; v < 4; ++v) {
         double bq[4][QMAX];
         bool nz = false;
-        for (int f = 0; f < 4; ++f)
-            for (int i = 0; i < QMAX; ++i) {
-                bq[f][i] = b[f][i].v[v];
-                nz = nz || bq[f][i] != 0.0;
-            }
+        for (int i = 0; i < QMAX; ++i) {
+            for (int f = 0; f < 4; ++f) bq[f][i] = 0.0;
+            if (i >= nq) continue;
+            bq[PHX_E][i] = fE[FA * i + v];
+            bq[PHX_W][i] = fW[FA * i + v];
+            bq[PHX_N][i] = fN[FA * i + v];
+            bq[PHX_S][i] = fS[FA * i + v];
+            if (j == nx - 1)
+                bq[PHX_E][i] += outer_own_adj<CART>(B, PHX_E, ld(fadj + fs * xface_index(O, B, k, nx) + FA * i + 4),
+                                                    k, nx).v[v];
+            if (j == 0)
+                bq[PHX_W][i] += outer_own_adj<CART>(B, PHX_W, ld(fadj + fs * xface_index(O, B, k, 0) + FA * i), k, 0).v[v];
+            if (k == ny - 1)
+                bq[PHX_N][i] += outer_own_adj<CART>(B, PHX_N, ld(fadj + fs * yface_index(O, B, ny, j) + FA * i + 4),
+                                                    ny - 1, j).v[v];
+            if (k == 0)
+                bq[PHX_S][i] += outer_own_adj<CART>(B, PHX_S, ld(fadj + fs * yface_index(O, B, 0, j) + FA * i), 0, j).v[v];
+            for (int f = 0; f < 4; ++f) nz = nz || bq[f][i] != 0.0;
+        }
         if (!nz) continue;
         double o[5];
         reconstruct_vjp_var_q(Uc.v[v], Uw.v[v], Ue.v[v], Us.v[v], Un.v[v], g, qg, nq, bq, ph.limiter, o);
