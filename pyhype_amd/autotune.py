@@ -5,6 +5,8 @@ item recomputes one reconstruction row above and below its range, so short items
 long items leave SIMDs idle on small meshes.  Shared-halo variant: workgroups of 4 waves with
 ``rows_per_item`` rows each exchange their boundary rows through LDS.  Split variant: a reconstruction kernel with no row
 halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per item).
+Element-wise variant: one lane per cell and variable for the reconstruction, one per face for
+the fluxes and one per cell for the update (no shape parameters).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
 flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
@@ -73,6 +75,7 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
             for r in shared_rows]
     out += [{"variant": 1, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in split_rows for rb in split_rows_b]
+    out += [{"variant": 3, "rows_per_item": 8, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}]
     return out
 
 

@@ -23,6 +23,7 @@ struct DevBlock {
     int nbr[4];
     long long off;
     long long yoff;  // offset (doubles) of the block's (ny+1) x nx x 4 y-face fluxes in the split stage
+    long long foff;  // offset (faces) of the block's ny*(nx+1) x-faces, then (ny+1)*nx y-faces (element-wise stage)
     const double* geom;
     const double* qoff;  // Gauss-point offsets (8*nq arrays of ny x nx) when nq > 1, else null
     double wall[8];

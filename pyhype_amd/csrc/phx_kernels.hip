@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/pyhype_hip.h"
+#include "phx_adjoint_ops.hpp"
 #include "phx_device.hpp"
 #include "phx_numerics.hpp"
 #include "phx_plan.hpp"
@@ -36,6 +37,16 @@ constexpr int HALO = 2;
 constexpr int STRIP = WAVE - 2 * HALO;  // owned columns per wavefront
 constexpr unsigned long long DT_INF_BITS = 0x7FF0000000000000ull;  // +inf
 constexpr int GRAPH_STEPS = 8;  // steps captured in the multi-step graph of phx_run
+// The CFL minimum of a case is accumulated in DT_SLOTS partial minima, each on a cache line of its
+// own (DT_STRIDE words apart): the waves of a stage spread their atomicMin over the slots instead of
+// serialising on one address, and k_finalize takes the minimum over the slots (an exact operation,
+// independent of the order).
+constexpr int DT_SLOTS = 64;
+constexpr int DT_STRIDE = 16;
+__host__ __device__ constexpr long dt_slot(int c, int slot) { return ((long)c * DT_SLOTS + slot) * DT_STRIDE; }
+__device__ __forceinline__ int wave_slot() {
+    return (int)((blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) & (DT_SLOTS - 1));
+}
 
 struct StageArgs {
     const DevBlock* blocks;
@@ -57,6 +68,8 @@ struct StageArgs {
     double* sn;        // split stage: limited N and S states per cell (8 doubles, offset 2*off)
     double* p1;        // split stage: LW*(2 FW) - LE*(2 FE) per cell (4 doubles, state layout)
     double* yf;        // split stage: y-face fluxes (DevBlock::yoff)
+    double* rb;        // element-wise stage: limited face states E, W, N, S per cell (16 doubles)
+    double* ff;        // element-wise stage: face fluxes (4 doubles per face, DevBlock::foff)
     Phys ph;
 };
 
@@ -127,7 +140,7 @@ __device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& 
         const bool anybad = __any(lane_bad);
         if (lane == 0) {
             const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot())], bits);
             if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
         }
     }
@@ -671,6 +684,108 @@ __global__ void __launch_bounds__(WAVE) k_update(StageArgs A) {
     for (int i = first; i < hi; i += stride) update_item<CART>(A, ld_const(A.items, i));
 }
 
+// ---------------------------------------------------------------------------------------------
+// element-wise stage (small meshes, where one lane per cell leaves most SIMDs with one or two
+// waves): the same arithmetic as stage_item in three passes with more lanes in flight,
+//   k_ew_recon   one lane per cell and variable: limited face states E, W, N, S (the variables of
+//                the reconstruction are independent: reconstruct_var)
+//   k_ew_face    one lane per face: left/right states (block-edge faces: outer state) -> flux
+//   k_ew_update  one lane per cell: residual, every Butcher output, CFL quantity, realizability
+// Tiles {block, first element, count}: EW_WG / 4 cells, EW_WG faces, EW_WG cells.  Cells are numbered
+// k*nx + j in their block; faces as in the adjoint (x-faces (k, jf) first, then y-faces (kf, j)).
+// ---------------------------------------------------------------------------------------------
+constexpr int EW_WG = 256;
+constexpr int EW_RCELLS = EW_WG / 4;
+
+// variable v of cell (k, j) (one cell outside the block at most, never a corner), in the
+// reconstruction variables; own cells are read directly, ghosts through fetch_r
+template <bool CART, bool GEN>
+__device__ __forceinline__ double ew_value(const StageArgs& A, const DevBlock& B, int k, int j, int v,
+                                           const Phys& ph) {
+    if (!(GEN && ph.prim) && k >= 0 && k < B.ny && j >= 0 && j < B.nx)
+        return gld(A.Uin, B.off + 4 * ((long)k * B.nx + j) + v);
+    const Q4 q = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k, j, ph);
+    return v == 0 ? q.v[0] : (v == 1 ? q.v[1] : (v == 2 ? q.v[2] : q.v[3]));
+}
+
+template <bool CART, bool GEN>
+__device__ void ew_recon_tile(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
+    if (c >= it.r0) return;
+    const int e = it.j0 + c;
+    const int k = e / B.nx, j = e - k * B.nx;
+    const Phys ph = A.ph;
+    const double u = ew_value<CART, GEN>(A, B, k, j, v, ph);
+    const double uw = ew_value<CART, GEN>(A, B, k, j - 1, v, ph);
+    const double ue = ew_value<CART, GEN>(A, B, k, j + 1, v, ph);
+    const double us = ew_value<CART, GEN>(A, B, k - 1, j, v, ph);
+    const double un = ew_value<CART, GEN>(A, B, k + 1, j, v, ph);
+    const phx::CellGeo g = cell_geo<CART>(B, k, j);
+    double qE, qW, qN, qS;
+    phx::reconstruct_var<GEN>(u, uw, ue, us, un, g, ph.limiter, qE, qW, qN, qS);
+    double* rb = A.rb + phx::adj::RB * (B.off / 4 + e) + v;
+    rb[phx::adj::RB_E] = qE;
+    rb[phx::adj::RB_W] = qW;
+    rb[phx::adj::RB_N] = qN;
+    rb[phx::adj::RB_S] = qS;
+}
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void ew_face_tile(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    if ((int)threadIdx.x >= it.r0) return;
+    const long long f = (long long)it.j0 + threadIdx.x;
+    const phx::host::AdjBlock O{B.off / 4, B.foff};
+    const phx::adj::FaceRef F = phx::adj::face_ref(B, f);
+    Q4 L, R;
+    phx::adj::face_states<CART>(A.blocks, B, A.Uin, A.rb, O, F, L, R, A.ph);
+    const Q4 Fl = F.xdir ? xflux<CART, FLUX, GEN>(B, L, R, F.k, F.j, A.ph)
+                         : yflux<CART, FLUX, GEN>(B, L, R, F.k, F.j, A.ph);
+    st4(A.ff + 4 * (B.foff + f), Fl);
+}
+
+template <bool CART>
+__device__ void ew_update_tile(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
+    const int t = (int)threadIdx.x;
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    if (t < it.r0) {
+        const int e = it.j0 + t;
+        const int k = e / B.nx, j = e - k * B.nx;
+        const phx::host::AdjBlock O{B.off / 4, B.foff};
+        const Q4 Res = phx::adj::cell_residual<CART>(B, O, A.ff, k, j);
+        const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
+        emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+    }
+    reduce_item(A, B, t & (WAVE - 1), lane_dt, lane_bad);  // every lane of the wave takes part
+}
+
+template <bool CART, bool GEN>
+__global__ void __launch_bounds__(EW_WG) k_ew_recon(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) ew_recon_tile<CART, GEN>(A, ld_const(A.items, i));
+}
+
+template <bool CART, int FLUX, bool GEN>
+__global__ void __launch_bounds__(EW_WG) k_ew_face(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) ew_face_tile<CART, FLUX, GEN>(A, ld_const(A.items, i));
+}
+
+template <bool CART>
+__global__ void __launch_bounds__(EW_WG) k_ew_update(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) ew_update_tile<CART>(A, ld_const(A.items, i));
+}
+
 // standalone CFL reduction over the owned cells of every work item
 struct DtArgs {
     const DevBlock* blocks;
@@ -712,7 +827,7 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
         const bool anybad = __any(lane_bad);
         if (lane == 0) {
             const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[B.case_id], bits);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot())], bits);
             if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
         }
     }
@@ -720,24 +835,35 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
 
 // dt = CFL*min, clipped to t_final - t (solvers/base.py:126-136); advance t (Euler2D.py:209).
 // One workgroup; a case that failed at an earlier step gets dt = 0 and keeps its time.
+// One wave per case: lane l reads (and, when advancing, clears) slot l of the case's partial minima.
 __global__ void k_finalize(int ncases, double cfl, unsigned long long* dtmin, const double* tfinal, double* t,
                            double* dt, int* step, const int* bad, const double* frozen, int advance) {
+    static_assert(DT_SLOTS == WAVE, "one lane per slot");
     const int s_next = step[0] + 1;
-    for (int c = threadIdx.x; c < ncases; c += blockDim.x) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    for (int c = threadIdx.x / WAVE; c < ncases; c += blockDim.x / WAVE) {
+        unsigned long long m = DT_INF_BITS;
+        if (frozen == nullptr) {
+            m = dtmin[dt_slot(c, lane)];
+#pragma unroll
+            for (int o = WAVE / 2; o > 0; o >>= 1) {
+                const unsigned long long x = __shfl_xor(m, o, WAVE);
+                m = x < m ? x : m;
+            }
+        }
+        if (advance) dtmin[dt_slot(c, lane)] = DT_INF_BITS;
+        if (lane != 0) continue;
         double d;
         if (frozen != nullptr) {
             d = frozen[c];
         } else {
-            d = cfl * __longlong_as_double((long long)dtmin[c]);
+            d = cfl * __longlong_as_double((long long)m);
             const double rem = tfinal[c] - t[c];
             if (rem < d) d = rem;
         }
         if (advance && bad[c] < s_next) d = 0.0;
         dt[c] = d;
-        if (advance) {
-            t[c] = t[c] + d;
-            dtmin[c] = DT_INF_BITS;
-        }
+        if (advance) t[c] = t[c] + d;
     }
     __syncthreads();
     if (advance && threadIdx.x == 0) step[0] = s_next;
@@ -745,18 +871,18 @@ __global__ void k_finalize(int ncases, double cfl, unsigned long long* dtmin, co
 
 __global__ void k_reset(int ncases, const double* t0, double* t, unsigned long long* dtmin, int* bad, int* step,
                         int step0) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (case, slot)
+    if (c < ncases * DT_SLOTS) dtmin[(long)c * DT_STRIDE] = DT_INF_BITS;
     if (c < ncases) {
         t[c] = t0[c];
-        dtmin[c] = DT_INF_BITS;
         bad[c] = 0x7fffffff;
     }
     if (c == 0) step[0] = step0;
 }
 
 __global__ void k_fill_inf(int ncases, unsigned long long* dtmin) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncases) dtmin[c] = DT_INF_BITS;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (case, slot)
+    if (i < ncases * DT_SLOTS) dtmin[(long)i * DT_STRIDE] = DT_INF_BITS;
 }
 
 // ghost strips for tests: block b, cell index i over E(ny) W(ny) N(nx) S(nx)
@@ -862,6 +988,10 @@ int build_items(phx_plan* P) {
         if (P->ditems[kind]) { (void)hipFree(P->ditems[kind]); P->ditems[kind] = nullptr; }
         if (P->ditems_b[kind]) { (void)hipFree(P->ditems_b[kind]); P->ditems_b[kind] = nullptr; }
         if (P->ditems_f[kind]) { (void)hipFree(P->ditems_f[kind]); P->ditems_f[kind] = nullptr; }
+        for (int p = 0; p < 3; ++p) {
+            P->items_e[p][kind].clear();
+            if (P->ditems_e[p][kind]) { (void)hipFree(P->ditems_e[p][kind]); P->ditems_e[p][kind] = nullptr; }
+        }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
     const int strip = P->variant == 1 ? STRIP_A : STRIP;
@@ -872,6 +1002,12 @@ int build_items(phx_plan* P) {
         for (int r0 = 0; r0 < d.ny; r0 += item_rows)
             for (int j0 = 0; j0 < d.nx; j0 += strip)
                 P->items[kind].push_back({b, j0, r0, std::min(r0 + item_rows, d.ny)});
+        if (P->variant == 3) {
+            const int nc = d.nx * d.ny, nf = d.ny * (d.nx + 1) + (d.ny + 1) * d.nx;
+            for (int e = 0; e < nc; e += EW_RCELLS) P->items_e[0][kind].push_back({b, e, std::min(EW_RCELLS, nc - e), 0});
+            for (int e = 0; e < nf; e += EW_WG) P->items_e[1][kind].push_back({b, e, std::min(EW_WG, nf - e), 0});
+            for (int e = 0; e < nc; e += EW_WG) P->items_e[2][kind].push_back({b, e, std::min(EW_WG, nc - e), 0});
+        }
         if (P->variant == 1) {
             for (int r0 = 0; r0 <= d.ny; r0 += rows)
                 for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
@@ -887,6 +1023,10 @@ int build_items(phx_plan* P) {
     if (rc) return rc;
     rc = upload_items(P->items_f, P->ditems_f, P->grid_f, P->grid_waves);
     if (rc) return rc;
+    for (int p = 0; p < 3; ++p) {
+        rc = upload_items(P->items_e[p], P->ditems_e[p], P->grid_e[p], P->grid_waves);
+        if (rc) return rc;
+    }
     release_graphs(P);
     return PHX_OK;
 }
@@ -954,6 +1094,24 @@ struct k_yface_q_tag {
         hipLaunchKernelGGL((k_yface_q<C, F>), g, dim3(WAVE), 0, st, A);
     }
 };
+struct k_ew_recon_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_ew_recon<C, G>), g, dim3(EW_WG), 0, st, A);
+    }
+};
+struct k_ew_face_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_ew_face<C, F, G>), g, dim3(EW_WG), 0, st, A);
+    }
+};
+struct k_ew_update_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_ew_update<C>), g, dim3(EW_WG), 0, st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1019,6 +1177,23 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
                                           2, st);
         return;
     }
+    if (P->variant == 3) {
+        A.rb = P->ew_rb;
+        A.ff = P->ew_ff;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_e[0][kind].empty())
+                launch_kind<k_ew_recon_tag>(P, A, kind, P->ditems_e[0][kind], (int)P->items_e[0][kind].size(),
+                                            P->grid_e[0][kind], 2, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_e[1][kind].empty())
+                launch_kind<k_ew_face_tag>(P, A, kind, P->ditems_e[1][kind], (int)P->items_e[1][kind].size(),
+                                           P->grid_e[1][kind], 2, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_e[2][kind].empty())
+                launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
+                                             P->grid_e[2][kind], 2, st);
+        return;
+    }
     if (P->variant == 2) {
         A.rows_wave = P->rows_per_item > 0 ? P->rows_per_item : 1;
         for (int kind = 0; kind < 2; ++kind)
@@ -1035,7 +1210,7 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
 }
 
 void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_inf, dim3((P->ncases + 255) / 256), dim3(256), 0, st, P->ncases, P->dtmin);
+    hipLaunchKernelGGL(k_fill_inf, dim3((P->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, st, P->ncases, P->dtmin);
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
@@ -1108,7 +1283,8 @@ int phx_plan_destroy(phx_plan* P) {
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
     void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->ditems_f[0],
-                    P->ditems_f[1], P->sn, P->p1, P->yf,
+                    P->ditems_f[1], P->sn, P->p1, P->yf, P->ew_rb, P->ew_ff, P->ditems_e[0][0], P->ditems_e[0][1],
+                    P->ditems_e[1][0], P->ditems_e[1][1], P->ditems_e[2][0], P->ditems_e[2][1],
                     P->sbuf[0], P->sbuf[1], P->dt, P->t, P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1216,6 +1392,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
         h.off = d.state_offset;
         h.yoff = P->yf_len;
         P->yf_len += 4LL * (d.ny + 1) * d.nx;
+        h.foff = P->nfaces;
+        P->nfaces += (long long)d.ny * (d.nx + 1) + (long long)(d.ny + 1) * d.nx;
         h.geom = P->dgeom + goff[b];
         h.qoff = nq > 1 ? h.geom + (d.geom_len - 8LL * nq * d.nx * d.ny) : nullptr;
         if (d.geom_kind == PHX_GEOM_CARTESIAN)
@@ -1248,7 +1426,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipMalloc(&P->dt, cb));
     PHX_CKP(hipMalloc(&P->t, cb));
     PHX_CKP(hipMalloc(&P->tfinal, cb));
-    PHX_CKP(hipMalloc(&P->dtmin, D->ncases * sizeof(unsigned long long)));
+    PHX_CKP(hipMalloc(&P->dtmin, (size_t)D->ncases * DT_SLOTS * DT_STRIDE * sizeof(unsigned long long)));
     PHX_CKP(hipMalloc(&P->step, sizeof(int)));
     PHX_CKP(hipMalloc(&P->bad, D->ncases * sizeof(int)));
     PHX_CKP(hipMemcpy(P->tfinal, D->t_final, cb, hipMemcpyHostToDevice));
@@ -1257,7 +1435,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     double* t0 = nullptr;
     PHX_CKP(hipMalloc(&t0, cb));
     PHX_CKP(hipMemcpy(t0, zeros.data(), cb, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_reset, dim3((D->ncases + 255) / 256), dim3(256), 0, 0, D->ncases, t0, P->t, P->dtmin,
+    hipLaunchKernelGGL(k_reset, dim3((D->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, 0, D->ncases, t0, P->t, P->dtmin,
                        P->bad, P->step, 0);
     PHX_CKP(hipGetLastError());
     PHX_CKP(hipDeviceSynchronize());
@@ -1277,7 +1455,7 @@ int phx_reset(phx_plan* P, const double* t0_host, int32_t step0, void* stream) {
     double* t0 = nullptr;
     PHX_CK(hipMalloc(&t0, cb));
     PHX_CK(hipMemcpy(t0, t0_host, cb, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_reset, dim3((P->ncases + 255) / 256), dim3(256), 0, st, P->ncases, t0, P->t, P->dtmin,
+    hipLaunchKernelGGL(k_reset, dim3((P->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, st, P->ncases, t0, P->t, P->dtmin,
                        P->bad, P->step, step0);
     PHX_CK(hipGetLastError());
     PHX_CK(hipStreamSynchronize(st));
@@ -1417,7 +1595,7 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 2 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 3 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     if (P->ph.nq > 1 && variant != 1)
         return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stage only (variant 1)");
     PHX_CK(hipDeviceSynchronize());
@@ -1426,6 +1604,11 @@ int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
         PHX_CK(hipMalloc(&P->sn, 2 * (size_t)std::max(P->ph.nq, 1) * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
         PHX_CK(hipMalloc(&P->yf, (size_t)std::max<long long>(P->yf_len, 4) * sizeof(double)));
+    }
+    if (variant == 3 && !P->ew_rb) {
+        const size_t cells = (size_t)std::max<int64_t>(P->state_len, 4) / 4;
+        PHX_CK(hipMalloc(&P->ew_rb, phx::adj::RB * cells * sizeof(double)));
+        PHX_CK(hipMalloc(&P->ew_ff, 4 * (size_t)std::max<long long>(P->nfaces, 1) * sizeof(double)));
     }
     P->variant = variant;
     if (rows_b > 0) P->rows_b = rows_b;

@@ -16,7 +16,8 @@ namespace host {
 constexpr int MAX_OUT = 8;  // outputs of one stage program (stage state + accumulators)
 constexpr int MAX_ACC = 8;  // accumulator buffers of the RK programs
 
-// One work item of the stage kernel: block, first owned column, row range [r0, r1).
+// One work item of the stage kernel: block, first owned column, row range [r0, r1).  Tiles of the
+// element-wise stage use the same record as {block, first element, element count, 0}.
 struct WorkItem {
     int block, j0, r0, r1;
 };
@@ -71,7 +72,8 @@ struct phx_plan {
     std::vector<phx::host::WorkItem> items[2];  // [0] general, [1] cartesian
     phx::host::WorkItem* ditems[2] = {nullptr, nullptr};
     int grid[2] = {0, 0};
-    // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, y-faces, update)
+    // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, y-faces, update),
+    // 2 = shared-halo workgroups, 3 = element-wise (cell-variable lanes, face lanes, cell lanes)
     int variant = 0, rows_b = 8;
     std::vector<phx::host::WorkItem> items_b[2];  // split variant: update kernel
     phx::host::WorkItem* ditems_b[2] = {nullptr, nullptr};
@@ -79,6 +81,13 @@ struct phx_plan {
     std::vector<phx::host::WorkItem> items_f[2];  // split variant: y-face kernel
     phx::host::WorkItem* ditems_f[2] = {nullptr, nullptr};
     int grid_f[2] = {0, 0};
+    // element-wise variant: tiles of [0] the reconstruction (cells), [1] the faces, [2] the update (cells)
+    std::vector<phx::host::WorkItem> items_e[3][2];
+    phx::host::WorkItem* ditems_e[3][2] = {};
+    int grid_e[3][2] = {};
+    double* ew_rb = nullptr;  // element-wise variant: limited face states E, W, N, S per cell (16 doubles)
+    double* ew_ff = nullptr;  // element-wise variant: face fluxes (4 doubles per face, DevBlock::foff)
+    long long nfaces = 0;     // faces of all blocks (x-faces and y-faces)
     double* sn = nullptr;  // split variant: N/S states per cell
     double* p1 = nullptr;  // split variant: x-face part of the residual per cell
     double* yf = nullptr;  // split variant: y-face fluxes

@@ -243,6 +243,59 @@ def test_split_variant_matches_oracle(case, rows, rows_b, waves):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
+@pytest.mark.parametrize("case", ["single", "multiblock", "oblique", "skewed", "wide", "wide_multiblock", "tall"])
+def test_element_wise_variant_matches_oracle(case):
+    """The element-wise stage (variant 3: one lane per cell and variable for the reconstruction, one
+    per face for the fluxes, one per cell for the update) gives the oracle's results bit for bit:
+    residual, steps with the CFL reduction, ghosts of every boundary kind; tiles that end inside a
+    row and partial last tiles (block sizes not multiples of 64 or 256)."""
+    if case == "single":
+        cfg, mesh = W.explosion_single(26)
+    elif case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(14, 2, 4)
+    elif case == "oblique":
+        cfg, mesh = W.oblique_shock(18, 3)
+    elif case == "wide":
+        cfg, mesh = W.explosion_single(130)
+    elif case == "wide_multiblock":
+        cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    elif case == "tall":            # 3 x 40 cells: a tile spans many rows
+        from pyhype_amd.initial_conditions import ExplosionInitialCondition
+        cfg, mesh = W._config(3, 40, ic=ExplosionInitialCondition()), W.single_block_mesh()
+    else:
+        cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
+    s = _solver(cfg, mesh)
+    s.plan.set_launch(4, 0, 2, variant=3)
+    assert s.plan.launch["variant"] == 3
+    st = _perturbed(s.states()[0], seed=19)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(5)
+    sim.run(5)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+def test_element_wise_variant_batched_cases_with_failures():
+    """Batched uneven cases on the element-wise stage, one of which becomes unrealizable at step 1
+    and is frozen (its tiles are skipped): states, times and failures equal the fused stage's."""
+    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52], density_scaled=True)[0]
+    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28], density_scaled=True)[0]
+    meshes = [W.single_block_mesh() for _ in cfgs]
+    a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b.plan.set_launch(4, 0, 2, variant=3)
+    for s in (a, b):
+        s.apply_initial_condition()
+        s.run_steps(7)
+    for x, y in zip(a.states(), b.states()):
+        assert bitwise_equal_blocks(x, y)
+    assert a.failed == b.failed == {2: 1}
+    assert list(a.t) == list(b.t)
+
+
 def test_split_variant_batched_cases():
     cfgs, meshes, _ = W.explosion_batch(seed=3, ncases=4, sizes=[40, 64, 33, 70])
     a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
@@ -257,15 +310,15 @@ def test_split_variant_batched_cases():
 
 
 @pytest.mark.parametrize("limiter", ["BarthJespersen", "VanAlbada", "VanLeer"])
-@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("variant", [0, 1, 3])
 def test_other_limiters_match_oracle(limiter, variant):
     """SlopeLimiterFactory's other limiters (limiters/limiters.py:25-62) on the general-geometry
-    multiblock mesh: residual and RK4 steps bit for bit, fused and split stage."""
+    multiblock mesh: residual and RK4 steps bit for bit, fused, split and element-wise stage."""
     cfg, mesh = W.explosion_multiblock(12, 2, 2)
     cfg.fvm_slope_limiter_type = limiter
     s = _solver(cfg, mesh, rows_per_item=3)
     if variant:
-        s.plan.set_launch(2, 0, 3, variant=1, rows_b=5)
+        s.plan.set_launch(2, 0, 3 if variant == 1 else 2, variant=variant, rows_b=5)
     st = _perturbed(s.states()[0], seed=21)
     s.set_state([st])
     R = s.plan.get_states(s.plan.residual())[0]
@@ -322,7 +375,7 @@ def test_shared_halo_variant_batched_cases_and_limiter():
     assert a.failed == b.failed
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("case", ["multiblock", "oblique"])
 def test_primitive_reconstruction_matches_oracle(case, variant):
     """reconstruction_type = PrimitiveState (fvm/base.py:286-289, blocks/quad_block.py:225-240):
@@ -405,13 +458,13 @@ def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
     s = _solver(cfg, mesh, rows_per_item=2)
-    for v in (0, 2):
+    for v in (0, 2, 3):
         with pytest.raises(ValueError, match="Gauss point"):
             s.plan.set_launch(2, 0, 2, variant=v)
     s.run_steps(2)   # the plan still runs after the refusals
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("integrator", ["ExplicitEuler1", "RK2", "Ralston2", "Generic2", "RK3", "RK3SSP",
                                         "Ralston3", "Ralston4", "DormandPrince5", "Generic3"])
 def test_every_butcher_table_matches_oracle(integrator, variant):
@@ -437,7 +490,7 @@ def test_every_butcher_table_matches_oracle(integrator, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 def test_batch_mixing_cartesian_and_rotated_frame_blocks(variant):
     """One plan whose cases mix Cartesian blocks (one block), rotated-frame blocks with neighbours
     (a 2x2 rectangular mesh, off-by-ulp interior corners) and a skewed two-block mesh: both kernel
