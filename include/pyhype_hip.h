@@ -207,10 +207,12 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * reconstruction + x-face kernel, a y-face kernel and a residual + update kernel, which evaluates
  * every reconstruction and face flux exactly once (best for small meshes, where short fused items
  * spend most of their work on halo rows); rows_b = rows per work item of the update kernel
- * (0 = keep).  All variants give identical results.  Rebuilds the work lists; the split variant
- * allocates about 8*nq + 8 doubles per cell.  A plan with more than one Gauss point per face runs
- * the split variant only (variants 0 and 2 return PHX_E_INVALID).  No counterpart in the
- * reference. */
+ * (0 = keep); 3 = element-wise: one lane per cell and variable for the reconstruction, one lane
+ * per face for the fluxes, one lane per cell for the update (no shape parameters).  All variants
+ * give identical results.  Rebuilds the work lists; the split variant allocates about 8*nq + 8
+ * doubles per cell, the element-wise variant 16 per cell and 4 per face.  A plan with more than
+ * one Gauss point per face runs the split variant only (variants 0, 2 and 3 return
+ * PHX_E_INVALID).  No counterpart in the reference. */
 int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 
 /* Number of work items and the persistent grid used for the Cartesian and general kernels. */

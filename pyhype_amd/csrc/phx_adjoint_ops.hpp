@@ -110,18 +110,20 @@ struct FaceRef {
     int k, j;  // x-face: row k, face column j;  y-face: face row k, column j
 };
 
-__host__ __device__ inline FaceRef face_ref(const DevBlock& B, long long f) {
+// (32-bit index arithmetic: a block has fewer than 2^31 faces)
+__host__ __device__ inline FaceRef face_ref(const DevBlock& B, long long f64) {
     FaceRef r;
-    const long long nxf = (long long)B.ny * (B.nx + 1);
+    int f = (int)f64;
+    const int nxf = B.ny * (B.nx + 1);
     if (f < nxf) {
         r.xdir = true;
-        r.k = (int)(f / (B.nx + 1));
-        r.j = (int)(f % (B.nx + 1));
+        r.k = f / (B.nx + 1);
+        r.j = f - r.k * (B.nx + 1);
     } else {
         f -= nxf;
         r.xdir = false;
-        r.k = (int)(f / B.nx);
-        r.j = (int)(f % B.nx);
+        r.k = f / B.nx;
+        r.j = f - r.k * B.nx;
     }
     return r;
 }

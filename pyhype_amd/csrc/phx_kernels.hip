@@ -37,15 +37,21 @@ constexpr int HALO = 2;
 constexpr int STRIP = WAVE - 2 * HALO;  // owned columns per wavefront
 constexpr unsigned long long DT_INF_BITS = 0x7FF0000000000000ull;  // +inf
 constexpr int GRAPH_STEPS = 8;  // steps captured in the multi-step graph of phx_run
-// The CFL minimum of a case is accumulated in DT_SLOTS partial minima, each on a cache line of its
-// own (DT_STRIDE words apart): the waves of a stage spread their atomicMin over the slots instead of
-// serialising on one address, and k_finalize takes the minimum over the slots (an exact operation,
-// independent of the order).
+// The CFL minimum of a case is accumulated in nslot partial minima (a power of two, at most
+// DT_SLOTS), each on a cache line of its own (DT_STRIDE words apart): the waves of a stage spread
+// their atomicMin over the slots instead of serialising on one address, and k_finalize takes the
+// minimum over the slots (an exact operation, independent of the order).  Plans with many cases
+// use fewer slots per case (their waves are spread over the cases already).
 constexpr int DT_SLOTS = 64;
 constexpr int DT_STRIDE = 16;
-__host__ __device__ constexpr long dt_slot(int c, int slot) { return ((long)c * DT_SLOTS + slot) * DT_STRIDE; }
-__device__ __forceinline__ int wave_slot() {
-    return (int)((blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) & (DT_SLOTS - 1));
+__host__ __device__ constexpr long dt_slot(int c, int slot, int nslot) { return ((long)c * nslot + slot) * DT_STRIDE; }
+__device__ __forceinline__ int wave_slot(int nslot) {
+    return (int)((blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) & (nslot - 1));
+}
+int dt_slots_for(int ncases) {
+    int n = DT_SLOTS;
+    while (n > 4 && (long)n * ncases > 256) n >>= 1;
+    return n;
 }
 
 struct StageArgs {
@@ -70,6 +76,7 @@ struct StageArgs {
     double* yf;        // split stage: y-face fluxes (DevBlock::yoff)
     double* rb;        // element-wise stage: limited face states E, W, N, S per cell (16 doubles)
     double* ff;        // element-wise stage: face fluxes (4 doubles per face, DevBlock::foff)
+    int nslot;         // partial CFL minima per case
     Phys ph;
 };
 
@@ -140,7 +147,7 @@ __device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& 
         const bool anybad = __any(lane_bad);
         if (lane == 0) {
             const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot())], bits);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot(A.nslot), A.nslot)], bits);
             if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
         }
     }
@@ -796,6 +803,7 @@ struct DtArgs {
     unsigned long long* dtmin;
     int* bad;
     const int* step;
+    int nslot;
     Phys ph;
 };
 
@@ -827,7 +835,7 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
         const bool anybad = __any(lane_bad);
         if (lane == 0) {
             const unsigned long long bits = (unsigned long long)__double_as_longlong(m);
-            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot())], bits);
+            if (bits < DT_INF_BITS) atomicMin(&A.dtmin[dt_slot(B.case_id, wave_slot(A.nslot), A.nslot)], bits);
             if (anybad) atomicMin(&A.bad[B.case_id], *A.step);
         }
     }
@@ -835,24 +843,28 @@ __global__ void __launch_bounds__(WAVE) k_dt(DtArgs A) {
 
 // dt = CFL*min, clipped to t_final - t (solvers/base.py:126-136); advance t (Euler2D.py:209).
 // One workgroup; a case that failed at an earlier step gets dt = 0 and keeps its time.
-// One wave per case: lane l reads (and, when advancing, clears) slot l of the case's partial minima.
-__global__ void k_finalize(int ncases, double cfl, unsigned long long* dtmin, const double* tfinal, double* t,
-                           double* dt, int* step, const int* bad, const double* frozen, int advance) {
-    static_assert(DT_SLOTS == WAVE, "one lane per slot");
+// One lane per slot (nslot lanes per case, one memory round trip), the minima combined by shuffles
+// inside the lane group; slots cleared when advancing.  The case loop bound is uniform, so every
+// lane takes part in the shuffles.  Launched as one workgroup of FIN_THREADS threads.
+constexpr int FIN_THREADS = 1024;
+__global__ void __launch_bounds__(FIN_THREADS) k_finalize(int ncases, int nslot, double cfl, unsigned long long* dtmin,
+                                                          const double* tfinal, double* t, double* dt, int* step,
+                                                          const int* bad, const double* frozen, int advance) {
     const int s_next = step[0] + 1;
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (int c = threadIdx.x / WAVE; c < ncases; c += blockDim.x / WAVE) {
+    const int g = threadIdx.x & (nslot - 1);
+    const int per_pass = blockDim.x / nslot;
+    for (int base = 0; base < ncases; base += per_pass) {
+        const int c = base + threadIdx.x / nslot;
+        const bool active = c < ncases;
         unsigned long long m = DT_INF_BITS;
-        if (frozen == nullptr) {
-            m = dtmin[dt_slot(c, lane)];
-#pragma unroll
-            for (int o = WAVE / 2; o > 0; o >>= 1) {
-                const unsigned long long x = __shfl_xor(m, o, WAVE);
-                m = x < m ? x : m;
-            }
+        if (active && frozen == nullptr) m = dtmin[dt_slot(c, g, nslot)];
+        for (int o = nslot / 2; o > 0; o >>= 1) {
+            const unsigned long long x = __shfl_xor(m, o, nslot);
+            m = x < m ? x : m;
         }
-        if (advance) dtmin[dt_slot(c, lane)] = DT_INF_BITS;
-        if (lane != 0) continue;
+        if (!active) continue;
+        if (advance) dtmin[dt_slot(c, g, nslot)] = DT_INF_BITS;
+        if (g != 0) continue;
         double d;
         if (frozen != nullptr) {
             d = frozen[c];
@@ -868,11 +880,10 @@ __global__ void k_finalize(int ncases, double cfl, unsigned long long* dtmin, co
     __syncthreads();
     if (advance && threadIdx.x == 0) step[0] = s_next;
 }
-
-__global__ void k_reset(int ncases, const double* t0, double* t, unsigned long long* dtmin, int* bad, int* step,
-                        int step0) {
+__global__ void k_reset(int ncases, int nslot, const double* t0, double* t, unsigned long long* dtmin, int* bad,
+                        int* step, int step0) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (case, slot)
-    if (c < ncases * DT_SLOTS) dtmin[(long)c * DT_STRIDE] = DT_INF_BITS;
+    if (c < ncases * nslot) dtmin[(long)c * DT_STRIDE] = DT_INF_BITS;
     if (c < ncases) {
         t[c] = t0[c];
         bad[c] = 0x7fffffff;
@@ -880,9 +891,9 @@ __global__ void k_reset(int ncases, const double* t0, double* t, unsigned long l
     if (c == 0) step[0] = step0;
 }
 
-__global__ void k_fill_inf(int ncases, unsigned long long* dtmin) {
+__global__ void k_fill_inf(int ncases, int nslot, unsigned long long* dtmin) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (case, slot)
-    if (i < ncases * DT_SLOTS) dtmin[(long)i * DT_STRIDE] = DT_INF_BITS;
+    if (i < ncases * nslot) dtmin[(long)i * DT_STRIDE] = DT_INF_BITS;
 }
 
 // ghost strips for tests: block b, cell index i over E(ny) W(ny) N(nx) S(nx)
@@ -1047,6 +1058,7 @@ StageArgs base_args(phx_plan* P) {
     A.state_out = -1;
     A.dt = P->dt;
     A.bad = P->bad;
+    A.nslot = P->nslot;
     A.step = P->step;
     A.ph = P->ph;
     return A;
@@ -1210,11 +1222,12 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
 }
 
 void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
-    hipLaunchKernelGGL(k_fill_inf, dim3((P->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, st, P->ncases, P->dtmin);
+    hipLaunchKernelGGL(k_fill_inf, dim3((P->ncases * P->nslot + 255) / 256), dim3(256), 0, st, P->ncases, P->nslot,
+                       P->dtmin);
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
-        DtArgs D{P->dblocks, P->ditems[kind], n, P->variant == 1 ? STRIP_A : STRIP, U, P->dtmin, P->bad, P->step,
+        DtArgs D{P->dblocks, P->ditems[kind], n, P->variant == 1 ? STRIP_A : STRIP, U, P->dtmin, P->bad, P->step, P->nslot,
                  P->ph};
         if (kind == 1) hipLaunchKernelGGL(k_dt<true>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
         else hipLaunchKernelGGL(k_dt<false>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
@@ -1259,7 +1272,7 @@ void launch_prog_stage(phx_plan* P, double* U, size_t s, hipStream_t st) {
 }
 
 void launch_step(phx_plan* P, double* U, hipStream_t st, const double* frozen) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, P->ncases, P->nslot, P->ph.cfl, P->dtmin, P->tfinal, P->t,
                        P->dt, P->step, P->bad, frozen, 1);
     for (size_t s = 0; s < P->progs.size(); ++s) launch_prog_stage(P, U, s, st);
     if (P->copy_back)
@@ -1362,6 +1375,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, P->device));
     P->nblocks = D->nblocks;
     P->ncases = D->ncases;
+    P->nslot = dt_slots_for(D->ncases);
     P->flux = D->flux;
     P->ph = Phys{D->gamma, D->gm1, D->one_over_gm1, D->g_over_gm1, D->cfl, D->limiter,
                  D->reconstruction == PHX_RECON_PRIMITIVE ? 1 : 0, nq};
@@ -1426,7 +1440,7 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipMalloc(&P->dt, cb));
     PHX_CKP(hipMalloc(&P->t, cb));
     PHX_CKP(hipMalloc(&P->tfinal, cb));
-    PHX_CKP(hipMalloc(&P->dtmin, (size_t)D->ncases * DT_SLOTS * DT_STRIDE * sizeof(unsigned long long)));
+    PHX_CKP(hipMalloc(&P->dtmin, (size_t)D->ncases * P->nslot * DT_STRIDE * sizeof(unsigned long long)));
     PHX_CKP(hipMalloc(&P->step, sizeof(int)));
     PHX_CKP(hipMalloc(&P->bad, D->ncases * sizeof(int)));
     PHX_CKP(hipMemcpy(P->tfinal, D->t_final, cb, hipMemcpyHostToDevice));
@@ -1435,7 +1449,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     double* t0 = nullptr;
     PHX_CKP(hipMalloc(&t0, cb));
     PHX_CKP(hipMemcpy(t0, zeros.data(), cb, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_reset, dim3((D->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, 0, D->ncases, t0, P->t, P->dtmin,
+    hipLaunchKernelGGL(k_reset, dim3((D->ncases * P->nslot + 255) / 256), dim3(256), 0, 0, D->ncases, P->nslot, t0, P->t,
+                       P->dtmin,
                        P->bad, P->step, 0);
     PHX_CKP(hipGetLastError());
     PHX_CKP(hipDeviceSynchronize());
@@ -1455,7 +1470,8 @@ int phx_reset(phx_plan* P, const double* t0_host, int32_t step0, void* stream) {
     double* t0 = nullptr;
     PHX_CK(hipMalloc(&t0, cb));
     PHX_CK(hipMemcpy(t0, t0_host, cb, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(k_reset, dim3((P->ncases * DT_SLOTS + 255) / 256), dim3(256), 0, st, P->ncases, t0, P->t, P->dtmin,
+    hipLaunchKernelGGL(k_reset, dim3((P->ncases * P->nslot + 255) / 256), dim3(256), 0, st, P->ncases, P->nslot, t0,
+                       P->t, P->dtmin,
                        P->bad, P->step, step0);
     PHX_CK(hipGetLastError());
     PHX_CK(hipStreamSynchronize(st));
@@ -1515,7 +1531,7 @@ int phx_run_record(phx_plan* P, double* U, int32_t nsteps, const double* dt_sche
     // step), so only the first input is copied.
     PHX_CK(hipMemcpyAsync(stages, U, len * sizeof(double), hipMemcpyDeviceToDevice, st));
     for (int n = 0; n < nsteps; ++n) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, P->ncases, P->nslot, P->ph.cfl, P->dtmin, P->tfinal, P->t,
                            P->dt, P->step, P->bad, dt_sched + (size_t)n * P->ncases, 1);
         double* rec = stages + (size_t)n * S * len;
         double* next = (n + 1 < nsteps) ? rec + (size_t)S * len : U;
@@ -1555,7 +1571,7 @@ int phx_dt(phx_plan* P, const double* U, double* dt_dev, void* stream) {
     if (!P || !U || !dt_dev) return fail(PHX_E_INVALID, "null argument");
     hipStream_t st = (hipStream_t)stream;
     launch_dt_reduce(P, U, st);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, P->ncases, P->ph.cfl, P->dtmin, P->tfinal, P->t,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, P->ncases, P->nslot, P->ph.cfl, P->dtmin, P->tfinal, P->t,
                        dt_dev, P->step, P->bad, (const double*)nullptr, 0);
     PHX_CK(hipGetLastError());
     return PHX_OK;

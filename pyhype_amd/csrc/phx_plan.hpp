@@ -99,7 +99,8 @@ struct phx_plan {
     double* sbuf[2] = {nullptr, nullptr};
     double* acc[phx::host::MAX_ACC] = {};
     double *dt = nullptr, *t = nullptr, *tfinal = nullptr;
-    unsigned long long* dtmin = nullptr;
+    unsigned long long* dtmin = nullptr;  // nslot partial CFL minima per case, one cache line each
+    int nslot = 1;
     int *step = nullptr, *bad = nullptr;
     long long* dghost_off = nullptr;
     hipStream_t cap_stream = nullptr;
