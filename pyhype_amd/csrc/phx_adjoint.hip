@@ -7,6 +7,7 @@
 //   k_adj_recon   limited face states of every cell at the linearisation point
 //   k_adj_face    seed of every face from kappa, flux VJP -> left/right state adjoints
 //   k_adj_cell    reconstruction VJP of every cell -> cell + 4 stencil-neighbour adjoints
+//   (midpoint faces: k_adj_recon_v and k_adj_cell_v, one lane per cell and variable)
 //   k_adj_gather  lambda_U of every cell from its own and its neighbours' slots, ghosts, neighbours
 // The forward trajectory is not stored by these kernels: the caller passes the stage states,
 // recorded by phx_run_record (frozen time steps) from a checkpoint.
@@ -107,6 +108,40 @@ __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const
     else phx::adj::cell_adj<false, NQ>(blocks, B, U, O, fadj, k, j, out, ph);
 }
 
+// Midpoint faces: the reconstruction and its VJP with one lane per cell and variable (4*TILE lanes
+// per tile of TILE cells): four times the waves of the per-cell kernels on the same tiles, and the
+// registers of one variable's chain.
+constexpr int VTILE = 4 * TILE;
+
+__global__ void __launch_bounds__(VTILE) k_adj_recon_v(const DevBlock* blocks, const AdjBlock* offs,
+                                                       const AdjTile* tiles, const double* U, double* rb, Phys ph) {
+    const AdjTile t = tiles[blockIdx.x];
+    const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
+    if (c >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const AdjBlock O = offs[t.block];
+    const int e = t.start + c;
+    const int k = e / B.nx, j = e - k * B.nx;
+    double* out = rb + (long long)phx::adj::RB * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::recon_cell_var<true>(blocks, B, U, k, j, v, out, ph);
+    else phx::adj::recon_cell_var<false>(blocks, B, U, k, j, v, out, ph);
+}
+
+__global__ void __launch_bounds__(VTILE) k_adj_cell_v(const DevBlock* blocks, const AdjBlock* offs,
+                                                      const AdjTile* tiles, const double* U, const double* fadj,
+                                                      double* slot, Phys ph) {
+    const AdjTile t = tiles[blockIdx.x];
+    const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
+    if (c >= t.count) return;
+    const DevBlock& B = blocks[t.block];
+    const AdjBlock O = offs[t.block];
+    const int e = t.start + c;
+    const int k = e / B.nx, j = e - k * B.nx;
+    double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::adj::cell_adj_var<true>(blocks, B, U, O, fadj, k, j, v, out, ph);
+    else phx::adj::cell_adj_var<false>(blocks, B, U, O, fadj, k, j, v, out, ph);
+}
+
 // lamU = (dR/dU)^T kappa + extra terms (pointwise; extra may include lamU itself)
 template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
@@ -184,17 +219,25 @@ int adj_setup(phx_plan* P) {
 template <int NQ>
 void launch_residual_vjp_nq(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
                             hipStream_t st) {
-    const dim3 blk(TILE);
+    const dim3 blk(TILE), vblk(VTILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
-    hipLaunchKernelGGL(k_adj_recon<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
+    if (NQ == 1)
+        hipLaunchKernelGGL(k_adj_recon_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
+    else
+        hipLaunchKernelGGL(k_adj_recon<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
+                           P->ph);
     if (P->flux == PHX_FLUX_ROE)
         hipLaunchKernelGGL((k_adj_face<0, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
     else
         hipLaunchKernelGGL((k_adj_face<1, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
                            kappa, P->adj_face, P->ph);
-    hipLaunchKernelGGL(k_adj_cell<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
-                       P->adj_slot, P->ph);
+    if (NQ == 1)
+        hipLaunchKernelGGL(k_adj_cell_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
+                           P->adj_slot, P->ph);
+    else
+        hipLaunchKernelGGL(k_adj_cell<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
+                           P->adj_slot, P->ph);
     hipLaunchKernelGGL(k_adj_gather<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot,
                        P->adj_face, U, P->ph, extra, lamU);
 }

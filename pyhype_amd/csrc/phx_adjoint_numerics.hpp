@@ -429,41 +429,41 @@ __host__ __device__ inline void reconstruct_vjp_var(double u, double uw, double 
         if (vals[q] < mn) { mn = vals[q]; imin = q; }
     }
     const double dmax = mx - u, dmin = mn - u;
-    double phif[4], sf[4], df[4];
+    // the active face (first minimum of the four limiter values) and its d and slope; the arrays
+    // are indexed by loop constants only, so that they stay in registers on the device
+    int fa = 0;
+    double pa = 0.0, da = 0.0, sa = 0.0;
     for (int f = 0; f < 4; ++f) {
-        df[f] = (u + hf[f]) - u;
-        sf[f] = face_slope(dmax, dmin, df[f]);
-        phif[f] = limiter_fn(lim, sf[f]);
+        const double d = (u + hf[f]) - u;
+        const double sl = face_slope(dmax, dmin, d);
+        const double p = limiter_fn(lim, sl);
+        if (f == 0 || p < pa) { pa = p; da = d; sa = sl; fa = f; }
     }
-    int fmin = 0;
-    for (int f = 1; f < 4; ++f)
-        if (phif[f] < phif[fmin]) fmin = f;
-    const double phi_raw = phif[fmin];
+    const double phi_raw = pa;
     const double phi = (phi_raw < 0.0) ? 0.0 : phi_raw;
     // Q_f = u + phi h_f
     double bu = bq[0] + bq[1] + bq[2] + bq[3];
-    double bphi = 0.0, bh[4];
-    for (int f = 0; f < 4; ++f) {
-        bphi += bq[f] * hf[f];
-        bh[f] = phi * bq[f];
-    }
+    double bphi = 0.0;
+    for (int f = 0; f < 4; ++f) bphi += bq[f] * hf[f];
     double bval[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (!(phi_raw < 0.0)) {
-        const double dfs = df[fmin];
-        if (dfs > 0.0 || dfs < 0.0) {
-            const double sv = sf[fmin];
-            const double bsv = bphi * limiter_prime(lim, sv);
-            const double bnum = bsv / dfs;
-            const double bd = -bsv * sv / dfs;
-            if (dfs > 0.0) { bval[imax] += bnum; bu -= bnum; }
-            else { bval[imin] += bnum; bu -= bnum; }
-            bh[fmin] += bd;
-        }
+    double bd = 0.0;
+    bool active = false;
+    if (!(phi_raw < 0.0) && (da > 0.0 || da < 0.0)) {
+        const double bsv = bphi * limiter_prime(lim, sa);
+        const double bnum = bsv / da;
+        bd = -bsv * sa / da;
+        active = true;
+        const int iv = (da > 0.0) ? imax : imin;
+        for (int t = 0; t < 5; ++t)
+            if (t == iv) bval[t] += bnum;
+        bu -= bnum;
     }
     double bgx = 0.0, bgy = 0.0;
     for (int f = 0; f < 4; ++f) {
-        bgx += bh[f] * hx[f];
-        bgy += bh[f] * hy[f];
+        double bh = phi * bq[f];
+        if (active && f == fa) bh += bd;
+        bgx += bh * hx[f];
+        bgy += bh * hy[f];
     }
     const double be = (bgx * g.xlE + bgy * g.ylE) * g.ainv;
     const double bw = (bgx * g.xlW + bgy * g.ylW) * g.ainv;
@@ -524,8 +524,10 @@ __host__ __device__ inline void reconstruct_vjp_var_q(double u, double uw, doubl
         const double bsv = bphi * limiter_prime(lim, sa);
         const double bnum = bsv / da;
         const double bd = -bsv * sa / da;
-        if (da > 0.0) { bval[imax] += bnum; bu -= bnum; }
-        else { bval[imin] += bnum; bu -= bnum; }
+        const int iv = (da > 0.0) ? imax : imin;
+        for (int t = 0; t < 5; ++t)
+            if (t == iv) bval[t] += bnum;
+        bu -= bnum;
         bgx += bd * xa;
         bgy += bd * ya;
     }

@@ -100,6 +100,38 @@ __host__ __device__ inline void recon_cell(const DevBlock* blocks, const DevBloc
             if (i < nq) st(rb + rbo(f, i, nq), qstate(r, qg, f, i));
 }
 
+// Component v of a Q4 (selection by constant indices: no private-array indexing on the device).
+__host__ __device__ inline double comp(const Q4& q, int v) {
+    return v == 0 ? q.v[0] : (v == 1 ? q.v[1] : (v == 2 ? q.v[2] : q.v[3]));
+}
+
+// Component v of dev::fetch_r (own conservative cells read directly).
+template <bool CART>
+__host__ __device__ inline double fetch_var(const DevBlock* blocks, const DevBlock& B, const double* U, int k, int j,
+                                            int v, const Phys& ph) {
+    if (!ph.prim && k >= 0 && k < B.ny && j >= 0 && j < B.nx) return U[B.off + 4 * ((long long)k * B.nx + j) + v];
+    return comp(dev::fetch_r<CART, true>(blocks, B, U, k, j, ph), v);
+}
+
+// recon_cell (midpoint faces) for variable v only: the same reconstruct_var on the same values, so
+// the four calls v = 0..3 write exactly what recon_cell writes (one lane per cell and variable).
+template <bool CART>
+__host__ __device__ inline void recon_cell_var(const DevBlock* blocks, const DevBlock& B, const double* U, int k,
+                                               int j, int v, double* rb, const Phys& ph) {
+    const double u = fetch_var<CART>(blocks, B, U, k, j, v, ph);
+    const double uw = fetch_var<CART>(blocks, B, U, k, j - 1, v, ph);
+    const double ue = fetch_var<CART>(blocks, B, U, k, j + 1, v, ph);
+    const double us = fetch_var<CART>(blocks, B, U, k - 1, j, v, ph);
+    const double un = fetch_var<CART>(blocks, B, U, k + 1, j, v, ph);
+    const CellGeo g = dev::cell_geo<CART>(B, k, j);
+    double qE, qW, qN, qS;
+    reconstruct_var<true>(u, uw, ue, us, un, g, ph.limiter, qE, qW, qN, qS);
+    rb[RB_E + v] = qE;
+    rb[RB_W + v] = qW;
+    rb[RB_N + v] = qN;
+    rb[RB_S + v] = qS;
+}
+
 // ---------------------------------------------------------------------------------------------
 // pass 2: the two states of a face.  Face f of block B: x-faces (k, jf), jf = 0..nx, first, then
 // y-faces (kf, j), kf = 0..ny.  Interior faces take the limited states of the cells on either
@@ -390,6 +422,38 @@ __host__ __device__ inline void cell_adj(const DevBlock* blocks, const DevBlock&
         slot[SL_S + v] = o[3];
         slot[SL_N + v] = o[4];
     }
+}
+
+// cell_adj (midpoint faces) for variable v only: component v of the same face adjoints (outer-state
+// terms of block-edge faces taken as Q4 and then selected), the same reconstruct_vjp_var; the four
+// calls v = 0..3 write exactly what cell_adj writes.
+template <bool CART>
+__host__ __device__ inline void cell_adj_var(const DevBlock* blocks, const DevBlock& B, const double* U,
+                                             const AdjBlock& O, const double* fadj, int k, int j, int v, double* slot,
+                                             const Phys& ph) {
+    const int nx = B.nx, ny = B.ny;
+    double bq[4] = {fadj[FA * xface_index(O, B, k, j + 1) + v], fadj[FA * xface_index(O, B, k, j) + 4 + v],
+                    fadj[FA * yface_index(O, B, k + 1, j) + v], fadj[FA * yface_index(O, B, k, j) + 4 + v]};
+    if (j == nx - 1) bq[0] += comp(outer_own_adj<CART>(B, PHX_E, ld(fadj + FA * xface_index(O, B, k, nx) + 4), k, nx), v);
+    if (j == 0) bq[1] += comp(outer_own_adj<CART>(B, PHX_W, ld(fadj + FA * xface_index(O, B, k, 0)), k, 0), v);
+    if (k == ny - 1)
+        bq[2] += comp(outer_own_adj<CART>(B, PHX_N, ld(fadj + FA * yface_index(O, B, ny, j) + 4), ny - 1, j), v);
+    if (k == 0) bq[3] += comp(outer_own_adj<CART>(B, PHX_S, ld(fadj + FA * yface_index(O, B, 0, j)), 0, j), v);
+    double o[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (bq[0] != 0.0 || bq[1] != 0.0 || bq[2] != 0.0 || bq[3] != 0.0) {
+        const double u = fetch_var<CART>(blocks, B, U, k, j, v, ph);
+        const double uw = fetch_var<CART>(blocks, B, U, k, j - 1, v, ph);
+        const double ue = fetch_var<CART>(blocks, B, U, k, j + 1, v, ph);
+        const double us = fetch_var<CART>(blocks, B, U, k - 1, j, v, ph);
+        const double un = fetch_var<CART>(blocks, B, U, k + 1, j, v, ph);
+        const CellGeo g = dev::cell_geo<CART>(B, k, j);
+        reconstruct_vjp_var(u, uw, ue, us, un, g, bq, ph.limiter, o);
+    }
+    slot[SL_C + v] = o[0];
+    slot[SL_W + v] = o[1];
+    slot[SL_E + v] = o[2];
+    slot[SL_S + v] = o[3];
+    slot[SL_N + v] = o[4];
 }
 
 // Adjoint, with respect to the edge cell's conservative state q, of the ghost value of a wall side,
