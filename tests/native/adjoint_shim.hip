@@ -5,6 +5,7 @@
 // with the host adjoint.
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../pyhype_amd/csrc/phx_adjoint_ops.hpp"
@@ -51,17 +52,27 @@ HostPlan make_plan(const phx_block_desc* d, int nb, int nq) {
     return P;
 }
 
+// by_var: the midpoint rule through recon_cell_var, one call per variable (as the device runs it)
 template <int NQ>
-void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, const Phys& ph) {
+void recon_all(const HostPlan& P, const double* U, std::vector<double>& rb, const Phys& ph, bool by_var = false) {
     const int nq = adj::npts<NQ>(ph);
     rb.assign((size_t)P.cells * adj::RB * nq, 0.0);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
+        const bool cart = B.kind == PHX_GEOM_CARTESIAN;
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 double* out = rb.data() + (long long)adj::RB * nq * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN) adj::recon_cell<true, NQ>(P.blocks.data(), B, U, k, j, out, ph);
-                else adj::recon_cell<false, NQ>(P.blocks.data(), B, U, k, j, out, ph);
+                if (NQ == 1 && by_var) {
+                    for (int v = 0; v < 4; ++v) {
+                        if (cart) adj::recon_cell_var<true>(P.blocks.data(), B, U, k, j, v, out, ph);
+                        else adj::recon_cell_var<false>(P.blocks.data(), B, U, k, j, v, out, ph);
+                    }
+                } else if (cart) {
+                    adj::recon_cell<true, NQ>(P.blocks.data(), B, U, k, j, out, ph);
+                } else {
+                    adj::recon_cell<false, NQ>(P.blocks.data(), B, U, k, j, out, ph);
+                }
             }
     }
 }
@@ -91,10 +102,11 @@ void residual_t(const HostPlan& P, const double* U, double* R, const Phys& ph) {
 }
 
 template <int FLUX, int NQ>
-void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, double* lamU, const Phys& ph) {
+void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, double* lamU, const Phys& ph,
+                    bool by_var = false) {
     const int nq = adj::npts<NQ>(ph);
     std::vector<double> rb, fadj((size_t)P.faces * adj::FA * nq), slot((size_t)P.cells * adj::SLOT, 0.0);
-    recon_all<NQ>(P, U, rb, ph);
+    recon_all<NQ>(P, U, rb, ph, by_var);
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const DevBlock& B = P.blocks[b];
         const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
@@ -113,10 +125,18 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
         for (int k = 0; k < B.ny; ++k)
             for (int j = 0; j < B.nx; ++j) {
                 double* out = slot.data() + adj::SLOT * adj::cell_index(P.offs[b], B, k, j);
-                if (B.kind == PHX_GEOM_CARTESIAN)
+                if (NQ == 1 && by_var) {
+                    for (int v = 0; v < 4; ++v) {
+                        if (B.kind == PHX_GEOM_CARTESIAN)
+                            adj::cell_adj_var<true>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, v, out, ph);
+                        else
+                            adj::cell_adj_var<false>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, v, out, ph);
+                    }
+                } else if (B.kind == PHX_GEOM_CARTESIAN) {
                     adj::cell_adj<true, NQ>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
-                else
+                } else {
                     adj::cell_adj<false, NQ>(P.blocks.data(), B, U, P.offs[b], fadj.data(), k, j, out, ph);
+                }
             }
     }
     for (size_t b = 0; b < P.blocks.size(); ++b) {
@@ -148,6 +168,28 @@ int shim_residual(const phx_block_desc* d, int nb, int flux, const double* phys,
         if (flux == 0) residual_t<0, 1>(P, U, R, ph);
         else residual_t<1, 1>(P, U, R, ph);
     }
+    return 0;
+}
+
+// limited face states of every cell (same layout as phx_face_states); by_var: through recon_cell_var
+int shim_face_states(const phx_block_desc* d, int nb, const double* phys, const double* U, double* out, int by_var) {
+    const Phys ph = make_phys(phys);
+    const HostPlan P = make_plan(d, nb, ph.nq);
+    std::vector<double> rb;
+    if (ph.nq > 1) recon_all<QMAX>(P, U, rb, ph);
+    else recon_all<1>(P, U, rb, ph, by_var != 0);
+    std::copy(rb.begin(), rb.end(), out);
+    return 0;
+}
+
+// shim_residual_vjp through the per-variable reconstruction and cell adjoint (midpoint faces)
+int shim_residual_vjp_var(const phx_block_desc* d, int nb, int flux, const double* phys, const double* U,
+                          const double* kappa, double* lamU) {
+    const Phys ph = make_phys(phys);
+    if (ph.nq > 1) return -1;
+    const HostPlan P = make_plan(d, nb, 1);
+    if (flux == 0) residual_vjp_t<0, 1>(P, U, kappa, lamU, ph, true);
+    else residual_vjp_t<1, 1>(P, U, kappa, lamU, ph, true);
     return 0;
 }
 

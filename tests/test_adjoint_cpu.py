@@ -124,6 +124,8 @@ def shim(adjoint_shim):
     s.shim_face_flux_vjp.argtypes = [C.c_int, C.c_int, dp, dp, dp, dp, dp, dp]
     s.shim_reconstruct_vjp.argtypes = [dp, dp, dp, dp, C.c_int]
     s.shim_reconstruct1.argtypes = [dp, dp, dp, C.c_int]
+    s.shim_face_states.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_int]
+    s.shim_residual_vjp_var.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
     return s
 
 
@@ -246,6 +248,26 @@ def test_residual_vjp_is_the_transposed_jacobian(shim, name):
     assert unexplained == 0
     assert kinks <= max(2, n // 100), kinks  # perturbed entries with a switch within the step
     assert np.count_nonzero(JT) > 20 * n  # the stencil is really there
+
+
+@pytest.mark.parametrize("name", [n for n in CASES if "gauss" not in n])
+def test_per_variable_element_operations_equal_the_per_cell_ones(shim, name):
+    """The device runs the midpoint reconstruction and its VJP with one lane per cell and variable
+    (recon_cell_var, cell_adj_var): their host build gives the per-cell functions' face states and
+    residual VJP bit for bit (including block-edge outer states, walls, primitive variables)."""
+    cfg, mesh, md, lay, flux = _setup(name)
+    phys = _phys(cfg)
+    U = _smooth_state(lay, seed=7, mach=0.9)
+    n = 16 * (U.size // 4)
+    a, b = np.zeros(n), np.zeros(n)
+    shim.shim_face_states(lay.descs, len(lay.blocks), _p(phys), _p(U), _p(a), 0)
+    shim.shim_face_states(lay.descs, len(lay.blocks), _p(phys), _p(U), _p(b), 1)
+    assert np.array_equal(a, b) and np.count_nonzero(a) > n // 2
+    w = np.random.default_rng(5).standard_normal(U.size)
+    ref = _vjp(shim, lay, flux, phys, U, w)
+    got = np.zeros_like(U)
+    assert shim.shim_residual_vjp_var(lay.descs, len(lay.blocks), flux, _p(phys), _p(U), _p(w), _p(got)) == 0
+    assert np.array_equal(got, ref)
 
 
 def test_residual_vjp_is_linear_and_deterministic(shim):
