@@ -6,7 +6,8 @@ long items leave SIMDs idle on small meshes.  Shared-halo variant: workgroups of
 ``rows_per_item`` rows each exchange their boundary rows through LDS.  Split variant: a reconstruction kernel with no row
 halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per item).
 Element-wise variant: one lane per cell and variable for the reconstruction, one per face for
-the fluxes and one per cell for the update (no shape parameters).
+the fluxes and one per cell for the update (no shape parameters).  Paired variant: the fused
+kernel's items with two waves each, one per pair of variables (``rows_per_item`` rows).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
 flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
@@ -67,8 +68,10 @@ def _time_steps(plan, steps: int) -> float:
 
 def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: Sequence[int] = (2, 3),
                split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (1, 2, 4, 8, 16, 32),
-               split_waves: Sequence[int] = (3, 4), shared_rows: Sequence[int] = (1, 2, 3, 4, 6)):
-    """The default search space: fused shapes, shared-halo shapes (rows per wave), split shapes."""
+               split_waves: Sequence[int] = (3, 4), shared_rows: Sequence[int] = (1, 2, 3, 4, 6),
+               paired_rows: Sequence[int] = (1, 2, 3, 4, 6, 8, 12)):
+    """The default search space: fused shapes, shared-halo shapes (rows per wave), split shapes,
+    the element-wise stage, paired shapes (rows per item)."""
     out = [{"variant": 0, "rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "rows_b": 0}
            for w in waves for g in grids for r in rows]
     out += [{"variant": 2, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}
@@ -76,6 +79,8 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
     out += [{"variant": 1, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in split_rows for rb in split_rows_b]
     out += [{"variant": 3, "rows_per_item": 8, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}]
+    out += [{"variant": 4, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": 0}
+            for w in (2, 3) for r in paired_rows]
     return out
 
 

@@ -423,6 +423,219 @@ __global__ void __launch_bounds__(WGW * WAVE, MINW) k_stage_wg(StageArgs A) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// paired stage: two waves per work item (the fused kernel's strip and rows, one halo row below).
+// Wave H reconstructs variables 2H and 2H+1 (the variables of the reconstruction are independent);
+// per row the waves exchange through LDS the halves of the face states the other needs (E and W
+// go to wave 0, N and S to wave 1), wave 0 evaluates the x-face flux and wave 1 the y-face flux,
+// the flux halves are exchanged, and each wave completes the residual and every Butcher output of
+// its two variables.  On the last stage of a step wave 1 hands its half of the new state to
+// wave 0, which evaluates the CFL quantity and realizability.  Same arithmetic, same association
+// order as stage_item: each variable and each face flux is computed exactly as there.  Both waves
+// run the same rows, so they reach the same barriers; every LDS slot is written after a barrier
+// that follows the other wave's last read of it.
+// ---------------------------------------------------------------------------------------------
+constexpr int PAIR_WAVES = 2;
+
+struct PairLds {
+    double x1[PAIR_WAVES][4][WAVE];  // face-state halves: wave 0 writes N, S (vars 0-1); wave 1 E, W (vars 2-3)
+    double x2[PAIR_WAVES][4][WAVE];  // flux halves: wave 0 writes FW, FE (vars 2-3); wave 1 F (vars 0-1)
+    double x3[2][WAVE];              // wave 1's half (vars 2-3) of the new state, last stage
+};
+
+template <int H>
+__device__ __forceinline__ double hv(const Q4& q, int i) { return q.v[2 * H + i]; }
+
+// the two variables of this wave of every output of the stage for cell (kk, j); o = the state
+// output's values of these variables
+template <bool CART, int H>
+__device__ __forceinline__ void emit_half(const StageArgs& A, const DevBlock& B, int kk, int j, double dt,
+                                          const double res[2], double o[2]) {
+    const long idx = B.off + 4 * ((long)kk * B.nx + j) + 2 * H;
+    if (A.Rout) *reinterpret_cast<double2*>(A.Rout + idx) = make_double2(res[0], res[1]);
+    constexpr int EMIT_CHUNK = 3;
+#pragma unroll 1
+    for (int m0 = 0; m0 < A.nout; m0 += EMIT_CHUNK) {
+        double2 s[EMIT_CHUNK];
+#pragma unroll
+        for (int q = 0; q < EMIT_CHUNK; ++q)
+            if (m0 + q < A.nout) s[q] = *reinterpret_cast<const double2*>(A.src[m0 + q] + idx);
+#pragma unroll
+        for (int q = 0; q < EMIT_CHUNK; ++q) {
+            const int m = m0 + q;
+            if (m >= A.nout) break;
+            const double am = A.a[m];
+            double2 r = s[q];
+            if (am != 0.0) {
+                const double cdt = dt * am;  // dt * a[stage][step] (explicit_runge_kutta.py:72)
+                r.x = s[q].x + cdt * res[0];
+                r.y = s[q].y + cdt * res[1];
+            }
+            *reinterpret_cast<double2*>(A.dst[m] + idx) = r;
+            if (m == A.state_out) { o[0] = r.x; o[1] = r.y; }
+        }
+    }
+}
+
+template <bool CART, int FLUX, bool GEN, int H>
+__device__ void stage_pair_body(const StageArgs& A, const WorkItem it, PairLds& L) {
+    const DevBlock& B = A.blocks[it.block];
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = B.nx, ny = B.ny;
+    const int j = it.j0 - HALO + lane;
+    const bool valid = (j >= -1) && (j <= nx);
+    const bool inner = (j >= 0) && (j < nx);
+    const bool gcol = (j == -1) || (j == nx);
+    const bool recon = inner && (j >= it.j0 - 1) && (j <= it.j0 + STRIP);
+    const bool xface = (j >= it.j0) && (j <= min(it.j0 + STRIP, nx));
+    const bool owner = inner && (j >= it.j0) && (j < it.j0 + STRIP);
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
+    const Phys ph = A.ph;
+    const bool cfl = A.dtmin != nullptr && A.state_out >= 0;
+
+    const int kr0 = max(it.r0 - 1, 0);
+    const int kr1 = min(it.r1, ny);  // row ny is a virtual row closing the north boundary face
+
+    Q4 Um = phx::q4zero(), Uc = phx::q4zero(), Up = phx::q4zero();
+    if (inner) Um = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0 - 1, j, ph);
+    if (valid) Uc = fetch_r<CART, GEN>(A.blocks, B, A.Uin, kr0, j, ph);
+
+    Q4 QNprev = phx::q4zero();  // wave 1: N state of the previous row
+    double part[2] = {0.0, 0.0}, prevLN = 0.0, prevA = 1.0;
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+
+    for (int k = kr0; k <= kr1; ++k) {
+        const bool real = (k < ny);
+        if (real) {
+            if (inner) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+            else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+        }
+        // reconstruction of this wave's two variables
+        double e[2] = {0.0, 0.0}, w[2] = {0.0, 0.0}, n[2] = {0.0, 0.0}, s[2] = {0.0, 0.0};
+        {
+            double uw[2], ue[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                uw[i] = __shfl_up(hv<H>(Uc, i), 1, WAVE);
+                ue[i] = __shfl_down(hv<H>(Uc, i), 1, WAVE);
+            }
+            if (real && recon) {
+                const phx::CellGeo g = cell_geo<CART>(B, k, j);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    phx::reconstruct_var<GEN>(hv<H>(Uc, i), uw[i], ue[i], hv<H>(Um, i), hv<H>(Up, i), g, ph.limiter,
+                                              e[i], w[i], n[i], s[i]);
+            }
+        }
+        // exchange 1: wave 0 needs E and W of variables 2-3, wave 1 N and S of variables 0-1
+        if (H == 0) {
+            L.x1[0][0][lane] = n[0]; L.x1[0][1][lane] = n[1];
+            L.x1[0][2][lane] = s[0]; L.x1[0][3][lane] = s[1];
+        } else {
+            L.x1[1][0][lane] = e[0]; L.x1[1][1][lane] = e[1];
+            L.x1[1][2][lane] = w[0]; L.x1[1][3][lane] = w[1];
+        }
+        __syncthreads();
+        const bool xrow = (k >= it.r0) && (k < it.r1);
+        double fw[2] = {0.0, 0.0}, fe[2] = {0.0, 0.0}, fy[2] = {0.0, 0.0};
+        if (H == 0) {
+            Q4 rE, rW;
+            rE.v[0] = e[0]; rE.v[1] = e[1]; rE.v[2] = L.x1[1][0][lane]; rE.v[3] = L.x1[1][1][lane];
+            rW.v[0] = w[0]; rW.v[1] = w[1]; rW.v[2] = L.x1[1][2][lane]; rW.v[3] = L.x1[1][3][lane];
+            if (!(real && recon)) rE = rW = phx::q4zero();
+            const Q4 qsend = (j == -1) ? Uc : rE;
+            const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 (ghost value for j == 0)
+            Q4 FW = phx::q4zero();
+            if (xrow && xface) {
+                const Q4 Lx = (j == 0) ? outer_state<CART>(B, PHX_W, rW, QEl, k, j) : QEl;
+                const Q4 Rx = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : rW;
+                FW = xflux<CART, FLUX, GEN>(B, Lx, Rx, k, j, ph);
+            }
+            const Q4 FE = shfl_down1(FW);
+            // exchange 2 (write): variables 2-3 of FW and FE for wave 1
+            L.x2[0][0][lane] = FW.v[2]; L.x2[0][1][lane] = FW.v[3];
+            L.x2[0][2][lane] = FE.v[2]; L.x2[0][3][lane] = FE.v[3];
+            fw[0] = FW.v[0]; fw[1] = FW.v[1];
+            fe[0] = FE.v[0]; fe[1] = FE.v[1];
+        } else {
+            Q4 rN, rS;
+            rN.v[0] = L.x1[0][0][lane]; rN.v[1] = L.x1[0][1][lane]; rN.v[2] = n[0]; rN.v[3] = n[1];
+            rS.v[0] = L.x1[0][2][lane]; rS.v[1] = L.x1[0][3][lane]; rS.v[2] = s[0]; rS.v[3] = s[1];
+            if (!(real && recon)) rN = rS = phx::q4zero();
+            Q4 F = phx::q4zero();
+            if (k >= it.r0 && owner) {
+                // face between rows k-1 and k (south face of row k, north face of row k-1)
+                const Q4 Ly = (k == 0) ? outer_state<CART>(B, PHX_S, rS, Um, k, j) : QNprev;
+                const Q4 Ry = (k == ny) ? outer_state<CART>(B, PHX_N, QNprev, Uc, k - 1, j) : rS;
+                F = yflux<CART, FLUX, GEN>(B, Ly, Ry, k, j, ph);
+            }
+            // exchange 2 (write): variables 0-1 of F for wave 0
+            L.x2[1][0][lane] = F.v[0]; L.x2[1][1][lane] = F.v[1];
+            fy[0] = F.v[2]; fy[1] = F.v[3];
+            QNprev = rN;
+        }
+        __syncthreads();
+        if (H == 0) {
+            fy[0] = L.x2[1][0][lane]; fy[1] = L.x2[1][1][lane];
+        } else {
+            fw[0] = L.x2[0][0][lane]; fw[1] = L.x2[0][1][lane];
+            fe[0] = L.x2[0][2][lane]; fe[1] = L.x2[0][3][lane];
+        }
+        double o[2] = {0.0, 0.0};
+        const bool emit = (k >= it.r0) && owner && (k - 1 >= it.r0);
+        if (k >= it.r0 && owner) {
+            if (emit) {
+                // residual of row k-1: 0.5*(W - E + S - N)/A with I = L*(2*F) (fvm/base.py:141-190)
+                double res[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) res[i] = (0.5 * (part[i] - prevLN * (2.0 * fy[i]))) / prevA;
+                emit_half<CART, H>(A, B, k - 1, j, dt, res, o);
+            }
+            if (xrow) {
+                double LW, LE, LS, LN, Ac;
+                cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) part[i] = (LW * (2.0 * fw[i]) - LE * (2.0 * fe[i])) + LS * (2.0 * fy[i]);
+                prevLN = LN;
+                prevA = Ac;
+            }
+        }
+        if (cfl) {  // uniform: the new state's CFL quantity and realizability on wave 0
+            if (H == 1) { L.x3[0][lane] = o[0]; L.x3[1][lane] = o[1]; }
+            __syncthreads();
+            if (H == 0 && emit) {
+                Q4 q;
+                q.v[0] = o[0]; q.v[1] = o[1]; q.v[2] = L.x3[0][lane]; q.v[3] = L.x3[1][lane];
+                double dx, dy;
+                cell_dxdy<CART>(B, k - 1, j, dx, dy);
+                const double t = phx::cell_dt(q, dx, dy, ph);
+                if (!(q.v[0] > 0.0) || !(q.v[3] > 0.0) || !(t == t)) lane_bad = true;
+                else lane_dt = fmin(lane_dt, t);
+            }
+        }
+        Um = Uc;
+        Uc = Up;
+    }
+    if (H == 0) reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+template <bool CART, int FLUX, int MINW, bool GEN>
+__global__ void __launch_bounds__(PAIR_WAVES * WAVE, MINW) k_stage_pair(StageArgs A) {
+    __shared__ PairLds L;
+    const int h = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        const DevBlock& B = A.blocks[it.block];
+        if (A.skip_dead && A.bad[B.case_id] < *A.step) continue;  // uniform over the workgroup
+        if (h == 0) stage_pair_body<CART, FLUX, GEN, 0>(A, it, L);
+        else stage_pair_body<CART, FLUX, GEN, 1>(A, it, L);
+        __syncthreads();  // the next item reuses the exchange buffers
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // split stage: for small meshes, where short rolling-row items would spend most of their work
 // on the halo rows they reconstruct again, the stage runs as three kernels: every cell is
 // reconstructed exactly once (k_recon_x, with the x-faces), every y-face flux is evaluated once
@@ -1081,6 +1294,13 @@ struct k_stage_wg_tag {
         hipLaunchKernelGGL((k_stage_wg<C, F, 2, G>), g, dim3(WGW * WAVE), 0, st, A);
     }
 };
+struct k_stage_pair_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        constexpr int MM = G ? 2 : (M > 3 ? 3 : (M < 2 ? 2 : M));
+        hipLaunchKernelGGL((k_stage_pair<C, F, MM, G>), g, dim3(PAIR_WAVES * WAVE), 0, st, A);
+    }
+};
 struct k_recon_x_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1204,6 +1424,13 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
             if (!P->items_e[2][kind].empty())
                 launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
                                              P->grid_e[2][kind], 2, st);
+        return;
+    }
+    if (P->variant == 4) {
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items[kind].empty())
+                launch_kind<k_stage_pair_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                              std::min(P->waves_per_simd, 3), st);
         return;
     }
     if (P->variant == 2) {
@@ -1611,7 +1838,7 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 3 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 4 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     if (P->ph.nq > 1 && variant != 1)
         return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stage only (variant 1)");
     PHX_CK(hipDeviceSynchronize());

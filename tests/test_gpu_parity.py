@@ -296,6 +296,55 @@ def test_element_wise_variant_batched_cases_with_failures():
     assert list(a.t) == list(b.t)
 
 
+@pytest.mark.parametrize("case,rows,waves", [("single", 1, 2), ("single", 3, 3), ("multiblock", 2, 2),
+                                             ("oblique", 2, 3), ("skewed", 1, 2), ("wide", 2, 2),
+                                             ("wide_multiblock", 4, 3)])
+def test_paired_variant_matches_oracle(case, rows, waves):
+    """Two waves per work item (variant 4: each wave reconstructs two variables, one evaluates the
+    x-face flux and the other the y-face flux, halves exchanged through LDS): residual and steps
+    bit for bit, the CFL reduction included."""
+    if case == "single":
+        cfg, mesh = W.explosion_single(26)
+    elif case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(14, 2, 4)
+    elif case == "oblique":
+        cfg, mesh = W.oblique_shock(18, 3)
+    elif case == "wide":
+        cfg, mesh = W.explosion_single(130)
+    elif case == "wide_multiblock":
+        cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    else:
+        cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
+    s = _solver(cfg, mesh)
+    s.plan.set_launch(rows, 0, waves, variant=4)
+    st = _perturbed(s.states()[0], seed=23)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(5)
+    sim.run(5)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
+def test_paired_variant_batched_cases_with_failures():
+    """Batched uneven cases on the paired stage, one of which fails at step 1 and is frozen."""
+    cfgs = W.explosion_batch(seed=5, ncases=2, sizes=[36, 52], density_scaled=True)[0]
+    cfgs += W.explosion_batch(seed=0, ncases=1, sizes=[28], density_scaled=True)[0]
+    meshes = [W.single_block_mesh() for _ in cfgs]
+    a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b = Euler2DBatch(cfgs, meshes, rows_per_item=2)
+    b.plan.set_launch(3, 0, 2, variant=4)
+    for s in (a, b):
+        s.apply_initial_condition()
+        s.run_steps(7)
+    for x, y in zip(a.states(), b.states()):
+        assert bitwise_equal_blocks(x, y)
+    assert a.failed == b.failed == {2: 1}
+    assert list(a.t) == list(b.t)
+
+
 def test_split_variant_batched_cases():
     cfgs, meshes, _ = W.explosion_batch(seed=3, ncases=4, sizes=[40, 64, 33, 70])
     a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
@@ -310,10 +359,11 @@ def test_split_variant_batched_cases():
 
 
 @pytest.mark.parametrize("limiter", ["BarthJespersen", "VanAlbada", "VanLeer"])
-@pytest.mark.parametrize("variant", [0, 1, 3])
+@pytest.mark.parametrize("variant", [0, 1, 3, 4])
 def test_other_limiters_match_oracle(limiter, variant):
     """SlopeLimiterFactory's other limiters (limiters/limiters.py:25-62) on the general-geometry
-    multiblock mesh: residual and RK4 steps bit for bit, fused, split and element-wise stage."""
+    multiblock mesh: residual and RK4 steps bit for bit, fused, split, element-wise and paired
+    stage."""
     cfg, mesh = W.explosion_multiblock(12, 2, 2)
     cfg.fvm_slope_limiter_type = limiter
     s = _solver(cfg, mesh, rows_per_item=3)
@@ -375,7 +425,7 @@ def test_shared_halo_variant_batched_cases_and_limiter():
     assert a.failed == b.failed
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("case", ["multiblock", "oblique"])
 def test_primitive_reconstruction_matches_oracle(case, variant):
     """reconstruction_type = PrimitiveState (fvm/base.py:286-289, blocks/quad_block.py:225-240):
@@ -458,13 +508,13 @@ def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
     s = _solver(cfg, mesh, rows_per_item=2)
-    for v in (0, 2, 3):
+    for v in (0, 2, 3, 4):
         with pytest.raises(ValueError, match="Gauss point"):
             s.plan.set_launch(2, 0, 2, variant=v)
     s.run_steps(2)   # the plan still runs after the refusals
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("integrator", ["ExplicitEuler1", "RK2", "Ralston2", "Generic2", "RK3", "RK3SSP",
                                         "Ralston3", "Ralston4", "DormandPrince5", "Generic3"])
 def test_every_butcher_table_matches_oracle(integrator, variant):
@@ -490,7 +540,7 @@ def test_every_butcher_table_matches_oracle(integrator, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_batch_mixing_cartesian_and_rotated_frame_blocks(variant):
     """One plan whose cases mix Cartesian blocks (one block), rotated-frame blocks with neighbours
     (a 2x2 rectangular mesh, off-by-ulp interior corners) and a skewed two-block mesh: both kernel
