@@ -208,10 +208,13 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * every reconstruction and face flux exactly once (best for small meshes, where short fused items
  * spend most of their work on halo rows); rows_b = rows per work item of the update kernel
  * (0 = keep); 3 = element-wise: one lane per cell and variable for the reconstruction, one lane
- * per face for the fluxes, one lane per cell for the update (no shape parameters).  All variants
- * give identical results.  Rebuilds the work lists; the split variant allocates about 8*nq + 8
+ * per face for the fluxes, one lane per cell for the update (no shape parameters); 4 = paired: the
+ * fused kernel's work items with two waves each, one per pair of variables, the x- and y-face
+ * fluxes split between them; 5 = split with the y-face and update kernels fused (each lane walks
+ * rows_b cell rows of its column, evaluating the face below the first row again).  All variants
+ * give identical results.  Rebuilds the work lists; the split variants allocate about 8*nq + 8
  * doubles per cell, the element-wise variant 16 per cell and 4 per face.  A plan with more than
- * one Gauss point per face runs the split variant only (variants 0, 2 and 3 return
+ * one Gauss point per face runs the split variant only (every other variant returns
  * PHX_E_INVALID).  No counterpart in the reference. */
 int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 

@@ -7,7 +7,8 @@ long items leave SIMDs idle on small meshes.  Shared-halo variant: workgroups of
 halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per item).
 Element-wise variant: one lane per cell and variable for the reconstruction, one per face for
 the fluxes and one per cell for the update (no shape parameters).  Paired variant: the fused
-kernel's items with two waves each, one per pair of variables (``rows_per_item`` rows).
+kernel's items with two waves each, one per pair of variables (``rows_per_item`` rows).  Variant
+5: the split stage with its y-face and update kernels fused (``rows_b`` cell rows per lane).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
 flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
@@ -81,6 +82,8 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
     out += [{"variant": 3, "rows_per_item": 8, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}]
     out += [{"variant": 4, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": 0}
             for w in (2, 3) for r in paired_rows]
+    out += [{"variant": 5, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
+            for w in split_waves for r in (1, 2) for rb in (1, 2, 3, 4, 8)]
     return out
 
 

@@ -867,6 +867,59 @@ __device__ void update_item(const StageArgs& A, const WorkItem it) {
     reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
+// y-faces and update in one pass (variant 5): the lane of column j walks the face rows of cell
+// rows [r0, r1); after the flux of face k+1 it has both y-fluxes of cell (k, j) and completes its
+// residual and Butcher outputs as update_item does.  Face r0 is evaluated again by the item
+// below (one extra flux per item and column instead of a y-flux buffer and a third kernel).
+template <bool CART, int FLUX, bool GEN>
+__device__ __forceinline__ Q4 yface_flux_at(const StageArgs& A, const DevBlock& B, const double* sn, int f, int j,
+                                            const Phys& ph) {
+    const int nx = B.nx, ny = B.ny;
+    Q4 L, R;
+    if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + 8 * (long)j + 4), fetch_r<CART, GEN>(A.blocks, B, A.Uin, -1, j, ph), 0, j);
+    else L = ld4(sn + 8 * ((long)(f - 1) * nx + j));
+    if (f == ny) R = outer_state<CART>(B, PHX_N, L, fetch_r<CART, GEN>(A.blocks, B, A.Uin, ny, j, ph), ny - 1, j);
+    else R = ld4(sn + 8 * ((long)f * nx + j) + 4);
+    return yflux<CART, FLUX, GEN>(B, L, R, f, j, ph);
+}
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void yface_update_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = B.nx;
+    const int j = it.j0 + lane;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
+    const Phys ph = A.ph;
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    if (j < nx) {
+        const double* sn = A.sn + 2 * B.off;
+        Q4 FS = yface_flux_at<CART, FLUX, GEN>(A, B, sn, it.r0, j, ph);
+        for (int k = it.r0; k < it.r1; ++k) {
+            const Q4 FN = yface_flux_at<CART, FLUX, GEN>(A, B, sn, k + 1, j, ph);
+            const Q4 p = ld4(A.p1 + B.off + 4 * ((long)k * nx + j));
+            double LW, LE, LS, LN, Ac;
+            cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+            Q4 Res;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+            emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+            FS = FN;
+        }
+    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+template <bool CART, int FLUX, bool GEN>
+__global__ void __launch_bounds__(WAVE) k_yface_update(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) yface_update_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
+}
+
 template <bool CART, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     int lo, hi, first, stride;
@@ -1218,7 +1271,8 @@ int build_items(phx_plan* P) {
         }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
-    const int strip = P->variant == 1 ? STRIP_A : STRIP;
+    const bool split = P->variant == 1 || P->variant == 5;
+    const int strip = split ? STRIP_A : STRIP;
     const int item_rows = P->variant == 2 ? WGW * rows : rows;  // shared-halo items: WGW waves of `rows`
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
@@ -1231,6 +1285,11 @@ int build_items(phx_plan* P) {
             for (int e = 0; e < nc; e += EW_RCELLS) P->items_e[0][kind].push_back({b, e, std::min(EW_RCELLS, nc - e), 0});
             for (int e = 0; e < nf; e += EW_WG) P->items_e[1][kind].push_back({b, e, std::min(EW_WG, nf - e), 0});
             for (int e = 0; e < nc; e += EW_WG) P->items_e[2][kind].push_back({b, e, std::min(EW_WG, nc - e), 0});
+        }
+        if (P->variant == 5) {
+            for (int r0 = 0; r0 < d.ny; r0 += P->rows_b)
+                for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
+                    P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
         }
         if (P->variant == 1) {
             for (int r0 = 0; r0 <= d.ny; r0 += rows)
@@ -1344,6 +1403,12 @@ struct k_ew_update_tag {
         hipLaunchKernelGGL((k_ew_update<C>), g, dim3(EW_WG), 0, st, A);
     }
 };
+struct k_yface_update_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yface_update<C, F, G>), g, dim3(WAVE), 0, st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1380,6 +1445,19 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
 }
 
 void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
+    if (P->variant == 5) {
+        A.sn = P->sn;
+        A.p1 = P->p1;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items[kind].empty())
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                           P->waves_per_simd, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_b[kind].empty())
+                launch_kind<k_yface_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                P->grid_b[kind], 2, st);
+        return;
+    }
     if (P->variant == 1) {
         A.sn = P->sn;
         A.p1 = P->p1;
@@ -1454,7 +1532,8 @@ void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
-        DtArgs D{P->dblocks, P->ditems[kind], n, P->variant == 1 ? STRIP_A : STRIP, U, P->dtmin, P->bad, P->step, P->nslot,
+        DtArgs D{P->dblocks, P->ditems[kind], n, (P->variant == 1 || P->variant == 5) ? STRIP_A : STRIP, U, P->dtmin,
+                 P->bad, P->step, P->nslot,
                  P->ph};
         if (kind == 1) hipLaunchKernelGGL(k_dt<true>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
         else hipLaunchKernelGGL(k_dt<false>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
@@ -1838,11 +1917,11 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 4 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 5 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     if (P->ph.nq > 1 && variant != 1)
         return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stage only (variant 1)");
     PHX_CK(hipDeviceSynchronize());
-    if (variant == 1 && !P->sn) {
+    if ((variant == 1 || variant == 5) && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
         PHX_CK(hipMalloc(&P->sn, 2 * (size_t)std::max(P->ph.nq, 1) * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
