@@ -1,5 +1,5 @@
 """Parity of the HIP backend with the oracle (gpu).  Every comparison is bit-for-bit unless noted;
-the north-star bar (max relative error <= 1e-9 per variable) is asserted as well."""
+the parity tolerance of the path (max relative error <= 1e-9 per variable) is asserted as well."""
 import numpy as np
 import pytest
 
