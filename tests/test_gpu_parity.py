@@ -214,7 +214,8 @@ def test_skewed_multiblock_residual_and_steps_match_oracle():
 
 @pytest.mark.parametrize("case,rows,rows_b,waves", [
     ("single", 1, 4, 3), ("single", 3, 32, 4), ("multiblock", 2, 8, 3), ("oblique", 4, 5, 4),
-    ("skewed", 1, 3, 3), ("wide", 2, 7, 3), ("wide_multiblock", 3, 16, 4)])
+    ("skewed", 1, 3, 3), ("wide", 2, 7, 3), ("wide_multiblock", 3, 16, 4), ("narrow", 1, 1, 3),
+    ("narrow", 2, 64, 4)])
 @pytest.mark.parametrize("variant", [1, 5])
 def test_split_variant_matches_oracle(case, rows, rows_b, waves, variant):
     """The split stage (reconstruction + x-faces, then y-faces, then the update; variant 5 with the
@@ -230,6 +231,9 @@ def test_split_variant_matches_oracle(case, rows, rows_b, waves, variant):
         cfg, mesh = W.explosion_single(130)
     elif case == "wide_multiblock":
         cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    elif case == "narrow":          # 3 x 40 cells: one strip, most lanes outside the block
+        from pyhype_amd.initial_conditions import ExplosionInitialCondition
+        cfg, mesh = W._config(3, 40, ic=ExplosionInitialCondition()), W.single_block_mesh()
     else:
         cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
     s = _solver(cfg, mesh, rows_per_item=rows)
@@ -300,7 +304,7 @@ def test_element_wise_variant_batched_cases_with_failures():
 
 @pytest.mark.parametrize("case,rows,waves", [("single", 1, 2), ("single", 3, 3), ("multiblock", 2, 2),
                                              ("oblique", 2, 3), ("skewed", 1, 2), ("wide", 2, 2),
-                                             ("wide_multiblock", 4, 3)])
+                                             ("wide_multiblock", 4, 3), ("narrow", 1, 2), ("narrow", 7, 3)])
 def test_paired_variant_matches_oracle(case, rows, waves):
     """Two waves per work item (variant 4: each wave reconstructs two variables, one evaluates the
     x-face flux and the other the y-face flux, halves exchanged through LDS): residual and steps
@@ -315,6 +319,9 @@ def test_paired_variant_matches_oracle(case, rows, waves):
         cfg, mesh = W.explosion_single(130)
     elif case == "wide_multiblock":
         cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    elif case == "narrow":
+        from pyhype_amd.initial_conditions import ExplosionInitialCondition
+        cfg, mesh = W._config(3, 40, ic=ExplosionInitialCondition()), W.single_block_mesh()
     else:
         cfg, mesh = W.explosion_single(20)[0], skewed_mesh()
     s = _solver(cfg, mesh)
