@@ -35,6 +35,8 @@ def frozen_schedule(plan, nsteps: int):
 class Trajectory:
     """``len(dt_sched)`` frozen-dt steps of ``plan`` as a map of the state, with checkpoints."""
 
+    RECORD_BUFFERS = 3  # segments whose recorded stage inputs can be held at once in the reverse sweep
+
     def __init__(self, plan, dt_sched, checkpoint_every: int = 10):
         if checkpoint_every < 1:
             raise ValueError("checkpoint_every must be >= 1")
@@ -46,7 +48,7 @@ class Trajectory:
         torch = plan.torch
         m = min(self.every, max(1, self.nsteps))
         self.stages = torch.empty((m, plan.nstages, plan.state_len), dtype=torch.float64, device="cuda")
-        self._stages_b = None   # second record buffer, allocated by the first backward with > 1 segment
+        self._stages_more = []  # further record buffers, allocated by the first backward with > 1 segment
         self._side = None       # stream of the segment recomputation (default priority)
 
     def forward(self, U0):
@@ -63,11 +65,12 @@ class Trajectory:
     def backward(self, lam_T):
         """Adjoint of U_0 from the adjoint of U_n (the checkpoints of the last forward).
 
-        The recomputation of segment k-1 (``run_record`` from its checkpoint, on a second stream)
-        overlaps the reverse sweep of segment k (``step_vjp``, on the caller's stream): the two
-        use disjoint device buffers (the plan's state and scratch versus its adjoint work
-        arrays), and the records alternate between two buffers, each reused only after the sweep
-        that read it has finished (events)."""
+        The recomputation of the next segments (``run_record`` from their checkpoints, on a second
+        stream) overlaps the reverse sweep of the current one (``step_vjp``, on the caller's
+        stream): the two use disjoint device buffers (the plan's state and scratch versus its
+        adjoint work arrays), and the records rotate through ``RECORD_BUFFERS`` buffers, each
+        reused only after the sweep that read it has finished (events), so the recomputation can
+        run up to two segments ahead of the sweep."""
         if len(self.checkpoints) != (self.nsteps + self.every - 1) // self.every:
             raise RuntimeError("Trajectory.backward needs a preceding forward")
         P = self.plan
@@ -87,35 +90,40 @@ class Trajectory:
                 P.step_vjp(self.stages[i], self.dt[s0 + i], lam)
             return lam
 
-        if self._stages_b is None:
-            self._stages_b = torch.empty_like(self.stages)
+        nbuf = min(self.RECORD_BUFFERS, len(segs))
+        while len(self._stages_more) < nbuf - 1:
+            self._stages_more.append(torch.empty_like(self.stages))
         if self._side is None:
             self._side = torch.cuda.Stream()
         main, side = torch.cuda.current_stream(), self._side
-        bufs = (self.stages, self._stages_b)
-        ready = (torch.cuda.Event(), torch.cuda.Event())
-        free = (torch.cuda.Event(), torch.cuda.Event())
+        bufs = [self.stages] + self._stages_more[:nbuf - 1]
+        ready = [torch.cuda.Event() for _ in range(nbuf)]
+        free = [torch.cuda.Event() for _ in range(nbuf)]
 
-        def recompute(seg, slot):
-            s0, m = span(seg)
+        def recompute(idx):
+            """Re-run segment segs[idx] into record buffer idx % nbuf, once the sweep that read that
+            buffer before (segment segs[idx - nbuf]) has finished."""
+            s0, m = span(segs[idx])
+            b = idx % nbuf
+            if idx >= nbuf:
+                side.wait_event(free[b])
             with torch.cuda.stream(side):
-                P.U.copy_(self.checkpoints[seg])
-                P.run_record(self.dt[s0:s0 + m], bufs[slot][:m])
-                ready[slot].record(side)
+                P.U.copy_(self.checkpoints[segs[idx]])
+                P.run_record(self.dt[s0:s0 + m], bufs[b][:m])
+                ready[b].record(side)
 
         side.wait_stream(main)          # checkpoints, schedule and state written on the caller's stream
-        recompute(segs[0], 0)
+        for idx in range(nbuf - 1):
+            recompute(idx)
         for k, seg in enumerate(segs):
-            slot = k % 2
-            if k + 1 < len(segs):
-                if k >= 1:
-                    side.wait_event(free[1 - slot])   # the sweep of segment k-1 read that buffer
-                recompute(segs[k + 1], 1 - slot)
-            main.wait_event(ready[slot])
+            if k + nbuf - 1 < len(segs):
+                recompute(k + nbuf - 1)
+            b = k % nbuf
+            main.wait_event(ready[b])
             s0, m = span(seg)
             for i in reversed(range(m)):
-                P.step_vjp(bufs[slot][i], self.dt[s0 + i], lam)
-            free[slot].record(main)
+                P.step_vjp(bufs[b][i], self.dt[s0 + i], lam)
+            free[b].record(main)
         main.wait_stream(side)          # the plan's state is the caller's again
         return lam
 
