@@ -100,8 +100,8 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
 
     if shapes is None:
         shapes = candidates()
-        if getattr(plan, "quadrature_points", 1) > 1:   # Gauss faces run the split stage only
-            shapes = [c for c in shapes if c["variant"] == 1]
+        if getattr(plan, "quadrature_points", 1) > 1:   # Gauss faces run the split stages only
+            shapes = [c for c in shapes if c["variant"] in (1, 5)]
     U_saved = plan.U.clone()
     _, step0, t0, _ = plan.status()
     trials = []

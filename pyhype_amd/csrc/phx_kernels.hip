@@ -725,36 +725,71 @@ __device__ void recon_x_item_q(const StageArgs& A, const WorkItem it) {
     }
 }
 
+// Gauss sum S_F = sum_i w_i F_i (from 0) of y-face f of column j
+template <bool CART, int FLUX>
+__device__ Q4 yface_sum_q(const StageArgs& A, const DevBlock& B, const double* sn, int f, int j, const Phys& ph) {
+    const int nx = B.nx, ny = B.ny;
+    const int nq = ph.nq;
+    // N states of row f-1 (f > 0) and S states of row f (f < ny)
+    const long below = f > 0 ? 8 * nq * ((long)(f - 1) * nx + j) : 0;
+    const long above = f < ny ? 8 * nq * ((long)f * nx + j) + 4 * nq : 0;
+    Q4 gS = phx::q4zero(), gN = phx::q4zero();
+    if (f == 0) gS = fetch_r<CART, true>(A.blocks, B, A.Uin, -1, j, ph);
+    if (f == ny) gN = fetch_r<CART, true>(A.blocks, B, A.Uin, ny, j, ph);
+    Q4 S = phx::q4zero();
+#pragma unroll
+    for (int i = 0; i < phx::QMAX; ++i) {
+        if (i >= nq) break;
+        Q4 L, R;
+        if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + above + 4 * i), gS, 0, j);
+        else L = ld4(sn + below + 4 * i);
+        if (f == ny) R = outer_state<CART>(B, PHX_N, L, gN, ny - 1, j);
+        else R = ld4(sn + above + 4 * i);
+        acc_w(S, phx::gauss_w(nq, i), yflux<CART, FLUX, true>(B, L, R, f, j, ph));
+    }
+    return S;
+}
+
 template <bool CART, int FLUX>
 __device__ void yface_item_q(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
-    const int nx = B.nx, ny = B.ny;
+    const int nx = B.nx;
     const int j = it.j0 + (int)(threadIdx.x % WAVE);
     if (j >= nx) return;
+    const double* sn = A.sn + 2 * A.ph.nq * B.off;
+    for (int f = it.r0; f < it.r1; ++f)
+        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), yface_sum_q<CART, FLUX>(A, B, sn, f, j, A.ph));
+}
+
+// Gauss faces, y-faces and update in one pass (variant 5; yface_update_item with the sums S_F)
+template <bool CART, int FLUX>
+__device__ void yface_update_item_q(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nx = B.nx;
+    const int j = it.j0 + lane;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
     const Phys ph = A.ph;
-    const int nq = ph.nq;
-    const double* sn = A.sn + 2 * nq * B.off;
-    for (int f = it.r0; f < it.r1; ++f) {
-        // N states of row f-1 (f > 0) and S states of row f (f < ny)
-        const long below = f > 0 ? 8 * nq * ((long)(f - 1) * nx + j) : 0;
-        const long above = f < ny ? 8 * nq * ((long)f * nx + j) + 4 * nq : 0;
-        Q4 gS = phx::q4zero(), gN = phx::q4zero();
-        if (f == 0) gS = fetch_r<CART, true>(A.blocks, B, A.Uin, -1, j, ph);
-        if (f == ny) gN = fetch_r<CART, true>(A.blocks, B, A.Uin, ny, j, ph);
-        Q4 S = phx::q4zero();
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    if (j < nx) {
+        const double* sn = A.sn + 2 * ph.nq * B.off;
+        Q4 FS = yface_sum_q<CART, FLUX>(A, B, sn, it.r0, j, ph);
+        for (int k = it.r0; k < it.r1; ++k) {
+            const Q4 FN = yface_sum_q<CART, FLUX>(A, B, sn, k + 1, j, ph);
+            const Q4 p = ld4(A.p1 + B.off + 4 * ((long)k * nx + j));
+            double LW, LE, LS, LN, Ac;
+            cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+            Q4 Res;
 #pragma unroll
-        for (int i = 0; i < phx::QMAX; ++i) {
-            if (i >= nq) break;
-            Q4 L, R;
-            if (f == 0) L = outer_state<CART>(B, PHX_S, ld4(sn + above + 4 * i), gS, 0, j);
-            else L = ld4(sn + below + 4 * i);
-            if (f == ny) R = outer_state<CART>(B, PHX_N, L, gN, ny - 1, j);
-            else R = ld4(sn + above + 4 * i);
-            acc_w(S, phx::gauss_w(nq, i), yflux<CART, FLUX, true>(B, L, R, f, j, ph));
+            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * ((p.v[c] + LS * FS.v[c]) - LN * FN.v[c])) / Ac;
+            emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+            FS = FN;
         }
-        st4(A.yf + B.yoff + 4 * ((long)f * nx + j), S);
     }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
 template <bool CART, int FLUX, bool GEN>
@@ -948,6 +983,13 @@ __global__ void __launch_bounds__(WAVE) k_yface_q(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) yface_item_q<CART, FLUX>(A, ld_const(A.items, i));
+}
+
+template <bool CART, int FLUX>
+__global__ void __launch_bounds__(WAVE) k_yface_update_q(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) yface_update_item_q<CART, FLUX>(A, ld_const(A.items, i));
 }
 
 template <bool CART>
@@ -1409,6 +1451,12 @@ struct k_yface_update_tag {
         hipLaunchKernelGGL((k_yface_update<C, F, G>), g, dim3(WAVE), 0, st, A);
     }
 };
+struct k_yface_update_q_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yface_update_q<C, F>), g, dim3(WAVE), 0, st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1448,14 +1496,25 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
     if (P->variant == 5) {
         A.sn = P->sn;
         A.p1 = P->p1;
+        const bool gauss = P->ph.nq > 1;
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty())
-                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                           P->waves_per_simd, st);
+            if (!P->items[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                                 P->grid[kind], 2, st);
+                else
+                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                               P->grid[kind], P->waves_per_simd, st);
+            }
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty())
-                launch_kind<k_yface_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                P->grid_b[kind], 2, st);
+            if (!P->items_b[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_yface_update_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                      P->grid_b[kind], 2, st);
+                else
+                    launch_kind<k_yface_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                    P->grid_b[kind], 2, st);
+            }
         return;
     }
     if (P->variant == 1) {
@@ -1918,8 +1977,8 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
     if (!P || variant < 0 || variant > 5 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
-    if (P->ph.nq > 1 && variant != 1)
-        return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stage only (variant 1)");
+    if (P->ph.nq > 1 && variant != 1 && variant != 5)
+        return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stages only (variants 1 and 5)");
     PHX_CK(hipDeviceSynchronize());
     if ((variant == 1 || variant == 5) && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);

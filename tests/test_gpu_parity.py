@@ -461,15 +461,16 @@ def test_primitive_reconstruction_matches_oracle(case, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
+@pytest.mark.parametrize("variant", [1, 5])
 @pytest.mark.parametrize("nq", [2, 3])
 @pytest.mark.parametrize("case,rows,rows_b,waves", [
     ("single", 2, 8, 3), ("multiblock", 1, 4, 3), ("oblique", 3, 5, 4), ("skewed", 2, 3, 3),
     ("wide_multiblock", 2, 16, 4)])
-def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq):
+def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq, variant):
     """fvm_num_quadrature_points = 2 and 3 (mesh/quadratures.py:33-95, fvm/base.py:168-190): one
-    flux per face and Gauss point, limiter over every point; the plan runs the split stage.
-    Residual and steps bit for bit, Cartesian and rotated-frame blocks, walls, outlets and
-    neighbours, interior strip edges."""
+    flux per face and Gauss point, limiter over every point; the plan runs the split stage (and the
+    split stage with fused y-faces and update).  Residual and steps bit for bit, Cartesian and
+    rotated-frame blocks, walls, outlets and neighbours, interior strip edges."""
     if case == "single":
         cfg, mesh = W.explosion_single(22)
     elif case == "multiblock":
@@ -483,7 +484,7 @@ def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq):
     cfg.fvm_num_quadrature_points = nq
     s = _solver(cfg, mesh, rows_per_item=rows)
     assert s.plan.launch_info()["variant"] == 1
-    s.plan.set_launch(rows, 0, waves, variant=1, rows_b=rows_b)
+    s.plan.set_launch(rows, 0, waves, variant=variant, rows_b=rows_b)
     st = _perturbed(s.states()[0], seed=21)
     s.set_state([st])
     R = s.plan.get_states(s.plan.residual())[0]
@@ -495,7 +496,8 @@ def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-def test_gauss_faces_with_primitive_reconstruction_and_van_leer():
+@pytest.mark.parametrize("variant", [1, 5])
+def test_gauss_faces_with_primitive_reconstruction_and_van_leer(variant):
     from pyhype_amd import PrimitiveState
 
     cfg, mesh = W.explosion_multiblock(12, 2, 2)
@@ -503,6 +505,7 @@ def test_gauss_faces_with_primitive_reconstruction_and_van_leer():
     cfg.fvm_slope_limiter_type = "VanLeer"
     cfg.reconstruction_type = PrimitiveState
     s = _solver(cfg, mesh, rows_per_item=2)
+    s.plan.set_launch(2, 0, 2, variant=variant, rows_b=3)
     st = _perturbed(s.states()[0], seed=23)
     s.set_state([st])
     R = s.plan.get_states(s.plan.residual())[0]
@@ -518,7 +521,7 @@ def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
     s = _solver(cfg, mesh, rows_per_item=2)
-    for v in (0, 2, 3, 4, 5):
+    for v in (0, 2, 3, 4):
         with pytest.raises(ValueError, match="Gauss point"):
             s.plan.set_launch(2, 0, 2, variant=v)
     s.run_steps(2)   # the plan still runs after the refusals
