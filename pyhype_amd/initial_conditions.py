@@ -2,7 +2,8 @@
 
 ``ExplosionInitialCondition`` is the reference example (examples/explosion/initial_condition.py:
 32-61).  ``SquareExplosionIC`` generalises its box, states and pressure ratio for batched cases;
-``ObliqueShockIC`` is the double-Mach-reflection-style state split by a line at 60 degrees.
+``ObliqueShockIC`` is the double-Mach-reflection-style state split by a line at 60 degrees;
+``DoubleMachReflectionIC`` is the reference's own double-Mach-reflection example.
 """
 from __future__ import annotations
 
@@ -74,6 +75,18 @@ class ObliqueShockIC(InitialCondition):
         pre = _cons(f, *self.pre)
         behind = block.mesh.x < self.x_shock + block.mesh.y / np.sqrt(3.0)
         block.state.data = np.where(behind, post.data, pre.data)
+        block.state.make_non_dimensional()
+
+
+class DoubleMachReflectionIC(InitialCondition):
+    """The reference's double-Mach-reflection example (examples/dmr/initial_condition.py:32-61):
+    primitive (8, 8.25, 0, 116.5) where x <= 0.95, (1.4, 0, 0, 1) elsewhere."""
+
+    def apply_to_block(self, block):
+        f = block.config.fluid
+        left = _cons(f, 8.0, 8.25, 0.0, 116.5)
+        right = _cons(f, 1.4, 0.0, 0.0, 1.0)
+        block.state.data = np.where(block.mesh.x <= 0.95, left.data, right.data)
         block.state.make_non_dimensional()
 
 

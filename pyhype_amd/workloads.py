@@ -25,10 +25,11 @@ from typing import List, Tuple
 import numpy as np
 
 from .fluids import Air
-from .initial_conditions import (DensityBumpIC, ExplosionInitialCondition, ObliqueShockIC, SmoothExplosionIC,
-                                 SquareExplosionIC)
-from .mesh import RectagularMeshGenerator
+from .initial_conditions import (DensityBumpIC, DoubleMachReflectionIC, ExplosionInitialCondition, ObliqueShockIC,
+                                 SmoothExplosionIC, SquareExplosionIC)
+from .mesh import QuadMeshGenerator, RectagularMeshGenerator
 from .solver_config import SolverConfig
+from .states import PrimitiveState
 
 T_FINAL_NONE = 1e30   # step-count runs: t_final never clips dt
 
@@ -70,6 +71,25 @@ def oblique_shock(n: int = 500, nblocks: int = 5):
     cfg = _config(n, n, ic=ObliqueShockIC(), flux="HLLL", integrator="SSPRK2", cfl=0.4,
                   fluid=Air(a_inf=1.0, rho_inf=1.0))
     return cfg, mesh
+
+
+def double_mach_reflection(n: int = 50):
+    """The reference's double-Mach-reflection example: 4 x 1 blocks of n x n cells over a 30 degree
+    ramp (examples/dmr/mesh.py:1-33: transfinite QuadMeshGenerator mesh, outlets, slip walls on the
+    ramp), HLLL, RK2, PrimitiveState reconstruction, Venkatakrishnan, CFL 0.4
+    (examples/dmr/config.py:8-34)."""
+    k = 1
+    a = 2 / np.sqrt(3)
+    d = np.tan(30 * np.pi / 180)
+    x = [0, k, 2 * k, 3 * k, 4 * k]
+    gen = QuadMeshGenerator(nx_blk=4, ny_blk=1, BCE=["OutletDirichlet"], BCW=["OutletDirichlet"],
+                            BCN=["OutletDirichlet"], BCS=["OutletDirichlet", "Slipwall", "Slipwall", "Slipwall"],
+                            top_x=x, bot_x=x, top_y=[a, a, a + d, a + 2 * d, a + 3 * d],
+                            bot_y=[0, 0, d, 2 * d, 3 * d], left_x=[0, 0], right_x=[4 * k, 4 * k],
+                            left_y=[0, a], right_y=[3 * d, a + 3 * d])
+    cfg = _config(n, n, ic=DoubleMachReflectionIC(), flux="HLLL", integrator="RK2", cfl=0.4)
+    cfg.reconstruction_type = PrimitiveState
+    return cfg, gen.dict
 
 
 def explosion_batch(seed: int = 0, ncases: int = 64, sizes=None,
@@ -120,4 +140,5 @@ WORKLOADS = {
     "oblique_shock": (oblique_shock, 500, 2),
     "explosion_batch": (explosion_batch, 100, 4),
     "explosion_adjoint": (explosion_adjoint, 100, 2),
+    "double_mach_reflection": (double_mach_reflection, 100, 2),
 }
