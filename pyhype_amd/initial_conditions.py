@@ -90,6 +90,33 @@ class DoubleMachReflectionIC(InitialCondition):
         block.state.make_non_dimensional()
 
 
+class ImplosionIC(InitialCondition):
+    """The reference's implosion example (examples/implosion/initial_condition.py): still air at the
+    low state (rho 1.1742, p 101100) where x <= 5 and y <= 5, the high state (rho 4.6968, p 404400)
+    elsewhere."""
+
+    def apply_to_block(self, block):
+        f = block.config.fluid
+        high = _cons(f, 4.6968, 0.0, 0.0, 404400.0)
+        low = _cons(f, 1.1742, 0.0, 0.0, 101100.0)
+        block.state.data = np.where(np.logical_and(block.mesh.x <= 5, block.mesh.y <= 5), low.data, high.data)
+        block.state.make_non_dimensional()
+
+
+class ShockBoxIC(InitialCondition):
+    """The reference's shock-box example (examples/shockbox/initial_condition.py): the low state in
+    the quadrants x <= 5, y <= 5 and x > 5, y > 5, the high state in the other two."""
+
+    def apply_to_block(self, block):
+        f = block.config.fluid
+        high = _cons(f, 4.6968, 0.0, 0.0, 404400.0)
+        low = _cons(f, 1.1742, 0.0, 0.0, 101100.0)
+        x, y = block.mesh.x, block.mesh.y
+        low_quadrants = np.logical_or(np.logical_and(x <= 5, y <= 5), np.logical_and(x > 5, y > 5))
+        block.state.data = np.where(low_quadrants, low.data, high.data)
+        block.state.make_non_dimensional()
+
+
 class SmoothExplosionIC(InitialCondition):
     """The explosion of ``ExplosionInitialCondition`` with differentiable data: the square's edges
     are tanh ramps of half-width ``width``; density and pressure carry a smooth background variation

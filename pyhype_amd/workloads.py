@@ -25,8 +25,8 @@ from typing import List, Tuple
 import numpy as np
 
 from .fluids import Air
-from .initial_conditions import (DensityBumpIC, DoubleMachReflectionIC, ExplosionInitialCondition, ObliqueShockIC,
-                                 SmoothExplosionIC, SquareExplosionIC)
+from .initial_conditions import (DensityBumpIC, DoubleMachReflectionIC, ExplosionInitialCondition, ImplosionIC,
+                                 ObliqueShockIC, ShockBoxIC, SmoothExplosionIC, SquareExplosionIC)
 from .mesh import QuadMeshGenerator, RectagularMeshGenerator
 from .solver_config import SolverConfig
 from .states import PrimitiveState
@@ -90,6 +90,18 @@ def double_mach_reflection(n: int = 50):
     cfg = _config(n, n, ic=DoubleMachReflectionIC(), flux="HLLL", integrator="RK2", cfl=0.4)
     cfg.reconstruction_type = PrimitiveState
     return cfg, gen.dict
+
+
+def implosion(n: int = 40):
+    """The reference's implosion example: one 10 x 10 block with reflecting walls, Roe, RK2, CFL 0.4
+    (examples/implosion/config.py, mesh.py)."""
+    return _config(n, n, ic=ImplosionIC(), integrator="RK2", cfl=0.4), single_block_mesh()
+
+
+def shockbox(n: int = 50):
+    """The reference's shock-box example: one 10 x 10 block with reflecting walls, Roe, RK2, CFL 0.4
+    (examples/shockbox/config.py, mesh.py)."""
+    return _config(n, n, ic=ShockBoxIC(), integrator="RK2", cfl=0.4), single_block_mesh()
 
 
 def explosion_batch(seed: int = 0, ncases: int = 64, sizes=None,

@@ -218,6 +218,19 @@ def test_double_mach_reflection_example_matches_oracle(variant):
     assert s.t[0] == sim.t
 
 
+@pytest.mark.parametrize("example", ["implosion", "shockbox"])
+def test_reference_examples_match_oracle(example):
+    """The reference's implosion and shock-box examples (one walled block, Roe, RK2, CFL 0.4) at
+    24 x 24: 20 steps bit for bit, time included."""
+    cfg, mesh = getattr(W, example)(24)
+    s = _solver(cfg, mesh)
+    sim = oracle_for(cfg, mesh, s.states()[0])
+    s.run_steps(20)
+    sim.run(20)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+    assert s.t[0] == sim.t
+
+
 def test_skewed_multiblock_residual_and_steps_match_oracle():
     cfg, _ = W.explosion_single(18)
     mesh = skewed_mesh()
