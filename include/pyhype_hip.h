@@ -211,7 +211,9 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * per face for the fluxes, one lane per cell for the update (no shape parameters); 4 = paired: the
  * fused kernel's work items with two waves each, one per pair of variables, the x- and y-face
  * fluxes split between them; 5 = split with the y-face and update kernels fused (each lane walks
- * rows_b cell rows of its column, evaluating the face below the first row again).  All variants
+ * rows_b cell rows of its column, evaluating the face below the first row again); 6 = the same with
+ * one face per wave (workgroups of rows_b + 1 waves for rows_b <= 7 cell rows, the faces exchanged
+ * through LDS).  All variants
  * give identical results.  Rebuilds the work lists; the split variants allocate about 8*nq + 8
  * doubles per cell, the element-wise variant 16 per cell and 4 per face.  A plan with more than
  * one Gauss point per face runs the split variants only (1 and 5; every other variant returns

@@ -8,7 +8,8 @@ halo (``rows_per_item`` rows per item) and a y-face kernel (``rows_b`` rows per 
 Element-wise variant: one lane per cell and variable for the reconstruction, one per face for
 the fluxes and one per cell for the update (no shape parameters).  Paired variant: the fused
 kernel's items with two waves each, one per pair of variables (``rows_per_item`` rows).  Variant
-5: the split stage with its y-face and update kernels fused (``rows_b`` cell rows per lane).
+5: the split stage with its y-face and update kernels fused (``rows_b`` cell rows per lane);
+variant 6: the same with one face per wave (``rows_b`` rows and ``rows_b`` + 1 waves per workgroup).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
 flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
@@ -84,6 +85,8 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
             for w in (2, 3) for r in paired_rows]
     out += [{"variant": 5, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in (1, 2) for rb in (1, 2, 3, 4, 8)]
+    out += [{"variant": 6, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
+            for w in split_waves for r in (1, 2) for rb in (1, 2, 3, 4, 7)]
     return out
 
 

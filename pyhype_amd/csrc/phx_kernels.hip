@@ -948,6 +948,54 @@ __device__ void yface_update_item(const StageArgs& A, const WorkItem it) {
     reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
+// y-faces and update with one face per wave (variant 6): a workgroup of R+1 waves takes R cell rows
+// [r0, r1) of a 64-column strip; wave w evaluates face r0+w into LDS, and after one barrier wave w < R
+// completes cell row r0+w from faces r0+w and r0+w+1 as update_item does.  Every wave's chain is one
+// flux and at most one update, and a face is evaluated twice only at item boundaries.
+constexpr int YU_MAX_ROWS = 7;  // rows per item (R + 1 <= 8 waves per workgroup)
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* lds) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const int nx = B.nx;
+    const int j = it.j0 + lane;
+    const int nrow = it.r1 - it.r0;
+    const Phys ph = A.ph;
+    if (w <= nrow && j < nx)
+        lds[w * WAVE + lane] = yface_flux_at<CART, FLUX, GEN>(A, B, A.sn + 2 * B.off, it.r0 + w, j, ph);
+    __syncthreads();
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    if (w < nrow && j < nx) {
+        const int k = it.r0 + w;
+        const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
+        const Q4 FS = lds[w * WAVE + lane], FN = lds[(w + 1) * WAVE + lane];
+        const Q4 p = ld4(A.p1 + B.off + 4 * ((long)k * nx + j));
+        double LW, LE, LS, LN, Ac;
+        cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+        Q4 Res;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+        emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+template <bool CART, int FLUX, bool GEN>
+__global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(StageArgs A) {
+    __shared__ Q4 lds[(YU_MAX_ROWS + 1) * WAVE];
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) {
+        yface_update_wg_item<CART, FLUX, GEN>(A, ld_const(A.items, i), lds);
+        __syncthreads();  // the next item reuses the face buffer
+    }
+}
+
 template <bool CART, int FLUX, bool GEN>
 __global__ void __launch_bounds__(WAVE) k_yface_update(StageArgs A) {
     int lo, hi, first, stride;
@@ -1313,7 +1361,7 @@ int build_items(phx_plan* P) {
         }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
-    const bool split = P->variant == 1 || P->variant == 5;
+    const bool split = P->variant == 1 || P->variant == 5 || P->variant == 6;
     const int strip = split ? STRIP_A : STRIP;
     const int item_rows = P->variant == 2 ? WGW * rows : rows;  // shared-halo items: WGW waves of `rows`
     for (int b = 0; b < P->nblocks; ++b) {
@@ -1328,10 +1376,11 @@ int build_items(phx_plan* P) {
             for (int e = 0; e < nf; e += EW_WG) P->items_e[1][kind].push_back({b, e, std::min(EW_WG, nf - e), 0});
             for (int e = 0; e < nc; e += EW_WG) P->items_e[2][kind].push_back({b, e, std::min(EW_WG, nc - e), 0});
         }
-        if (P->variant == 5) {
-            for (int r0 = 0; r0 < d.ny; r0 += P->rows_b)
+        if (P->variant == 5 || P->variant == 6) {
+            const int rb = P->variant == 6 ? std::min(std::max(P->rows_b, 1), YU_MAX_ROWS) : P->rows_b;
+            for (int r0 = 0; r0 < d.ny; r0 += rb)
                 for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
-                    P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
+                    P->items_b[kind].push_back({b, j0, r0, std::min(r0 + rb, d.ny)});
         }
         if (P->variant == 1) {
             for (int r0 = 0; r0 <= d.ny; r0 += rows)
@@ -1457,6 +1506,12 @@ struct k_yface_update_q_tag {
         hipLaunchKernelGGL((k_yface_update_q<C, F>), g, dim3(WAVE), 0, st, A);
     }
 };
+struct k_yface_update_wg_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yface_update_wg<C, F, G>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1493,6 +1548,20 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
 }
 
 void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
+    if (P->variant == 6) {
+        A.sn = P->sn;
+        A.p1 = P->p1;
+        A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items[kind].empty())
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                           P->waves_per_simd, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_b[kind].empty())
+                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                   P->grid_b[kind], 2, st);
+        return;
+    }
     if (P->variant == 5) {
         A.sn = P->sn;
         A.p1 = P->p1;
@@ -1591,7 +1660,8 @@ void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
-        DtArgs D{P->dblocks, P->ditems[kind], n, (P->variant == 1 || P->variant == 5) ? STRIP_A : STRIP, U, P->dtmin,
+        DtArgs D{P->dblocks, P->ditems[kind], n, (P->variant == 1 || P->variant == 5 || P->variant == 6) ? STRIP_A : STRIP,
+                 U, P->dtmin,
                  P->bad, P->step, P->nslot,
                  P->ph};
         if (kind == 1) hipLaunchKernelGGL(k_dt<true>, dim3(P->grid[kind]), dim3(WAVE), 0, st, D);
@@ -1976,11 +2046,11 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 5 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 6 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     if (P->ph.nq > 1 && variant != 1 && variant != 5)
         return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stages only (variants 1 and 5)");
     PHX_CK(hipDeviceSynchronize());
-    if ((variant == 1 || variant == 5) && !P->sn) {
+    if ((variant == 1 || variant == 5 || variant == 6) && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
         PHX_CK(hipMalloc(&P->sn, 2 * (size_t)std::max(P->ph.nq, 1) * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));
