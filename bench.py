@@ -256,7 +256,9 @@ def main():
               3: "k_ew_recon + k_ew_face + k_ew_update (element-wise stage: one RK stage over all blocks)",
               4: "k_stage_pair (paired stage: one RK stage over all blocks)",
               5: "k_recon_x + k_yface_update (split stage with fused y-faces and update: one RK stage over all "
-                 "blocks)"}.get(
+                 "blocks)",
+              6: "k_recon_x + k_yface_update_wg (split stage, y-faces and update with one face per wave: one RK "
+                 "stage over all blocks)"}.get(
                   variant, "k_stage (one RK stage over all blocks)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
