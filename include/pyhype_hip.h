@@ -120,7 +120,7 @@ typedef struct {
     int32_t grid_waves;       /* waves of the persistent grid; 0 = one per work item */
     int32_t reconstruction;   /* PHX_RECON_* (SolverConfig.reconstruction_type) */
     int32_t quadrature_points; /* Gauss points per face, 1..PHX_MAX_QUADRATURE_POINTS; > 1 runs
-                                  the split stages (phx_set_variant 1, the default, or 5) */
+                                  the split stages (phx_set_variant 1, 5 or 6, the default) */
 } phx_plan_desc;
 
 typedef struct phx_plan phx_plan;
@@ -216,7 +216,7 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * through LDS).  All variants
  * give identical results.  Rebuilds the work lists; the split variants allocate about 8*nq + 8
  * doubles per cell, the element-wise variant 16 per cell and 4 per face.  A plan with more than
- * one Gauss point per face runs the split variants only (1 and 5; every other variant returns
+ * one Gauss point per face runs the split variants only (1, 5 and 6; every other variant returns
  * PHX_E_INVALID).  No counterpart in the reference. */
 int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 

@@ -104,7 +104,7 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
     if shapes is None:
         shapes = candidates()
         if getattr(plan, "quadrature_points", 1) > 1:   # Gauss faces run the split stages only
-            shapes = [c for c in shapes if c["variant"] in (1, 5)]
+            shapes = [c for c in shapes if c["variant"] in (1, 5, 6)]
     U_saved = plan.U.clone()
     _, step0, t0, _ = plan.status()
     trials = []

@@ -208,8 +208,8 @@ class DevicePlan:
         self.U = torch.zeros(self.state_len, dtype=torch.float64, device="cuda")
         self.nsteps_done = 0
         self.launch = {"rows_per_item": rows_per_item or 32, "grid_waves": grid_waves, "waves_per_simd": 2}
-        if self.quadrature_points > 1:   # the plan runs the split stage (8 rows per update item)
-            self.launch.update({"variant": 1, "rows_b": 8})
+        if self.quadrature_points > 1:   # the plan runs a split stage (one y-face per wave, 3 rows per item)
+            self.launch.update({"variant": 6, "rows_b": 3})
 
     # ------------------------------------------------------------------ lifetime
     def close(self):

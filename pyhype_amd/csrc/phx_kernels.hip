@@ -954,7 +954,8 @@ __device__ void yface_update_item(const StageArgs& A, const WorkItem it) {
 // flux and at most one update, and a face is evaluated twice only at item boundaries.
 constexpr int YU_MAX_ROWS = 7;  // rows per item (R + 1 <= 8 waves per workgroup)
 
-template <bool CART, int FLUX, bool GEN>
+// Q = true: Gauss faces (the sums S_F of yface_sum_q, the residual of the nq-point rule)
+template <bool CART, int FLUX, bool GEN, bool Q>
 __device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* lds) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
@@ -964,8 +965,10 @@ __device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* 
     const int j = it.j0 + lane;
     const int nrow = it.r1 - it.r0;
     const Phys ph = A.ph;
-    if (w <= nrow && j < nx)
-        lds[w * WAVE + lane] = yface_flux_at<CART, FLUX, GEN>(A, B, A.sn + 2 * B.off, it.r0 + w, j, ph);
+    if (w <= nrow && j < nx) {
+        if (Q) lds[w * WAVE + lane] = yface_sum_q<CART, FLUX>(A, B, A.sn + 2 * ph.nq * B.off, it.r0 + w, j, ph);
+        else lds[w * WAVE + lane] = yface_flux_at<CART, FLUX, GEN>(A, B, A.sn + 2 * B.off, it.r0 + w, j, ph);
+    }
     __syncthreads();
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
     bool lane_bad = false;
@@ -977,21 +980,26 @@ __device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* 
         double LW, LE, LS, LN, Ac;
         cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
         Q4 Res;
+        if (Q) {  // the face entries are the Gauss sums already
 #pragma unroll
-        for (int c = 0; c < 4; ++c)
-            Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+            for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * ((p.v[c] + LS * FS.v[c]) - LN * FN.v[c])) / Ac;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                Res.v[c] = (0.5 * ((p.v[c] + LS * (2.0 * FS.v[c])) - LN * (2.0 * FN.v[c]))) / Ac;
+        }
         emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
     }
     reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
-template <bool CART, int FLUX, bool GEN>
+template <bool CART, int FLUX, bool GEN, bool Q>
 __global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(StageArgs A) {
     __shared__ Q4 lds[(YU_MAX_ROWS + 1) * WAVE];
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) {
-        yface_update_wg_item<CART, FLUX, GEN>(A, ld_const(A.items, i), lds);
+        yface_update_wg_item<CART, FLUX, GEN, Q>(A, ld_const(A.items, i), lds);
         __syncthreads();  // the next item reuses the face buffer
     }
 }
@@ -1509,7 +1517,13 @@ struct k_yface_update_q_tag {
 struct k_yface_update_wg_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_yface_update_wg<C, F, G>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_yface_update_wg<C, F, G, false>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
+    }
+};
+struct k_yface_update_wg_q_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        hipLaunchKernelGGL((k_yface_update_wg<C, F, true, true>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
     }
 };
 struct k_update_tag {
@@ -1552,14 +1566,25 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
         A.sn = P->sn;
         A.p1 = P->p1;
         A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
+        const bool gauss = P->ph.nq > 1;
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty())
-                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                           P->waves_per_simd, st);
+            if (!P->items[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                                 P->grid[kind], 2, st);
+                else
+                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                               P->grid[kind], P->waves_per_simd, st);
+            }
         for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty())
-                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                   P->grid_b[kind], 2, st);
+            if (!P->items_b[kind].empty()) {
+                if (gauss)
+                    launch_kind<k_yface_update_wg_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                         P->grid_b[kind], 2, st);
+                else
+                    launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                       P->grid_b[kind], 2, st);
+            }
         return;
     }
     if (P->variant == 5) {
@@ -1861,9 +1886,9 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     if (rc) return bail(rc);
     rc = build_items(P);
     if (rc) return bail(rc);
-    if (nq > 1) {  // Gauss faces run the split stage
-        P->rows_b = 8;
-        rc = phx_set_variant(P, 1, 0);
+    if (nq > 1) {  // Gauss faces run a split stage: one y-face per wave, 3 cell rows per workgroup
+        P->rows_b = 3;
+        rc = phx_set_variant(P, 6, 0);
         if (rc) return bail(rc);
     }
 
@@ -2047,8 +2072,8 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
     if (!P || variant < 0 || variant > 6 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
-    if (P->ph.nq > 1 && variant != 1 && variant != 5)
-        return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stages only (variants 1 and 5)");
+    if (P->ph.nq > 1 && variant != 1 && variant != 5 && variant != 6)
+        return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stages only (variants 1, 5, 6)");
     PHX_CK(hipDeviceSynchronize());
     if ((variant == 1 || variant == 5 || variant == 6) && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);

@@ -496,7 +496,7 @@ def test_primitive_reconstruction_matches_oracle(case, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-@pytest.mark.parametrize("variant", [1, 5])
+@pytest.mark.parametrize("variant", [1, 5, 6])
 @pytest.mark.parametrize("nq", [2, 3])
 @pytest.mark.parametrize("case,rows,rows_b,waves", [
     ("single", 2, 8, 3), ("multiblock", 1, 4, 3), ("oblique", 3, 5, 4), ("skewed", 2, 3, 3),
@@ -518,7 +518,7 @@ def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq, variant):
         cfg, mesh = W.explosion_single(16)[0], skewed_mesh()
     cfg.fvm_num_quadrature_points = nq
     s = _solver(cfg, mesh, rows_per_item=rows)
-    assert s.plan.launch_info()["variant"] == 1
+    assert s.plan.launch_info()["variant"] == 6   # the default of Gauss-face plans
     s.plan.set_launch(rows, 0, waves, variant=variant, rows_b=rows_b)
     st = _perturbed(s.states()[0], seed=21)
     s.set_state([st])
@@ -531,7 +531,7 @@ def test_gauss_faces_match_oracle(case, rows, rows_b, waves, nq, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-@pytest.mark.parametrize("variant", [1, 5])
+@pytest.mark.parametrize("variant", [1, 5, 6])
 def test_gauss_faces_with_primitive_reconstruction_and_van_leer(variant):
     from pyhype_amd import PrimitiveState
 
@@ -556,7 +556,7 @@ def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
     s = _solver(cfg, mesh, rows_per_item=2)
-    for v in (0, 2, 3, 4, 6):
+    for v in (0, 2, 3, 4):
         with pytest.raises(ValueError, match="Gauss point"):
             s.plan.set_launch(2, 0, 2, variant=v)
     s.run_steps(2)   # the plan still runs after the refusals
