@@ -258,7 +258,11 @@ def main():
               5: "k_recon_x + k_yface_update (split stage with fused y-faces and update: one RK stage over all "
                  "blocks)",
               6: "k_recon_x + k_yface_update_wg (split stage, y-faces and update with one face per wave: one RK "
-                 "stage over all blocks)"}.get(
+                 "stage over all blocks)",
+              7: "k_recon_x_lin + k_yface_update_wg (split stage, row-major reconstruction, one y-face per wave: "
+                 "one RK stage over all blocks)",
+              8: "k_recon_x_lin + k_yface_update_flat (split stage, row-major reconstruction, y-faces and update "
+                 "over flat cell ranges: one RK stage over all blocks)"}.get(
                   variant, "k_stage (one RK stage over all blocks)")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,

@@ -213,7 +213,12 @@ int phx_set_launch(phx_plan* plan, int32_t rows_per_item, int32_t grid_waves, in
  * fluxes split between them; 5 = split with the y-face and update kernels fused (each lane walks
  * rows_b cell rows of its column, evaluating the face below the first row again); 6 = the same with
  * one face per wave (workgroups of rows_b + 1 waves for rows_b <= 7 cell rows, the faces exchanged
- * through LDS).  All variants
+ * through LDS); 7 = variant 6 with a reconstruction kernel over the row-major positions of each
+ * block (62 owned cells or boundary faces per wave, so rows need not fill whole column strips;
+ * one gradient-limiter point per face only); 8 = that reconstruction with a y-face and update
+ * kernel over flat cell ranges of each block (workgroups of rows_b + 1 waves evaluate the
+ * n + nx y-faces of n consecutive cells into LDS, at least rows_per_item faces per lane; blocks
+ * must satisfy nx + 64 <= 2048 faces per range).  All variants
  * give identical results.  Rebuilds the work lists; the split variants allocate about 8*nq + 8
  * doubles per cell, the element-wise variant 16 per cell and 4 per face.  A plan with more than
  * one Gauss point per face runs the split variants only (1, 5 and 6; every other variant returns

@@ -9,7 +9,10 @@ Element-wise variant: one lane per cell and variable for the reconstruction, one
 the fluxes and one per cell for the update (no shape parameters).  Paired variant: the fused
 kernel's items with two waves each, one per pair of variables (``rows_per_item`` rows).  Variant
 5: the split stage with its y-face and update kernels fused (``rows_b`` cell rows per lane);
-variant 6: the same with one face per wave (``rows_b`` rows and ``rows_b`` + 1 waves per workgroup).
+variant 6: the same with one face per wave (``rows_b`` rows and ``rows_b`` + 1 waves per workgroup);
+variant 7: variant 6 with the reconstruction over row-major positions (62 per wave, no shape
+parameter of its own); variant 8: that reconstruction with the y-faces and update over flat cell
+ranges (``rows_b`` + 1 waves per workgroup, at least ``rows_per_item`` faces per lane).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
 fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
 flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
@@ -87,6 +90,10 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
             for w in split_waves for r in (1, 2) for rb in (1, 2, 3, 4, 8)]
     out += [{"variant": 6, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in (1, 2) for rb in (1, 2, 3, 4, 7)]
+    out += [{"variant": 7, "rows_per_item": 1, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
+            for w in split_waves for rb in (1, 2, 3, 4, 7)]
+    out += [{"variant": 8, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
+            for w in split_waves for r in (1, 2) for rb in (7, 15)]
     return out
 
 

@@ -362,7 +362,8 @@ class DevicePlan:
         """Launch shape of the stage: rows per work item, persistent grid (0 = one wave per item),
         register budget in waves per SIMD; optionally the stage variant (0 fused, 1 split, 2 shared
         halo, 3 element-wise, 4 paired, 5 split with fused y-face/update, 6 the same with one face per
-        wave) and the rows per item of the split variants' update kernel."""
+        wave, 7 variant 6 with the row-major reconstruction kernel, 8 that reconstruction with the
+        y-faces and update over flat cell ranges) and the rows per item of the split variants' update kernel."""
         if variant is not None:
             nat.check(self.lib.phx_set_variant(self.handle, int(variant), int(rows_b)), "phx_set_variant")
         nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves), int(waves_per_simd)),

@@ -77,6 +77,7 @@ struct StageArgs {
     double* rb;        // element-wise stage: limited face states E, W, N, S per cell (16 doubles)
     double* ff;        // element-wise stage: face fluxes (4 doubles per face, DevBlock::foff)
     int nslot;         // partial CFL minima per case
+    int fq;            // flat y-face stage: LDS faces per lane
     Phys ph;
 };
 
@@ -1004,6 +1005,63 @@ __global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(St
     }
 }
 
+// y-faces and update over flat cell ranges (variant 8): a workgroup of T lanes takes the cells
+// c in [c0, c0 + n) of one block in row-major order.  The S face of cell c has the face index c and
+// its N face c + nx (face (f, j) at f*nx + j, f = 0..ny), so the faces of the range are the n + nx
+// consecutive indices from c0: lane t evaluates faces c0 + t, c0 + t + T, ... into LDS and, after
+// one barrier, completes cells c0 + t, c0 + t + T, ... as update_item does.  No lane idles on a
+// partial column strip, and a face is evaluated twice only where two ranges meet (nx faces per
+// range).  T = (A.rows_wave + 1) * WAVE; the host sizes the ranges so that n + nx <= A.fq * T, the
+// faces the dynamic LDS holds.
+template <bool CART, int FLUX, bool GEN>
+__device__ void yface_update_flat_item(const StageArgs& A, const WorkItem it, Q4* lds) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
+    const int T = blockDim.x;
+    const int t = threadIdx.x;
+    const int lane = t & (WAVE - 1);
+    const int nx = B.nx;
+    const int c0 = it.j0, n = it.r0;
+    const Phys ph = A.ph;
+    const double* sn = A.sn + 2 * B.off;
+    for (int q = t; q < n + nx; q += T) {
+        const int fi = c0 + q;
+        const int f = fi / nx;
+        lds[q] = yface_flux_at<CART, FLUX, GEN>(A, B, sn, f, fi - f * nx, ph);
+    }
+    __syncthreads();
+    double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
+    bool lane_bad = false;
+    const double dt = (A.dt != nullptr) ? ld_const(A.dt, B.case_id) : 0.0;
+    for (int q = t; q < n; q += T) {
+        const int c = c0 + q;
+        const int k = c / nx, j = c - k * nx;
+        const Q4 FS = lds[q], FN = lds[q + nx];
+        const Q4 p = ld4(A.p1 + B.off + 4 * (long)c);
+        double LW, LE, LS, LN, Ac;
+        cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+        Q4 Res;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+            Res.v[cc] = (0.5 * ((p.v[cc] + LS * (2.0 * FS.v[cc])) - LN * (2.0 * FN.v[cc]))) / Ac;
+        emit_row<CART>(A, B, k, j, dt, Res, lane_dt, lane_bad);
+    }
+    reduce_item(A, B, lane, lane_dt, lane_bad);
+}
+
+constexpr int FLAT_MAX_WAVES = 16;  // workgroup of up to 1024 lanes
+
+template <bool CART, int FLUX, bool GEN>
+__global__ void __launch_bounds__(FLAT_MAX_WAVES * WAVE) k_yface_update_flat(StageArgs A) {
+    extern __shared__ Q4 flat_lds[];
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) {
+        yface_update_flat_item<CART, FLUX, GEN>(A, ld_const(A.items, i), flat_lds);
+        __syncthreads();  // the next item reuses the face buffer
+    }
+}
+
 template <bool CART, int FLUX, bool GEN>
 __global__ void __launch_bounds__(WAVE) k_yface_update(StageArgs A) {
     int lo, hi, first, stride;
@@ -1016,6 +1074,79 @@ __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
+}
+
+// k_recon_x_lin: the reconstruction kernel of the split stage over the row-major positions of a
+// block, p = k*(nx+1) + j with j in [0, nx]: j < nx is cell (k, j) and its west face, j = nx the
+// east boundary face of row k.  Lane l of an item takes position j0 - 1 + l; lane 0 only
+// reconstructs (its limited east state feeds the west face of lane 1), lane WAVE-1 only evaluates
+// its face (the east face flux of lane WAVE-2), lanes 1..WAVE-2 own their positions.  East and west
+// neighbour values come from the adjacent lanes inside a row and from memory at row ends and at the
+// wave's two edge lanes.  Rows that are not a multiple of a column strip leave no idle lanes, and
+// each wave needs one halo lane per side instead of two.  Same arithmetic and association order as
+// recon_x_item.
+constexpr int LIN_OWN = WAVE - 2;
+
+template <bool CART, int FLUX, bool GEN>
+__device__ void recon_x_lin_item(const StageArgs& A, const WorkItem it) {
+    const DevBlock& B = A.blocks[it.block];
+    if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
+    const int nx = B.nx, npos = B.ny * (nx + 1);
+    const int lane = threadIdx.x % WAVE;
+    const int p = it.j0 - 1 + lane;
+    const bool valid = (p >= 0) && (p < npos);
+    const int k = valid ? p / (nx + 1) : 0;
+    const int j = valid ? p - k * (nx + 1) : 0;
+    const bool cell = valid && (j < nx);
+    const Phys ph = A.ph;
+
+    // every load is issued before the first shuffle, so that a wave waits for memory once
+    const bool wmem = cell && (j == 0 || lane == 0);
+    const bool emem = cell && (lane == WAVE - 1);
+    Q4 Uc = phx::q4zero(), Um = phx::q4zero(), Up = phx::q4zero(), Uw = phx::q4zero(), Ue = phx::q4zero();
+    if (valid) Uc = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k, j, ph);  // j == nx: the east ghost value
+    if (cell) {
+        Um = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k - 1, j, ph);
+        Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
+    }
+    if (wmem) Uw = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k, j - 1, ph);
+    if (emem) Ue = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k, j + 1, ph);
+    const Q4 Uwl = shfl_up1(Uc);
+    const Q4 Uel = shfl_down1(Uc);
+    if (!wmem) Uw = Uwl;
+    if (!emem) Ue = Uel;
+    phx::Recon r;
+    r.E = r.W = r.N = r.S = phx::q4zero();
+    if (cell) {
+        const phx::CellGeo g = cell_geo<CART>(B, k, j);
+        r = phx::reconstruct<GEN>(Uc, Uw, Ue, Um, Up, g, ph.limiter);
+    }
+    const Q4 QEl = shfl_up1(r.E);  // limited east state of position p-1 (cell j-1 of the same row for j > 0)
+    Q4 FW = phx::q4zero();
+    if (valid && lane > 0) {
+        const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, Uw, k, j) : QEl;
+        const Q4 R = (j == nx) ? outer_state<CART>(B, PHX_E, QEl, Uc, k, j) : r.W;
+        FW = xflux<CART, FLUX, GEN>(B, L, R, k, j, ph);
+    }
+    const Q4 FE = shfl_down1(FW);
+    if (cell && lane > 0 && lane < WAVE - 1) {
+        double LW, LE, LS, LN, Ac;
+        cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+        Q4 part;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) part.v[c] = LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c]);
+        const long cidx = (long)k * nx + j;
+        st4(A.p1 + B.off + 4 * cidx, part);
+        st4(A.sn + 2 * B.off + 8 * cidx, r.N);
+        st4(A.sn + 2 * B.off + 8 * cidx + 4, r.S);
+    }
+}
+
+template <bool CART, int FLUX, int MINW, bool GEN>
+__global__ void __launch_bounds__(WAVE, MINW) k_recon_x_lin(StageArgs A) {
+    int lo, hi, first, stride;
+    item_range(A.nitems, lo, hi, first, stride);
+    for (int i = first; i < hi; i += stride) recon_x_lin_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
 }
 
 template <bool CART, int FLUX, bool GEN>
@@ -1355,6 +1486,11 @@ int upload_items(std::vector<WorkItem> (&items)[2], WorkItem* (&ditems)[2], int 
 // for the update kernel.
 void release_graphs(phx_plan* P);
 
+// flat y-face stage: lanes per workgroup (rows_b + 1 waves, as the one-face-per-wave stage) and the
+// faces its dynamic LDS may hold (64 KiB)
+constexpr int FLAT_MAX_FACES = 2048;
+int flat_threads(const phx_plan* P) { return (std::min(std::max(P->rows_b, 1), FLAT_MAX_WAVES - 1) + 1) * WAVE; }
+
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
@@ -1369,7 +1505,8 @@ int build_items(phx_plan* P) {
         }
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
-    const bool split = P->variant == 1 || P->variant == 5 || P->variant == 6;
+    const bool split = P->variant == 1 || P->variant >= 5;
+    P->flat_q[0] = P->flat_q[1] = 0;
     const int strip = split ? STRIP_A : STRIP;
     const int item_rows = P->variant == 2 ? WGW * rows : rows;  // shared-halo items: WGW waves of `rows`
     for (int b = 0; b < P->nblocks; ++b) {
@@ -1384,8 +1521,20 @@ int build_items(phx_plan* P) {
             for (int e = 0; e < nf; e += EW_WG) P->items_e[1][kind].push_back({b, e, std::min(EW_WG, nf - e), 0});
             for (int e = 0; e < nc; e += EW_WG) P->items_e[2][kind].push_back({b, e, std::min(EW_WG, nc - e), 0});
         }
-        if (P->variant == 5 || P->variant == 6) {
-            const int rb = P->variant == 6 ? std::min(std::max(P->rows_b, 1), YU_MAX_ROWS) : P->rows_b;
+        if (P->variant == 7 || P->variant == 8)  // row-major reconstruction items: LIN_OWN positions each
+            for (int p0 = 0; p0 < d.ny * (d.nx + 1); p0 += LIN_OWN) P->items_f[kind].push_back({b, p0, 0, 0});
+        if (P->variant == 8) {
+            // flat y-face ranges: workgroups of T lanes, q >= rows_per_item faces per lane and at
+            // least 64 cells per range (n = q*T - nx)
+            const int T = flat_threads(P), nc = d.nx * d.ny;
+            const int qmin = (d.nx + 64 + T - 1) / T;  // phx_set_variant checked qmin * T <= FLAT_MAX_FACES
+            const int q = std::min(std::max(P->rows_per_item, qmin), FLAT_MAX_FACES / T);
+            P->flat_q[kind] = std::max(P->flat_q[kind], q);
+            const int n = q * T - d.nx;
+            for (int c0 = 0; c0 < nc; c0 += n) P->items_b[kind].push_back({b, c0, std::min(n, nc - c0), 0});
+        }
+        if (P->variant == 5 || P->variant == 6 || P->variant == 7) {
+            const int rb = P->variant != 5 ? std::min(std::max(P->rows_b, 1), YU_MAX_ROWS) : P->rows_b;
             for (int r0 = 0; r0 < d.ny; r0 += rb)
                 for (int j0 = 0; j0 < d.nx; j0 += STRIP_B)
                     P->items_b[kind].push_back({b, j0, r0, std::min(r0 + rb, d.ny)});
@@ -1466,6 +1615,13 @@ struct k_recon_x_tag {
         hipLaunchKernelGGL((k_recon_x<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
     }
 };
+struct k_recon_x_lin_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        constexpr int MM = G ? 3 : (M < 3 ? 3 : M);
+        hipLaunchKernelGGL((k_recon_x_lin<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
+    }
+};
 struct k_yface_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1526,6 +1682,13 @@ struct k_yface_update_wg_q_tag {
         hipLaunchKernelGGL((k_yface_update_wg<C, F, true, true>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
     }
 };
+struct k_yface_update_flat_tag {
+    template <bool C, int F, int M, bool G>
+    static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
+        const int T = (A.rows_wave + 1) * WAVE;
+        hipLaunchKernelGGL((k_yface_update_flat<C, F, G>), g, dim3(T), (size_t)A.fq * T * sizeof(Q4), st, A);
+    }
+};
 struct k_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
@@ -1562,6 +1725,36 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
 }
 
 void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
+    if (P->variant == 8) {
+        A.sn = P->sn;
+        A.p1 = P->p1;
+        A.rows_wave = flat_threads(P) / WAVE - 1;
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_f[kind].empty())
+                launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                               P->grid_f[kind], P->waves_per_simd, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_b[kind].empty()) {
+                A.fq = P->flat_q[kind];
+                launch_kind<k_yface_update_flat_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                     P->grid_b[kind], 2, st);
+            }
+        return;
+    }
+    if (P->variant == 7) {
+        A.sn = P->sn;
+        A.p1 = P->p1;
+        A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_f[kind].empty())
+                launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                               P->grid_f[kind], P->waves_per_simd, st);
+        for (int kind = 0; kind < 2; ++kind)
+            if (!P->items_b[kind].empty())
+                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                   P->grid_b[kind], 2, st);
+        return;
+    }
     if (P->variant == 6) {
         A.sn = P->sn;
         A.p1 = P->p1;
@@ -1685,7 +1878,7 @@ void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
     for (int kind = 0; kind < 2; ++kind) {
         const int n = (int)P->items[kind].size();
         if (!n) continue;
-        DtArgs D{P->dblocks, P->ditems[kind], n, (P->variant == 1 || P->variant == 5 || P->variant == 6) ? STRIP_A : STRIP,
+        DtArgs D{P->dblocks, P->ditems[kind], n, (P->variant == 1 || P->variant >= 5) ? STRIP_A : STRIP,
                  U, P->dtmin,
                  P->bad, P->step, P->nslot,
                  P->ph};
@@ -2071,11 +2264,18 @@ int phx_set_launch(phx_plan* P, int32_t rows_per_item, int32_t grid_waves, int32
 }
 
 int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
-    if (!P || variant < 0 || variant > 6 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
+    if (!P || variant < 0 || variant > 8 || rows_b < 0) return fail(PHX_E_INVALID, "invalid stage variant");
     if (P->ph.nq > 1 && variant != 1 && variant != 5 && variant != 6)
         return fail(PHX_E_INVALID, "more than one Gauss point per face runs the split stages only (variants 1, 5, 6)");
+    if (variant == 8) {
+        const int rb = rows_b > 0 ? rows_b : P->rows_b;
+        const int T = (std::min(std::max(rb, 1), FLAT_MAX_WAVES - 1) + 1) * WAVE;
+        for (int b = 0; b < P->nblocks; ++b)
+            if ((P->desc[b].nx + 64 + T - 1) / T * T > FLAT_MAX_FACES)
+                return fail(PHX_E_INVALID, "a block is too wide for the flat y-face stage (variant 8: nx + 64 faces per range)");
+    }
     PHX_CK(hipDeviceSynchronize());
-    if ((variant == 1 || variant == 5 || variant == 6) && !P->sn) {
+    if ((variant == 1 || variant >= 5) && !P->sn) {
         const size_t L = (size_t)std::max<int64_t>(P->state_len, 4);
         PHX_CK(hipMalloc(&P->sn, 2 * (size_t)std::max(P->ph.nq, 1) * L * sizeof(double)));
         PHX_CK(hipMalloc(&P->p1, L * sizeof(double)));

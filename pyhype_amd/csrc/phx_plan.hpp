@@ -73,12 +73,16 @@ struct phx_plan {
     phx::host::WorkItem* ditems[2] = {nullptr, nullptr};
     int grid[2] = {0, 0};
     // stage variant: 0 = fused rolling rows, 1 = split (reconstruct + x-faces, y-faces, update),
-    // 2 = shared-halo workgroups, 3 = element-wise (cell-variable lanes, face lanes, cell lanes)
+    // 2 = shared-halo workgroups, 3 = element-wise (cell-variable lanes, face lanes, cell lanes),
+    // 4 = paired waves, 5 = split with y-faces and update fused, 6 = split with one y-face per wave,
+    // 7 = variant 6 with the row-major reconstruction kernel, 8 = the row-major reconstruction and
+    // y-faces and update over flat cell ranges
     int variant = 0, rows_b = 8;
     std::vector<phx::host::WorkItem> items_b[2];  // split variant: update kernel
+    int flat_q[2] = {0, 0};                          // variant 8: LDS faces per lane of the update kernel
     phx::host::WorkItem* ditems_b[2] = {nullptr, nullptr};
     int grid_b[2] = {0, 0};
-    std::vector<phx::host::WorkItem> items_f[2];  // split variant: y-face kernel
+    std::vector<phx::host::WorkItem> items_f[2];  // variant 1: y-face kernel; variant 7: reconstruction
     phx::host::WorkItem* ditems_f[2] = {nullptr, nullptr};
     int grid_f[2] = {0, 0};
     // element-wise variant: tiles of [0] the reconstruction (cells), [1] the faces, [2] the update (cells)

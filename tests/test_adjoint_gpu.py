@@ -79,7 +79,7 @@ def test_step_vjp_is_the_rk_recurrence(integrator):
     assert torch.allclose(lam, ref, rtol=1e-12, atol=1e-12 * float(ref.abs().max()))
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_run_record_matches_run_frozen(variant):
     """phx_run_record on every stage variant (the tuned forward of a workload may be any of them)."""
     import torch
@@ -89,7 +89,7 @@ def test_run_record_matches_run_frozen(variant):
     s = Euler2D(cfg, mesh)
     s.apply_initial_condition()
     P = s.plan
-    P.set_launch(2, 0, 3 if variant in (1, 5, 6) else 2, variant=variant, rows_b=2)
+    P.set_launch(2, 0, 3 if variant in (1, 5, 6, 7, 8) else 2, variant=variant, rows_b=2)
     U0 = P.U.clone()
     dt = frozen_schedule(P, 5)
     P.U.copy_(U0)
@@ -118,8 +118,8 @@ def test_gradient_independent_of_the_forward_stage_variant():
 
     prob = _problem(24, 6, 4)
     J0, g0 = prob.gradient()
-    for variant in (1, 2, 4, 5, 6):
-        prob.plan.set_launch(1, 0, 3 if variant in (1, 5, 6) else 2, variant=variant, rows_b=1)
+    for variant in (1, 2, 4, 5, 6, 7, 8):
+        prob.plan.set_launch(1, 0, 3 if variant in (1, 5, 6, 7, 8) else 2, variant=variant, rows_b=1)
         J, g = prob.gradient()
         assert J == J0
         assert torch.equal(g, g0), variant

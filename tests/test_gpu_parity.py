@@ -196,7 +196,7 @@ def test_solve_writes_reference_files_and_stops_at_t_final(tmp_path):
         np.testing.assert_array_equal(np.load(root / str(step) / "expl_blk_0.npy"), st[0], err_msg=f"dir {step}")
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6, 7, 8])
 def test_double_mach_reflection_example_matches_oracle(variant):
     """The reference's double-Mach-reflection example (examples/dmr: 4 x 1 transfinite blocks over a
     30 degree ramp, outlets and slip walls, HLLL, RK2, PrimitiveState reconstruction) at 12 x 12
@@ -206,7 +206,7 @@ def test_double_mach_reflection_example_matches_oracle(variant):
     s = _solver(cfg, mesh, rows_per_item=2)
     assert {e.cartesian for e in s.plan.blocks} == {True, False}
     if variant:
-        s.plan.set_launch(2, 0, 3 if variant in (1, 5, 6) else 2, variant=variant, rows_b=2)
+        s.plan.set_launch(2, 0, 3 if variant in (1, 5, 6, 7, 8) else 2, variant=variant, rows_b=2)
     st = s.states()[0]
     sim = oracle_for(cfg, mesh, st)
     R = s.plan.get_states(s.plan.residual())[0]
@@ -251,10 +251,11 @@ def test_skewed_multiblock_residual_and_steps_match_oracle():
     ("single", 1, 4, 3), ("single", 3, 32, 4), ("multiblock", 2, 8, 3), ("oblique", 4, 5, 4),
     ("skewed", 1, 3, 3), ("wide", 2, 7, 3), ("wide_multiblock", 3, 16, 4), ("narrow", 1, 1, 3),
     ("narrow", 2, 64, 4)])
-@pytest.mark.parametrize("variant", [1, 5, 6])
+@pytest.mark.parametrize("variant", [1, 5, 6, 7, 8])
 def test_split_variant_matches_oracle(case, rows, rows_b, waves, variant):
     """The split stage (reconstruction + x-faces, then y-faces, then the update; variant 5 with the
-    y-faces and the update in one kernel) gives the oracle's results bit for bit: residual, steps
+    y-faces and the update in one kernel, variant 6 with one y-face per wave, variant 7 that with
+    the row-major reconstruction kernel, variant 8 with the y-faces over flat cell ranges) gives the oracle's results bit for bit: residual, steps
     (RK4 / SSP-RK2 with the CFL reduction) and the ghost semantics."""
     if case == "single":
         cfg, mesh = W.explosion_single(26)
@@ -389,7 +390,7 @@ def test_paired_variant_batched_cases_with_failures():
     assert list(a.t) == list(b.t)
 
 
-@pytest.mark.parametrize("variant", [1, 5, 6])
+@pytest.mark.parametrize("variant", [1, 5, 6, 7, 8])
 def test_split_variant_batched_cases(variant):
     cfgs, meshes, _ = W.explosion_batch(seed=3, ncases=4, sizes=[40, 64, 33, 70])
     a = Euler2DBatch(cfgs, meshes, rows_per_item=2)
@@ -404,7 +405,7 @@ def test_split_variant_batched_cases(variant):
 
 
 @pytest.mark.parametrize("limiter", ["BarthJespersen", "VanAlbada", "VanLeer"])
-@pytest.mark.parametrize("variant", [0, 1, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 3, 4, 5, 6, 7, 8])
 def test_other_limiters_match_oracle(limiter, variant):
     """SlopeLimiterFactory's other limiters (limiters/limiters.py:25-62) on the general-geometry
     multiblock mesh: residual and RK4 steps bit for bit, fused, split, element-wise and paired
@@ -470,7 +471,7 @@ def test_shared_halo_variant_batched_cases_and_limiter():
     assert a.failed == b.failed
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("case", ["multiblock", "oblique"])
 def test_primitive_reconstruction_matches_oracle(case, variant):
     """reconstruction_type = PrimitiveState (fvm/base.py:286-289, blocks/quad_block.py:225-240):
@@ -556,13 +557,13 @@ def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
     s = _solver(cfg, mesh, rows_per_item=2)
-    for v in (0, 2, 3, 4):
+    for v in (0, 2, 3, 4, 7, 8):
         with pytest.raises(ValueError, match="Gauss point"):
             s.plan.set_launch(2, 0, 2, variant=v)
     s.run_steps(2)   # the plan still runs after the refusals
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("integrator", ["ExplicitEuler1", "RK2", "Ralston2", "Generic2", "RK3", "RK3SSP",
                                         "Ralston3", "Ralston4", "DormandPrince5", "Generic3"])
 def test_every_butcher_table_matches_oracle(integrator, variant):
@@ -588,7 +589,7 @@ def test_every_butcher_table_matches_oracle(integrator, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_batch_mixing_cartesian_and_rotated_frame_blocks(variant):
     """One plan whose cases mix Cartesian blocks (one block), rotated-frame blocks with neighbours
     (a 2x2 rectangular mesh, off-by-ulp interior corners) and a skewed two-block mesh: both kernel
