@@ -958,7 +958,7 @@ constexpr int YU_MAX_ROWS = 7;  // rows per item (R + 1 <= 8 waves per workgroup
 // Q = true: Gauss faces (the sums S_F of yface_sum_q, the residual of the nq-point rule)
 template <bool CART, int FLUX, bool GEN, bool Q>
 __device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* lds) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int lane = threadIdx.x & (WAVE - 1);
     const int w = threadIdx.x / WAVE;
@@ -1015,7 +1015,7 @@ __global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(St
 // faces the dynamic LDS holds.
 template <bool CART, int FLUX, bool GEN>
 __device__ void yface_update_flat_item(const StageArgs& A, const WorkItem it, Q4* lds) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int T = blockDim.x;
     const int t = threadIdx.x;
@@ -1089,7 +1089,7 @@ constexpr int LIN_OWN = WAVE - 2;
 
 template <bool CART, int FLUX, bool GEN>
 __device__ void recon_x_lin_item(const StageArgs& A, const WorkItem it) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx, npos = B.ny * (nx + 1);
     const int lane = threadIdx.x % WAVE;
