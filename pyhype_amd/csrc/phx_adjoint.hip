@@ -25,6 +25,7 @@
 using phx::Phys;
 using phx::Q4;
 using phx::dev::DevBlock;
+using phx::dev::ld_const;
 using phx::host::AdjBlock;
 using phx::host::AdjTile;
 using phx::host::fail;
@@ -69,10 +70,10 @@ template <int FLUX, int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                    const double* U, const double* rb, Terms kappa, double* fadj,
                                                    Phys ph) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     if ((int)threadIdx.x >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const AdjBlock O = offs[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
     const long long f = (long long)t.start + threadIdx.x;
     double* out = fadj + (long long)phx::adj::FA * phx::adj::npts<NQ>(ph) * (O.face + f);
     const TermSeed seed{kappa};
@@ -83,10 +84,10 @@ __global__ void __launch_bounds__(TILE) k_adj_face(const DevBlock* blocks, const
 template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                     const double* U, double* rb, Phys ph) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     if ((int)threadIdx.x >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const AdjBlock O = offs[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = rb + (long long)phx::adj::RB * phx::adj::npts<NQ>(ph) * phx::adj::cell_index(O, B, k, j);
@@ -97,10 +98,10 @@ __global__ void __launch_bounds__(TILE) k_adj_recon(const DevBlock* blocks, cons
 template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                    const double* U, const double* fadj, double* slot, Phys ph) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     if ((int)threadIdx.x >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const AdjBlock O = offs[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
@@ -115,11 +116,11 @@ constexpr int VTILE = 4 * TILE;
 
 __global__ void __launch_bounds__(VTILE) k_adj_recon_v(const DevBlock* blocks, const AdjBlock* offs,
                                                        const AdjTile* tiles, const double* U, double* rb, Phys ph) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
     if (c >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const AdjBlock O = offs[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
     const int e = t.start + c;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = rb + (long long)phx::adj::RB * phx::adj::cell_index(O, B, k, j);
@@ -130,11 +131,11 @@ __global__ void __launch_bounds__(VTILE) k_adj_recon_v(const DevBlock* blocks, c
 __global__ void __launch_bounds__(VTILE) k_adj_cell_v(const DevBlock* blocks, const AdjBlock* offs,
                                                       const AdjTile* tiles, const double* U, const double* fadj,
                                                       double* slot, Phys ph) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
     if (c >= t.count) return;
-    const DevBlock& B = blocks[t.block];
-    const AdjBlock O = offs[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
     const int e = t.start + c;
     const int k = e / B.nx, j = e - k * B.nx;
     double* out = slot + phx::adj::SLOT * phx::adj::cell_index(O, B, k, j);
@@ -147,9 +148,9 @@ template <int NQ>
 __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
                                                      const double* slot, const double* fadj, const double* U,
                                                      Phys ph, Terms extra, double* lamU) {
-    const AdjTile t = tiles[blockIdx.x];
+    const AdjTile t = ld_const(tiles, blockIdx.x);
     if ((int)threadIdx.x >= t.count) return;
-    const DevBlock& B = blocks[t.block];
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
     const int e = t.start + threadIdx.x;
     const int k = e / B.nx, j = e - k * B.nx;
     Q4 lam = (B.kind == PHX_GEOM_CARTESIAN)
