@@ -553,6 +553,22 @@ def test_gauss_faces_with_primitive_reconstruction_and_van_leer(variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
+def test_flat_y_face_ranges_refuse_blocks_wider_than_their_lds():
+    """Variant 8 evaluates the nx + n y-faces of a range of n >= 64 cells in at most 2048 LDS
+    slots, so a block with nx + 64 > 2048 is refused; the plan keeps its stage and still runs."""
+    from pyhype_amd.initial_conditions import ExplosionInitialCondition
+    cfg, mesh = W._config(2000, 2, ic=ExplosionInitialCondition()), W.single_block_mesh()
+    s = _solver(cfg, mesh, rows_per_item=2)
+    with pytest.raises(ValueError, match="too wide"):
+        s.plan.set_launch(1, 0, 3, variant=8, rows_b=7)
+    assert s.plan.launch_info().get("variant", 0) != 8
+    st = s.states()[0]
+    sim = oracle_for(cfg, mesh, st)
+    s.run_steps(2)
+    sim.run(2)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
 def test_gauss_faces_refuse_fused_variants():
     cfg, mesh = W.explosion_single(12)
     cfg.fvm_num_quadrature_points = 3
