@@ -81,6 +81,14 @@ struct StageArgs {
     Phys ph;
 };
 
+// Element i of an array that no kernel writes, as a reference through the constant address
+// space: uniform field reads become scalar loads that the compiler may repeat instead of keeping
+// the whole element in registers.
+template <class T>
+__device__ __forceinline__ const T& const_ref(const T* p, long i) {
+    return *(const T*)(((const __attribute__((address_space(4))) T*)p) + i);
+}
+
 // cross-lane moves of a 4-vector
 __device__ __forceinline__ Q4 shfl_up1(const Q4& q) {
     Q4 r;
@@ -795,7 +803,7 @@ __device__ void yface_update_item_q(const StageArgs& A, const WorkItem it) {
 
 template <bool CART, int FLUX, bool GEN>
 __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock& B = const_ref(A.blocks, it.block);
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx;
     const int j = it.j0 - HALO + threadIdx.x % WAVE;
