@@ -487,7 +487,7 @@ __device__ __forceinline__ void emit_half(const StageArgs& A, const DevBlock& B,
 
 template <bool CART, int FLUX, bool GEN, int H>
 __device__ void stage_pair_body(const StageArgs& A, const WorkItem it, PairLds& L) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock& B = const_ref(A.blocks, it.block);
     const int lane = threadIdx.x & (WAVE - 1);
     const int nx = B.nx, ny = B.ny;
     const int j = it.j0 - HALO + lane;
@@ -636,7 +636,7 @@ __global__ void __launch_bounds__(PAIR_WAVES * WAVE, MINW) k_stage_pair(StageArg
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) {
         const WorkItem it = ld_const(A.items, i);
-        const DevBlock& B = A.blocks[it.block];
+        const DevBlock& B = const_ref(A.blocks, it.block);
         if (A.skip_dead && A.bad[B.case_id] < *A.step) continue;  // uniform over the workgroup
         if (h == 0) stage_pair_body<CART, FLUX, GEN, 0>(A, it, L);
         else stage_pair_body<CART, FLUX, GEN, 1>(A, it, L);
