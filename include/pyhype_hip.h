@@ -141,8 +141,9 @@ int phx_reset(phx_plan* plan, const double* t0_host, int32_t step0, void* stream
  * (solvers/base.py:114-136), then ExplicitRungeKutta.integrate (explicit_runge_kutta.py:47-89)
  * with the ghost fill of Blocks.apply_boundary_condition (blocks/base.py:448-471) after each
  * stage, then the realizability check (solvers/Euler2D.py:144-152).  Replaces the body of the
- * step loop Euler2D._solve (solvers/Euler2D.py:195-210).  use_graph != 0 replays a captured
- * hipGraph of one step.  Realizability is reported by phx_status. */
+ * step loop Euler2D._solve (solvers/Euler2D.py:195-210).  use_graph != 0 replays captured
+ * hipGraphs (captured on first use): one of 8 steps per full group of 8, then one of a single step
+ * per remaining step.  Realizability is reported by phx_status. */
 int phx_run(phx_plan* plan, double* U, int32_t nsteps, int32_t use_graph, void* stream);
 
 /* Advance nsteps with a frozen, caller-given time step per step (dt_sched: device pointer,

@@ -367,8 +367,43 @@ def _roe_coo(F):
     return Rc_i, Rc_j, Lp_i, Lp_j, L_i
 
 
+def roe_rc_data(Wr, aroe, Lm, Lp):
+    """FluxRoe.compute_Rc (flux/Roe.py:175-220): the 14 COO blocks of Rc, with the Harten-corrected
+    slow/fast speeds Lm, Lp in the (1,0) and (1,3) entries (quirk B.2); unset blocks keep the 1.0 of
+    np.ones (flux/eigen_system.py:36-42)."""
+    num = Wr.shape[0] * Wr.shape[1]
+    u = Wr[:, :, 1]
+    v = Wr[:, :, 2]
+    Ek = 0.5 * (u * u + v * v)                       # Ek_JIT
+    H = G_OVER_GM1 * Wr[:, :, 3] / Wr[:, :, 0] + Ek  # H_given_Ek_JIT
+    ua = u * aroe
+    Rc_data = np.ones(14 * num)
+    seg = lambda k, val: Rc_data.__setitem__(slice(k * num, (k + 1) * num), np.ravel(val))
+    seg(0, H - ua); seg(1, v); seg(2, Ek); seg(3, Lm); seg(4, v); seg(5, v)
+    seg(7, u); seg(9, H + ua); seg(11, v); seg(12, Lp)
+    return Rc_data
+
+
+def roe_lp_data(Wr, aroe):
+    """FluxRoe.compute_Lp (flux/Roe.py:222-251): the 7 COO blocks of Lp.  Block 2, the (2,2) entry,
+    is never written and keeps the 1.0 of np.ones (flux/eigen_system.py:52-57): the shear-wave
+    strength is dv, not rho* dv (quirk B.15)."""
+    num = Wr.shape[0] * Wr.shape[1]
+    _a = 1.0 / aroe
+    one_a2 = _a * _a
+    h_one_a2 = 0.5 * one_a2
+    r2a = 0.5 * Wr[:, :, 0] * _a
+    Lp_data = np.ones(7 * num)
+    for k, val in ((0, r2a), (3, h_one_a2), (4, -r2a), (5, -one_a2), (6, h_one_a2)):
+        Lp_data[k * num:(k + 1) * num] = np.ravel(val)
+    return Lp_data
+
+
 def flux_roe(WL, WR):
-    """FluxRoe.compute_flux (flux/Roe.py:175-304) on primitive (m, F, 4) face arrays."""
+    """FluxRoe.compute_flux (flux/Roe.py:262-304) on primitive (m, F, 4) face arrays: central flux
+    minus 0.5 Rc |Lambda| Lp dW with the Roe state of states/primitive.py:301-320, Harten-corrected
+    eigenvalues (flux/Roe.py:292-304, flux/base.py:119-146) and SciPy COO mat-vecs in the
+    reference's storage order."""
     shape = WL.shape
     F = shape[0] * shape[1]
     wl = WL.reshape(1, F, 4)
@@ -380,23 +415,9 @@ def flux_roe(WL, WR):
     Roe_p, Roe_m = wavespeeds(Wr, aroe)
     Lm, Lp = harten(R_p, R_m, L_p, L_m, Roe_p, Roe_m)   # slow, fast (corrected)
     u = Wr[:, :, 1]
-    v = Wr[:, :, 2]
-    Ek = 0.5 * (u * u + v * v)                       # Ek_JIT
-    H = G_OVER_GM1 * Wr[:, :, 3] / Wr[:, :, 0] + Ek  # H_given_Ek_JIT
-    ua = u * aroe
-    num = F
     Rc_i, Rc_j, Lp_i, Lp_j, L_i = _roe_coo(F)
-    Rc_data = np.ones(14 * num)
-    seg = lambda k, val: Rc_data.__setitem__(slice(k * num, (k + 1) * num), np.ravel(val))
-    seg(0, H - ua); seg(1, v); seg(2, Ek); seg(3, Lm); seg(4, v); seg(5, v)
-    seg(7, u); seg(9, H + ua); seg(11, v); seg(12, Lp)
-    _a = 1.0 / aroe
-    one_a2 = _a * _a
-    h_one_a2 = 0.5 * one_a2
-    r2a = 0.5 * Wr[:, :, 0] * _a
-    Lp_data = np.ones(7 * num)
-    for k, val in ((0, r2a), (3, h_one_a2), (4, -r2a), (5, -one_a2), (6, h_one_a2)):
-        Lp_data[k * num:(k + 1) * num] = np.ravel(val)
+    Rc_data = roe_rc_data(Wr, aroe, Lm, Lp)
+    Lp_data = roe_lp_data(Wr, aroe)
     L_data = np.concatenate([np.ravel(Lm), np.ravel(u), np.ravel(u), np.ravel(Lp)])
     Rc = coo_matrix((Rc_data, (Rc_i, Rc_j)))
     Lpm = coo_matrix((Lp_data, (Lp_i, Lp_j)))
@@ -408,9 +429,28 @@ def flux_roe(WL, WR):
 
 
 def nrm2(x):
-    """Euclidean norm over the last axis (restates BLAS dnrm2 used by flux/HLLL.py:86-93 as a
-    sequential sum of squares; ulp-level difference from an optimised BLAS: parity unpinned)."""
-    return np.sqrt(((x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]) + x[..., 2] * x[..., 2]) + x[..., 3] * x[..., 3])
+    """Euclidean norm over the last axis as flux/HLLL.py:86-93 computes it: Numba's np.linalg.norm
+    of a 1-D float64 array calls the BLAS dnrm2 of its SciPy (OpenBLAS 0.3.28 with scipy 1.15.3),
+    whose x86-64 kernel sums 0 + x0^2 + x1^2 + x2^2 + x3^2 in order in the x87 80-bit format, takes
+    the square root there and rounds once to double.  np.longdouble is that format on x86-64 Linux;
+    pinned against scipy.linalg.blas.dnrm2 in tests/test_oracle_pins.py."""
+    if np.finfo(np.longdouble).nmant != 63:
+        raise RuntimeError("nrm2 needs the x87 80-bit long double (x86-64)")
+    X = np.asarray(x, dtype=np.float64).astype(np.longdouble)
+    s = np.zeros(X.shape[:-1], dtype=np.longdouble)
+    for k in range(X.shape[-1]):
+        s = s + X[..., k] * X[..., k]
+    return np.sqrt(s).astype(np.float64)
+
+
+def hlll_coef(aroe, u, dU, dF, Lminus, Lplus):
+    """_HLLL_flux_JIT's dissipation coefficient (flux/HLLL.py:84-101): alpha = max(0, 1 - |dF - u dU|
+    / k) with k = a |dU| (k + 1e-14 when k < 1e-16), then Lm Lp (1 - alpha (1 - max(u/Lm, u/Lp)))."""
+    k = aroe * nrm2(dU)
+    n2 = nrm2(dF - u[:, :, None] * dU)
+    alpha = np.maximum(0, 1 - n2 / np.where(k < 1e-16, k + 1e-14, k))
+    mx = np.maximum(u / Lminus, u / Lplus)
+    return Lminus * Lplus * (1 - alpha * (1 - mx))
 
 
 def flux_hlll(WL, WR):
@@ -440,11 +480,7 @@ def flux_hlll(WL, WR):
     dU = UR - UL
     dF = FR - FL
     with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
-        k = aroe * nrm2(dU)
-        n2 = nrm2(dF - u[:, :, None] * dU)
-        alpha = np.maximum(0, 1 - n2 / np.where(k < 1e-16, k + 1e-14, k))
-        mx = np.maximum(u / Lminus, u / Lplus)
-        coef = Lminus * Lplus * (1 - alpha * (1 - mx))
+        coef = hlll_coef(aroe, u, dU, dF, Lminus, Lplus)
         mid = (Lplus[:, :, None] * FL - Lminus[:, :, None] * FR + coef[:, :, None] * dU) / (Lplus - Lminus)[:, :, None]
     out = np.where((Lminus >= 0)[:, :, None], FL, np.where((Lplus <= 0)[:, :, None], FR, mid))
     return out
@@ -565,6 +601,14 @@ def compute_slope(dmax, dmin, d):
     return s
 
 
+def reconstruction_delta(U, h):
+    """SecondOrderMUSCL.unlimited_solution_at_quadrature_point minus the cell average
+    (fvm/SecondOrderMUSCL.py:86-94, limiters/base.py:96-107): (U + hot) - U, which is 0, not hot,
+    wherever hot is below half an ulp of U (quirk B.4)."""
+    q = U + h           # unlimited solution at the quadrature point
+    return q - U
+
+
 def limiter_phi(U, ghost, hot: Dict[str, object], limiter_fn=venkatakrishnan):
     """SlopeLimiter._get_slope / _limit (limiters/base.py:47-108,147-187) with the limiter
     function of the configured limiter (default Venkatakrishnan).  hot[d]: the high-order term of
@@ -578,8 +622,7 @@ def limiter_phi(U, ghost, hot: Dict[str, object], limiter_fn=venkatakrishnan):
     phis = []
     for d in DIRS:
         for h in (hot[d] if isinstance(hot[d], (list, tuple)) else (hot[d],)):
-            q = U + h           # unlimited solution at the quadrature point
-            dd = q - U
+            dd = reconstruction_delta(U, h)
             with np.errstate(divide="ignore", invalid="ignore"):
                 phis.append(limiter_fn(compute_slope(dmax, dmin, dd)))
     phi = np.minimum.reduce(tuple(phis))

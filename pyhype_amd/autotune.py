@@ -71,12 +71,14 @@ def _time_steps(plan, steps: int) -> float:
     return (time.perf_counter() - t0) / steps
 
 
-def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: Sequence[int] = (2, 3),
+def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0, 1024, 2048, 3072), waves: Sequence[int] = (2, 3),
                split_rows: Sequence[int] = (1, 2, 3, 4, 8), split_rows_b: Sequence[int] = (1, 2, 4, 8, 16, 32),
                split_waves: Sequence[int] = (3, 4), shared_rows: Sequence[int] = (1, 2, 3, 4, 6),
                paired_rows: Sequence[int] = (1, 2, 3, 4, 6, 8, 12)):
-    """The default search space: fused shapes, shared-halo shapes (rows per wave), split shapes,
-    the element-wise stage, paired shapes (rows per item)."""
+    """The default search space: fused shapes (one workgroup per work item, grid 0, or a persistent
+    grid of `grids` workgroups that stride over the work list), shared-halo shapes (rows per wave),
+    split shapes, the element-wise stage, paired shapes (rows per item), and persistent grids of the
+    split stages' kernels."""
     out = [{"variant": 0, "rows_per_item": r, "grid_waves": g, "waves_per_simd": w, "rows_b": 0}
            for w in waves for g in grids for r in rows]
     out += [{"variant": 2, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": 2, "rows_b": 0}
@@ -94,6 +96,8 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0,), waves: S
             for w in split_waves for rb in (1, 2, 3, 4, 7)]
     out += [{"variant": 8, "rows_per_item": r, "grid_waves": 0, "waves_per_simd": w, "rows_b": rb}
             for w in split_waves for r in (1, 2) for rb in (7, 15)]
+    out += [{"variant": v, "rows_per_item": r, "grid_waves": g, "waves_per_simd": 3, "rows_b": rb}
+            for v, r, rb in ((6, 1, 3), (7, 1, 3), (8, 1, 7)) for g in grids if g]
     return out
 
 
@@ -114,20 +118,30 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
             shapes = [c for c in shapes if c["variant"] in (1, 5, 6)]
     U_saved = plan.U.clone()
     _, step0, t0, _ = plan.status()
-    trials = []
-    for c in shapes:
-        _apply(plan, c)
+    trials, skipped = [], []
+    best = None
+    try:
+        for c in shapes:
+            try:
+                _apply(plan, c)
+            except ValueError as e:   # a shape the library refuses for this mesh (e.g. LDS ranges)
+                skipped.append(dict(c, reason=str(e)))
+                continue
+            plan.U.copy_(U_saved)
+            plan.reset(t0, step0)
+            ms = _time_steps(plan, steps) * 1e3
+            trials.append(dict(c, ms_per_step=ms))
+            if verbose:
+                print(f"autotune {c}: {ms:.4f} ms/step", flush=True)
+        if not trials:
+            raise ValueError("autotune: the library refused every candidate shape")
+        best = min(trials, key=lambda t: t["ms_per_step"])
+        _apply(plan, best)
+    finally:
+        # the timed trials advanced the state: always hand the plan back as it was
         plan.U.copy_(U_saved)
-        plan.reset(t0, step0)
-        ms = _time_steps(plan, steps) * 1e3
-        trials.append(dict(c, ms_per_step=ms))
-        if verbose:
-            print(f"autotune {c}: {ms:.4f} ms/step", flush=True)
-    best = min(trials, key=lambda t: t["ms_per_step"])
-    _apply(plan, best)
-    plan.U.copy_(U_saved)
-    plan.reset(np.asarray(t0), step0)
-    result = dict(best, trials=trials)
+        plan.reset(np.asarray(t0), step0)
+    result = dict(best, trials=trials, skipped=skipped)
     if path is not None:
         cache = load_cache(path)
         cache[signature(plan)] = {k: best[k] for k in ("variant", "rows_per_item", "rows_b", "grid_waves",

@@ -152,10 +152,205 @@ __host__ __device__ inline Q4 roe_flux(const Prim& L, const Prim& R, const Phys&
     return F;
 }
 
-// Euclidean norm of a 4-vector: sqrt(((x0^2 + x1^2) + x2^2) + x3^2).  Restates the BLAS dnrm2
-// reached through Numba's np.linalg.norm in flux/HLLL.py:86-93 (agrees to an ulp).
+// ---- Euclidean norm of HLLL's alpha (flux/HLLL.py:86-93) ----
+//
+// Numba lowers np.linalg.norm of a 1-D float64 array to the BLAS routine dnrm2 of the SciPy build
+// it runs against.  SciPy's OpenBLAS (0.3.28 with scipy 1.15.3) uses the x87 kernel on x86-64:
+// for n < 8 it accumulates 0 + x0^2 + x1^2 + x2^2 + x3^2 in order in the x87 registers (64-bit
+// significand, round to nearest even after every multiply and add), takes fsqrt in the same
+// format and returns the result rounded to double.  That is 1 ulp away from the double-precision
+// formula sqrt(((x0^2+x1^2)+x2^2)+x3^2) on about one vector in five, so the extended format is
+// emulated exactly here with 64-bit integer arithmetic.  Non-negative operands only (squares and
+// their sums); no overflow or underflow exists in the extended range for squares of doubles.
+
+// A non-negative extended value m * 2^e: m has bit 63 set, or m = 0 for zero.
+struct Ext {
+    unsigned long long m;
+    int e;
+};
+
+__host__ __device__ inline unsigned long long umulhi64(unsigned long long a, unsigned long long b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (unsigned long long)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+__host__ __device__ inline int clz64(unsigned long long x) { return __builtin_clzll(x); }
+
+__host__ __device__ inline unsigned long long dbits(double x) { return __builtin_bit_cast(unsigned long long, x); }
+
+// Rounds the 128-bit significand (hi, lo) * 2^(e - 64), hi bit 63 set, to 64 bits, nearest even;
+// `sticky` flags non-zero bits below lo.
+__host__ __device__ inline Ext ext_round(unsigned long long hi, unsigned long long lo, bool sticky, int e) {
+    const bool half = (lo >> 63) != 0;
+    const bool rest = ((lo << 1) != 0) || sticky;
+    if (half && (rest || (hi & 1ull))) {
+        ++hi;
+        if (hi == 0) {
+            hi = 1ull << 63;
+            ++e;
+        }
+    }
+    return Ext{hi, e};
+}
+
+// x*x rounded to the extended format (x87 fmul of two doubles loaded exactly).  Finite x.
+__host__ __device__ inline Ext ext_sq(double x) {
+    const unsigned long long b = dbits(x) & 0x7fffffffffffffffull;
+    if (b == 0) return Ext{0ull, 0};
+    const int be = (int)(b >> 52);
+    unsigned long long M = b & 0x000fffffffffffffull;
+    int E;
+    if (be == 0) {
+        E = -1074;  // subnormal: M * 2^-1074
+    } else {
+        M |= 1ull << 52;
+        E = be - 1075;
+    }
+    const int lz = clz64(M);
+    M <<= lz;  // x = M * 2^(E - lz), M in [2^63, 2^64)
+    E -= lz;
+    unsigned long long hi = umulhi64(M, M), lo = M * M;  // x^2 = (hi*2^64 + lo) * 2^(2E)
+    int e = 2 * E + 64;
+    if ((hi >> 63) == 0) {
+        hi = (hi << 1) | (lo >> 63);
+        lo <<= 1;
+        e -= 1;
+    }
+    return ext_round(hi, lo, false, e);
+}
+
+// a + b rounded to the extended format (x87 fadd), a, b >= 0.
+__host__ __device__ inline Ext ext_add(Ext a, Ext b) {
+    if (b.m == 0) return a;
+    if (a.m == 0) return b;
+    if (a.e < b.e) {
+        const Ext t = a;
+        a = b;
+        b = t;
+    }
+    const int d = a.e - b.e;
+    // b < 2^(b.e + 64) <= 2^(a.e - 2), below half an ulp of a: a + b rounds to a.
+    if (d > 65) return a;
+    // a = (a.m * 2^64) * 2^(a.e - 64); b aligned to the same unit as (bh, bl) plus a sticky bit
+    unsigned long long bh, bl;
+    bool sticky = false;
+    if (d == 0) {
+        bh = b.m;
+        bl = 0;
+    } else if (d < 64) {
+        bh = b.m >> d;
+        bl = b.m << (64 - d);
+    } else if (d == 64) {
+        bh = 0;
+        bl = b.m;
+    } else {  // d == 65
+        bh = 0;
+        bl = b.m >> 1;
+        sticky = (b.m & 1ull) != 0;
+    }
+    unsigned long long lo = bl;                 // a's low word is zero
+    unsigned long long hi = a.m + bh;
+    int e = a.e;
+    if (hi < a.m) {  // carry out of bit 127: shift the 129-bit sum right by one
+        sticky = sticky || (lo & 1ull);
+        lo = (lo >> 1) | (hi << 63);
+        hi = (hi >> 1) | (1ull << 63);
+        e += 1;
+    }
+    return ext_round(hi, lo, sticky, e);
+}
+
+// 128-bit helpers for the square root (value = hi * 2^64 + lo).
+__host__ __device__ inline bool u128_gt(unsigned long long ah, unsigned long long al, unsigned long long bh,
+                                        unsigned long long bl) {
+    return ah > bh || (ah == bh && al > bl);
+}
+
+// RN_double(RN_ext(sqrt(s))) for an extended s (x87 fsqrt, then the store to double).
+__host__ __device__ inline double ext_sqrt_to_double(Ext s) {
+    if (s.m == 0) return 0.0;
+    // N = s.m * 2^k with s.e - k even, k in {63, 64}: N in [2^126, 2^128), sqrt(N) in [2^63, 2^64)
+    const int k = ((s.e - 64) & 1) ? 63 : 64;
+    const unsigned long long nh = (k == 64) ? s.m : (s.m >> 1);
+    const unsigned long long nl = (k == 64) ? 0ull : (s.m << 63);
+    const int E = (s.e - k) / 2;  // sqrt(s) = sqrt(N) * 2^E  (s.e - k is even)
+    // q = floor(sqrt(N)): a double estimate (|error| < 2^13), one Newton step, then an exact fix-up
+    const double dn = (double)nh * 18446744073709551616.0 + (double)nl;
+    const double sq = sqrt(dn);
+    unsigned long long q = (sq >= 18446744073709551616.0) ? 0xffffffffffffffffull : (unsigned long long)sq;
+    {
+        // rem = N - q^2 (|rem| < 2^79, two's complement over 128 bits)
+        const unsigned long long qh = umulhi64(q, q), ql = q * q;
+        const unsigned long long rl = nl - ql;
+        const long long rh = (long long)(nh - qh - (nl < ql ? 1ull : 0ull));
+        const double drem = (double)rh * 18446744073709551616.0 + (double)rl;
+        const double step = drem / (2.0 * (double)q);
+        const long long dq = (long long)(step + (step >= 0.0 ? 0.5 : -0.5));  // |dq| <= 2^13
+        if (dq >= 0) {
+            const unsigned long long up = (unsigned long long)dq;
+            q = (q > 0xffffffffffffffffull - up) ? 0xffffffffffffffffull : q + up;
+        } else {
+            q -= (unsigned long long)(-dq);
+        }
+    }
+    // now |q - sqrt(N)| < 1: q^2 > N -> q - 1; (q+1)^2 <= N -> q + 1 (twice each, for margin)
+    for (int it = 0; it < 2; ++it) {
+        const unsigned long long qh = umulhi64(q, q), ql = q * q;
+        if (u128_gt(qh, ql, nh, nl)) --q;
+    }
+    for (int it = 0; it < 2; ++it) {
+        if (q == 0xffffffffffffffffull) break;
+        const unsigned long long p = q + 1;
+        const unsigned long long ph = umulhi64(p, p), pl = p * p;
+        if (!u128_gt(ph, pl, nh, nl)) q = p;
+    }
+    // round to 64 bits: up iff N - q^2 > q (sqrt(N) is never exactly q + 1/2)
+    const unsigned long long qh = umulhi64(q, q), ql = q * q;
+    const unsigned long long rl = nl - ql;
+    const unsigned long long rh = nh - qh - (nl < ql ? 1ull : 0ull);  // 0 <= rem <= 2q < 2^65
+    int e64 = E;                                                       // r64 * 2^e64
+    unsigned long long r64 = q;
+    if (rh != 0 || rl > q) {
+        ++r64;
+        if (r64 == 0) {
+            r64 = 1ull << 63;
+            ++e64;
+        }
+    }
+    // round the 64-bit significand to the double's precision (53 bits, fewer if subnormal)
+    int shift = 11;
+    const int top = e64 + 63;  // exponent of the leading bit
+    if (top < -1022) shift += -1022 - top;
+    if (shift > 64) return 0.0;  // below half the smallest subnormal
+    unsigned long long r53, rem_bits, halfv;
+    if (shift == 64) {
+        r53 = 0;
+        rem_bits = r64;
+        halfv = 1ull << 63;
+    } else {
+        r53 = r64 >> shift;
+        rem_bits = r64 & ((1ull << shift) - 1ull);
+        halfv = 1ull << (shift - 1);
+    }
+    if (rem_bits > halfv || (rem_bits == halfv && (r53 & 1ull))) ++r53;
+    // r53 * 2^(e64 + shift) is exact in double (r53 <= 2^53); ldexp only scales
+    return ldexp((double)r53, e64 + shift);
+}
+
+// Euclidean norm of a 4-vector as dnrm2 computes it (see above).  Non-finite components take the
+// double formula (inf or NaN, as the extended computation gives).
 __host__ __device__ inline double nrm4(double a, double b, double c, double d) {
-    return sqrt(((a * a + b * b) + c * c) + d * d);
+    const unsigned long long ex = 0x7ff0000000000000ull;
+    if ((dbits(a) & ex) == ex || (dbits(b) & ex) == ex || (dbits(c) & ex) == ex || (dbits(d) & ex) == ex)
+        return sqrt(((a * a + b * b) + c * c) + d * d);
+    Ext s = ext_sq(a);
+    s = ext_add(s, ext_sq(b));
+    s = ext_add(s, ext_sq(c));
+    s = ext_add(s, ext_sq(d));
+    return ext_sqrt_to_double(s);
 }
 
 // FluxHLLL.compute_flux + _HLLL_flux_JIT (flux/HLLL.py:25-49,70-103) for one face.

@@ -120,6 +120,21 @@ class RectagularMeshGenerator:
                                  left_y=ew_y, right_y=ew_y)
 
 
+def mesh_dict_from_block_descriptions(descriptions: dict) -> dict:
+    """The mesh dictionary a solver was built from, recovered from the reference's per-block
+    BlockDescription objects (blocks/base.py:777-843; Solver.get_mesh_config, solvers/base.py:90-100,
+    keeps only these): nBLK from ``info``, the corners from ``geometry.vertices``, neighbours and
+    boundary types (sides and corners) from the ``neighbors`` / ``bc`` property dicts."""
+    out = {}
+    for n, d in descriptions.items():
+        v, nb, bc = d.geometry.vertices, d.neighbors, d.bc
+        out[n] = {"nBLK": d.info.nBLK, "NE": v.NE, "NW": v.NW, "SE": v.SE, "SW": v.SW,
+                  "NeighborE": nb.E, "NeighborW": nb.W, "NeighborN": nb.N, "NeighborS": nb.S,
+                  "BCTypeE": bc.E, "BCTypeW": bc.W, "BCTypeN": bc.N, "BCTypeS": bc.S,
+                  "BCTypeNE": bc.NE, "BCTypeNW": bc.NW, "BCTypeSE": bc.SE, "BCTypeSW": bc.SW}
+    return out
+
+
 # ------------------------------------------------------------------------------------------------
 # geometry of one block
 # ------------------------------------------------------------------------------------------------

@@ -71,6 +71,11 @@ def validate_config(config) -> List[List[float]]:
         raise ValueError("reconstruction_type must be ConservativeState or PrimitiveState")
     if config.interface_interpolation != "arithmetic_average":
         raise ValueError("Interface Interpolation method is not defined.")
+    if not config.use_JIT:
+        # the reference's use_JIT=False path evaluates other formulas (flux/HLLL.py:43-60,
+        # flux/base.py:113-117, limiters/limiters.py:33-36); only the JIT arithmetic is implemented
+        raise ValueError("use_JIT=False selects the reference's NumPy formulas, which this backend does not "
+                         "implement; use use_JIT=True (the reference default)")
     return butcher_table(config.time_integrator, config)
 
 

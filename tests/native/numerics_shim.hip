@@ -48,6 +48,10 @@ void shim_reconstruct_lim(int lim, int n, const double* U, const double* Uw, con
         put(out + 16 * i, r.E); put(out + 16 * i + 4, r.W); put(out + 16 * i + 8, r.N); put(out + 16 * i + 12, r.S);
     }
 }
+// Euclidean norm of n 4-vectors as HLLL's alpha computes it (nrm4)
+void shim_nrm4(int n, const double* X, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = nrm4(X[4 * i], X[4 * i + 1], X[4 * i + 2], X[4 * i + 3]);
+}
 void shim_cell_dt(int n, const double* U, const double* dx, const double* dy, double* out, const double* phys) {
     const Phys ph = make_phys(phys);
     for (int i = 0; i < n; ++i) out[i] = cell_dt(q4(U + 4 * i), dx[i], dy[i], ph);

@@ -97,3 +97,47 @@ def test_committed_cache_covers_the_tuned_workloads():
         if e["variant"] == 8:
             assert 1 <= e["rows_b"] <= FLAT_MAX_WAVES - 1
         assert "/q" not in k or e["variant"] in (1, 5, 6)   # Gauss faces run the split stages only
+
+
+def test_autotune_skips_refused_shapes_and_always_restores_the_plan(tmp_path, monkeypatch):
+    """A shape the library refuses (variant 8 on blocks wider than its LDS ranges) is skipped and
+    recorded; the trials' state changes are undone even when a trial fails midway."""
+    plan = fake_plan([True], [(2100, 8)])
+    plan.U = torch.arange(8, dtype=torch.float64)
+    state = {"launch": None, "resets": []}
+    plan.status = lambda: (None, 3, 0.5, None)
+    plan.reset = lambda t0, step0: state["resets"].append((float(t0), step0))
+
+    def set_launch(r, g, w, variant=0, rows_b=0):
+        if variant == 8:
+            raise ValueError("block too wide for the flat y-face ranges")
+        state["launch"] = (variant, r, g, w, rows_b)
+
+    plan.set_launch = set_launch
+
+    def timing(p, steps):
+        p.U.add_(1.0)          # a trial advances the state
+        return 1.0 - 0.5 * (state["launch"][0] == 6)
+
+    monkeypatch.setattr(AT, "_time_steps", timing)
+    best = AT.autotune(plan, steps=1, path=tmp_path / "tuning.json")
+    assert best["variant"] == 6
+    assert best["skipped"] and all(c["variant"] == 8 for c in best["skipped"])
+    assert torch.equal(plan.U, torch.arange(8, dtype=torch.float64))
+    assert state["resets"][-1] == (0.5, 3)
+
+    def failing(p, steps):
+        p.U.add_(1.0)
+        raise RuntimeError("trial failed")
+
+    monkeypatch.setattr(AT, "_time_steps", failing)
+    with pytest.raises(RuntimeError):
+        AT.autotune(plan, steps=1, path=None)
+    assert torch.equal(plan.U, torch.arange(8, dtype=torch.float64))
+    assert state["resets"][-1] == (0.5, 3)
+
+
+def test_candidates_include_persistent_grids():
+    cands = AT.candidates()
+    assert {c["grid_waves"] for c in cands if c["variant"] == 0} == {0, 1024, 2048, 3072}
+    assert {c["variant"] for c in cands if c["grid_waves"] > 0} == {0, 6, 7, 8}

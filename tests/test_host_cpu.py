@@ -176,7 +176,8 @@ def test_config_validation_errors():
     cfg, _ = W.explosion_single(8)
     validate_config(cfg)
     for attr, val, exc in (("fvm_num_quadrature_points", 4, KeyError), ("fvm_slope_limiter_type", "Minmod", ValueError),
-                           ("fvm_flux_function_type", "HLLE", ValueError), ("fvm_spatial_order", 1, ValueError)):
+                           ("fvm_flux_function_type", "HLLE", ValueError), ("fvm_spatial_order", 1, ValueError),
+                           ("use_JIT", False, ValueError)):
         old = getattr(cfg, attr)
         setattr(cfg, attr, val)
         with pytest.raises(exc):
@@ -202,3 +203,29 @@ def test_backend_butcher_tables_equal_the_oracle_tables():
         else:
             assert butcher_table(name) == O.BUTCHER[name], name
     assert butcher_table("SSPRK2") == O.generic2(1)
+
+
+def test_mesh_dict_recovered_from_block_descriptions():
+    """The reference's solver keeps BlockDescription objects only (solvers/base.py:90-100); the
+    binding in INTEGRATION.md recovers the mesh dictionary from their info / geometry.vertices /
+    neighbors / bc attributes.  Stand-ins with the same attribute structure round-trip the 2x4 mesh."""
+    from types import SimpleNamespace as NS
+
+    from pyhype_amd.mesh import mesh_dict_from_block_descriptions
+
+    _, gen = W.explosion_multiblock(8)
+    m = mesh_dict(gen)
+    descs = {}
+    for n, d in m.items():
+        side = lambda p: NS(E=d[p + "E"], W=d[p + "W"], N=d[p + "N"], S=d[p + "S"])
+        bc = side("BCType")
+        for c in ("NE", "NW", "SE", "SW"):
+            setattr(bc, c, d.get("BCType" + c))
+        descs[n] = NS(info=NS(nBLK=d["nBLK"]), geometry=NS(vertices=NS(NE=d["NE"], NW=d["NW"], SE=d["SE"], SW=d["SW"])),
+                      neighbors=side("Neighbor"), bc=bc)
+    got = mesh_dict_from_block_descriptions(descs)
+    for n, d in m.items():
+        for k, v in d.items():
+            if k.startswith("Neighbor") and len(k) == 10:   # corner neighbours: BlockInfo keeps none
+                continue
+            assert got[n][k] == v, (n, k)

@@ -117,3 +117,30 @@ def test_cell_dt(numerics_shim):
     out = np.zeros(77)
     numerics_shim.shim_cell_dt(77, dp(U), dp(g.dx), dp(g.dy), dp(out), dp(PHYS))
     assert 0.8 * out.min() == O.block_dt(b, 0.8)
+
+
+def test_hlll_norm_is_blas_dnrm2_bit_for_bit(numerics_shim):
+    """The kernels' nrm4 (64-bit-significand emulation of the x87 dnrm2 that Numba's np.linalg.norm
+    reaches in flux/HLLL.py:86-93) equals scipy.linalg.blas.dnrm2 bit for bit on 10^6 seeded vectors
+    over twelve decades, the dU / dF - u dU vectors of HLLL faces, exact squares and tie-prone integer
+    vectors, subnormal and huge magnitudes (squares outside the double range), zeros and signed zeros."""
+    from scipy.linalg import blas
+    rng = np.random.default_rng(31)
+    X = [rng.standard_normal((1_000_000, 4)) * 10.0 ** rng.uniform(-6, 6, (1_000_000, 1))]
+    WL, WR = face_pairs(rng, 20000)
+    UL = O.prim_to_cons(WL.reshape(1, -1, 4))[0]
+    UR = O.prim_to_cons(WR.reshape(1, -1, 4))[0]
+    dU = UR - UL
+    X.append(dU)
+    X.append((O.prim_F(WR.reshape(1, -1, 4)) - O.prim_F(WL.reshape(1, -1, 4)))[0] - rng.uniform(-2, 2, (20000, 1)) * dU)
+    X.append(np.ldexp(rng.integers(1, 2 ** 30, (20000, 4)).astype(float), rng.integers(-40, 40, (20000, 1))))
+    for scale in (1e-300, 1e-165, 1e-150, 1e150, 1e200, 1e300):
+        X.append(rng.standard_normal((2000, 4)) * scale)
+    X.append(np.array([[0.0, 0.0, 0.0, 0.0], [-0.0, 0.0, -0.0, 0.0], [3.0, 4.0, 0.0, 0.0], [0.0, 0.0, 0.0, 5e-324],
+                       [1.7e308, 0.0, 0.0, 0.0], [1.0, 1.0, 1.0, 1.0]]))
+    X = np.ascontiguousarray(np.vstack(X))
+    ref = np.array([blas.dnrm2(x) for x in X])
+    got = np.zeros(len(X))
+    numerics_shim.shim_nrm4(len(X), dp(X), dp(got))
+    bad = np.nonzero(got.view(np.int64) != ref.view(np.int64))[0]
+    assert bad.size == 0, f"{bad.size} of {len(X)} differ, e.g. {X[bad[:3]]}"

@@ -144,7 +144,33 @@ def test_limiter_on_one_dimensional_linear_field_is_three_quarters():
     np.testing.assert_array_equal(phi[inner][..., 3], 0.75)
 
 
+def test_limiter_delta_is_reconstructed_value_minus_average():
+    """limiters/base.py:96-107 with fvm/SecondOrderMUSCL.py:86-94: the slope's delta is
+    (U + hot) - U.  On a uniform state with hot = 1e-20 (far below half an ulp of U = 1) that delta
+    is exactly 0, so s = 1 and phi = 0.75 -- a delta taken as hot itself would give s = dmax/hot = 0
+    and phi = 0."""
+    U = np.ones((3, 4, 4))
+    ghost = {"E": U[:, -1:].copy(), "W": U[:, :1].copy(), "N": U[-1:].copy(), "S": U[:1].copy()}
+    hot = {d: np.full(U.shape, 1e-20) for d in O.DIRS}
+    np.testing.assert_array_equal(O.limiter_phi(U, ghost, hot), 0.75)
+
+
 # ------------------------------------------------------------------ fluxes
+def test_physical_flux_closed_form():
+    """PrimitiveState._F_from_prim_JIT (states/primitive.py:223-237): W = (2, 0.5, -1.5, 3) gives
+    F = (rho u, rho u^2 + p, rho u v, u (p/(gamma-1) + rho (u^2+v^2)/2 + p)) = (1, 3.5, -1.5, 6.5)."""
+    F = O.prim_F(_face(2.0, 0.5, -1.5, 3.0))[0, 0]
+    np.testing.assert_allclose(F, [1.0, 3.5, -1.5, 6.5], rtol=1e-15)
+
+
+def test_conservative_primitive_round_trip():
+    """states/converter/concrete_defs.py:85-141: W = (2, 0.5, -1.5, 3) -> U = (2, 1, -3,
+    3/(gamma-1) + 2.5) -> W again, with both velocity components in the kinetic energy."""
+    U = O.prim_to_cons(_face(2.0, 0.5, -1.5, 3.0))
+    np.testing.assert_allclose(U[0, 0], [2.0, 1.0, -3.0, 3.0 / 0.4 + 2.5], rtol=1e-15)
+    np.testing.assert_allclose(O.cons_to_prim(U)[0, 0], [2.0, 0.5, -1.5, 3.0], rtol=1e-14)
+
+
 def _prim(rng, n, umax=1.0):
     return np.stack([rng.uniform(0.5, 3, n), rng.uniform(-umax, umax, n), rng.uniform(-umax, umax, n),
                      rng.uniform(0.5, 3, n)], axis=1).reshape(1, n, 4)
@@ -323,11 +349,13 @@ def test_rk_stability_polynomial(monkeypatch, name, poly, rtol):
 
 
 def test_dt_closed_form_on_uniform_state():
-    sim = O.Simulation(mesh1(), 20, 10, CFL=0.8)
-    U = O.prim_to_cons(np.tile(np.array([1.4, 0.3, -0.1, 1.0]), (10, 20, 1)))
+    """QuadBlock.get_dt (blocks/quad_block.py:423-436): CFL min(dx / (|u| + a), dy / (|v| + a)) on
+    cells of 0.5 x 0.25 (dx != dy, so a swap of the two is caught)."""
+    sim = O.Simulation(mesh1(), 20, 20, CFL=0.8)
+    U = O.prim_to_cons(np.tile(np.array([1.4, 0.3, -0.1, 1.0]), (20, 20, 1)))
     sim.set_state({0: U})
     a = math.sqrt(1.4 * 1.0 / 1.4)
-    np.testing.assert_allclose(sim.get_dt(), 0.8 * min(0.5 / (0.3 + a), 0.5 / (0.1 + a)), rtol=1e-14)
+    np.testing.assert_allclose(sim.get_dt(), 0.8 * min(0.5 / (0.3 + a), 0.25 / (0.1 + a)), rtol=1e-14)
 
 
 def test_ghost_fill_neighbours_and_walls():
@@ -492,3 +520,151 @@ def test_gauss_face_states_pair_by_point(nq):
         assert np.max(np.abs(st["N"][i][:-1] - st["S"][i][1:])[2:-3, 2:-2]) < 1e-13
         if nq == 2:
             assert np.max(np.abs(st["E"][i][:, :-1] - st["W"][1 - i][:, 1:])[inner]) > 1e-3
+
+
+# ------------------------------------------------------------------ reference quirks, closed forms
+# Each expected value below is derived by hand from the cited reference lines (not by running the
+# oracle's own formula); tests/test_oracle_mutations.py applies the textbook misreading of each
+# quirk to the oracle and asserts that the pin written for it fails.
+def _face(*w):
+    return np.array(w, dtype=float).reshape(1, 1, 4)
+
+
+def test_roe_state_pressure_is_sqrt_rho_weighted():
+    """B.1, states/primitive.py:301-320: p* = (pL sqrt(rhoL) + pR sqrt(rhoR)) / (sqrt(rhoL) + sqrt(rhoR)).
+    With rhoL = rhoR = 4 (sqrt = 2, weights 1/4) that is (pL + pR)/2 = 2 exactly, and u*, v* are the
+    plain means.  The enthalpy-based (textbook) Roe pressure would add (gamma-1)/gamma rho du^2/8."""
+    Q = O.roe_state(_face(4.0, 0.0, 0.3, 1.0), _face(4.0, 0.8, -0.2, 3.0))[0, 0]
+    assert Q[0] == 4.0 and Q[1] == 0.4 and Q[3] == 2.0
+    assert abs(Q[2] - 0.05) < 1e-16
+
+
+def test_roe_mass_flux_uses_sqrt_rho_weighted_pressure():
+    """B.1 through the flux: WL = (4, 0, 0, 1), WR = (4, 0.1, 0, 1).  dW = (0, 0.1, 0, 0), so only the
+    acoustic strengths -+ rho* du / (2 a*) are non-zero, no Harten fix applies (|u* -+ a*| > 0.2),
+    and the mass flux is 0.5 (0 + 0.4) - 0.5 r2a du (|u*+a*| - |u*-a*|) = 0.2 - 0.01 / a* with
+    u* = 0.05, p* = 1, a* = sqrt(1.4 * 1 / 4)."""
+    F = O.flux_roe(_face(4.0, 0.0, 0.0, 1.0), _face(4.0, 0.1, 0.0, 1.0))[0, 0]
+    a = math.sqrt(1.4 / 4.0)
+    assert abs(F[0] - (0.2 - 0.01 / a)) < 1e-15
+
+
+def test_roe_symmetric_expansion_is_central():
+    """B.2, flux/Roe.py:175-220,292-304: Rc carries the Harten-corrected speeds.  Symmetric
+    expansion WL = (1.4, -0.5, 0, 1), WR = (1.4, 0.5, 0, 1), a = 1: u* = v* = 0, both speeds are
+    corrected to the same lambda_c = 0.5 (a*^2/2 + 2) (|-+a*| < 2 (R - L) = 2), the strengths are
+    -+ r2a, so Rc |Lambda| Lp dW = lambda_c r2a ((-1)(1, lambda_c, 0, H) + (1, lambda_c, 0, H)) = 0
+    exactly: the flux is the central one, (0, rho w^2 + p, 0, 0).  With the uncorrected u -+ a in Rc
+    the x-momentum would lose lambda_c a* r2a du."""
+    F = O.flux_roe(_face(1.4, -0.5, 0.0, 1.0), _face(1.4, 0.5, 0.0, 1.0))[0, 0]
+    assert F[0] == 0.0 and F[2] == 0.0 and F[3] == 0.0
+    assert F[1] == (1.4 * 0.5) * 0.5 + 1.0
+
+
+def test_roe_shear_strength_is_dv():
+    """B.15, flux/Roe.py:222-251 with flux/eigen_system.py:52-57: the (2,2) entry of Lp keeps the 1.0
+    of np.ones, so a pure shear jump (rho 4, u 0.5, p 1; v 0.25 | 1.25) has strength dv = 1 (not
+    rho* dv = 4).  Only the shear column (0, 0, 1, v*) of Rc is active, |lambda| = u* = 0.5,
+    v* = 0.75: the dissipation is 0.5 * 0.5 * (0, 0, 1, 0.75) = (0, 0, 0.25, 0.1875)."""
+    WL, WR = _face(4.0, 0.5, 0.25, 1.0), _face(4.0, 0.5, 1.25, 1.0)
+    F = O.flux_roe(WL, WR)[0, 0]
+    central = 0.5 * (O.prim_F(WL) + O.prim_F(WR))[0, 0]
+    np.testing.assert_allclose(central - F, [0.0, 0.0, 0.25, 0.1875], rtol=0, atol=2e-15)
+    assert F[2] == 1.25      # 0.5 (0.5 + 2.5) - 0.25
+
+
+def test_hlll_signal_speeds_carry_the_harten_fix():
+    """B.6, flux/HLLL.py:27-37 with flux/base.py:119-146: symmetric expansion of speed w = 0.9 a
+    (rho 1.4, p 1, a = 1).  The Roe speeds -+1 are corrected to lambda_c = 1/(8 w) + 2 w
+    (= 1.93889 > w + a), so S+ = max(w + a, lambda_c) = lambda_c while S- = min(-w - a,
+    +lambda_c) = -1.9.  alpha = max(0, 1 - |dF| / (a |dU|)) = 0 and u* = 0, so the flux is HLL:
+    mass = rho w (w + a - lambda_c) / (lambda_c + w + a) = -0.0127637..., not 0."""
+    w = 0.9
+    lam = 1.0 / (8 * w) + 2 * w
+    F = O.flux_hlll(_face(1.4, -w, 0.0, 1.0), _face(1.4, w, 0.0, 1.0))[0, 0]
+    expect = 1.4 * w * (w + 1.0 - lam) / (lam + w + 1.0)
+    assert abs(F[0] - expect) < 1e-11 * abs(expect)     # (w + a - lambda_c) cancels 2 digits
+
+
+def test_hlll_resolves_a_moving_contact_exactly():
+    """flux/HLLL.py:84-101: across a contact (rho 1 | 2, u 0.3, v 0.1, p 1) F(U) = u U + (0, p, 0, u p),
+    so dF = u dU and |dF - u* dU| ~ 0: alpha = 1 and the coefficient is S- S+ max(u/S-, u/S+) =
+    S- u, which makes the HLLL flux equal the upwind flux F(WL) (u > 0)."""
+    WL, WR = _face(1.0, 0.3, 0.1, 1.0), _face(2.0, 0.3, 0.1, 1.0)
+    np.testing.assert_allclose(O.flux_hlll(WL, WR), O.prim_F(WL), rtol=1e-13, atol=1e-15)
+
+
+def test_nrm2_is_blas_dnrm2_bit_for_bit():
+    """flux/HLLL.py:86-93: Numba's np.linalg.norm is the BLAS dnrm2 of its SciPy.  The oracle's nrm2
+    equals scipy.linalg.blas.dnrm2 bit for bit on seeded vectors of HLLL magnitudes, on the dU and
+    dF - u dU vectors of random faces, and on exact-square and tie-prone integer vectors."""
+    from scipy.linalg import blas
+    rng = np.random.default_rng(2024)
+    X = [rng.standard_normal((40000, 4)) * 10.0 ** rng.uniform(-6, 6, (40000, 1))]
+    X.append(np.ldexp(rng.integers(1, 2 ** 26, (5000, 4)).astype(float), rng.integers(-20, 20, (5000, 1))))
+    X.append(np.array([[3.0, 4.0, 0.0, 0.0], [0.0, 0.0, 0.0, 0.0], [1.0, 1.0, 1.0, 1.0], [-0.0, 2.0, 0.0, 0.0]]))
+    W = _prim(rng, 5000, 2.0)
+    WR = _prim(rng, 5000, 2.0)
+    UL, UR = O.prim_to_cons(W), O.prim_to_cons(WR)
+    dU = (UR - UL)[0]
+    X.append(dU)
+    X.append((O.prim_F(WR) - O.prim_F(W))[0] - rng.uniform(-1, 1, (5000, 1)) * dU)
+    X = np.vstack(X)
+    ref = np.array([blas.dnrm2(x) for x in X])
+    got = O.nrm2(X)
+    assert np.array_equal(got.view(np.int64), ref.view(np.int64))
+
+
+def _two_blocks(nx=6, ny=4):
+    """Blocks 0 | 1 on [0,5]x[0,4] and [5,10]x[0,4], joined at x = 5, walls elsewhere."""
+    walls = "Reflection"
+    def blk(n, x0, nb):
+        d = {"nBLK": n, "SW": [x0, 0.0], "SE": [x0 + 5.0, 0.0], "NW": [x0, 4.0], "NE": [x0 + 5.0, 4.0]}
+        for s in O.DIRS:
+            d["Neighbor" + s] = nb.get(s)
+            d["BCType" + s] = None if s in nb else walls
+        return d
+    return {0: blk(0, 0.0, {"E": 1}), 1: blk(1, 5.0, {"W": 0})}
+
+
+def test_interblock_face_takes_the_neighbour_cell_average():
+    """B.3, fvm/base.py:305-325: at a block interface the outer state is the neighbour's adjacent
+    cell average (first order).  Block 0 at rest in W0 = (1, 0, 0, 1), block 1 at rest in
+    W1 = (2, 0, 0, 3): block 0's last-column cells have phi = 0 (the west face's slope is
+    dmin/d = 0 or dmax/d = 0), so their limited east value is W0, and their residual is
+    L (F(W0) - Roe(W0, W1)) / A (the N/S fluxes cancel), everything else in block 0 is at rest."""
+    m = _two_blocks()
+    sim = O.Simulation(m, 6, 4)
+    W0 = np.tile([1.0, 0.0, 0.0, 1.0], (4, 6, 1))
+    W1 = np.tile([2.0, 0.0, 0.0, 3.0], (4, 6, 1))
+    sim.set_state({0: O.prim_to_cons(W0), 1: O.prim_to_cons(W1)})
+    R = O.residual(sim.blocks[0], sim.flux_fn)
+    g = sim.blocks[0].geom
+    Fc = O.prim_F(W0[:1, :1])[0, 0]
+    Fr = O.flux_roe(W0[:1, :1].copy(), W1[:1, :1].copy())[0, 0]
+    expect = g.face["E"].L[0, -1] * (Fc - Fr) / g.A[0, -1]
+    for i in range(4):
+        np.testing.assert_allclose(R[i, -1], expect, rtol=1e-13, atol=1e-12 * np.abs(expect).max())
+    assert np.max(np.abs(R[:, :-1])) < 1e-12 * np.abs(expect).max()
+    assert abs(expect[0]) > 0.1 and abs(expect[1]) > 0.1
+
+
+def test_interblock_residual_ignores_the_neighbours_second_column():
+    """B.3: block 0 sees block 1 only through block 1's first column (ghost cells and the outer
+    state), so changing block 1's second column leaves block 0's residual bit-identical, while a
+    change of its first column does not (a second-order outer state would read the second column
+    through block 1's gradient)."""
+    m = _two_blocks()
+    rng = np.random.default_rng(11)
+    sim = O.Simulation(m, 6, 4)
+    st = {n: O.prim_to_cons(_prim(rng, 24).reshape(4, 6, 4)) for n in m}
+    sim.set_state(st)
+    R = O.residual(sim.blocks[0], sim.flux_fn)
+    st2 = {n: v.copy() for n, v in st.items()}
+    st2[1][:, 1] *= 1.05
+    sim.set_state(st2)
+    assert np.array_equal(O.residual(sim.blocks[0], sim.flux_fn), R)
+    st3 = {n: v.copy() for n, v in st.items()}
+    st3[1][:, 0] *= 1.05
+    sim.set_state(st3)
+    assert not np.array_equal(O.residual(sim.blocks[0], sim.flux_fn), R)
