@@ -54,6 +54,10 @@ int dt_slots_for(int ncases) {
     return n;
 }
 
+// Every per-item function below is __forceinline__, so that no kernel makes a device-function call.
+// Outlined (s_swappc) item functions of the HLLL Gauss-face and generic shared-halo kernels, which
+// appeared when the HLLL norm grew, returned NaN residuals on gfx950 (ROCm 7.2) where the same
+// source inlined, and a stand-alone kernel of the same flux, are bit-exact.
 struct StageArgs {
     const DevBlock* blocks;
     const WorkItem* items;
@@ -166,7 +170,7 @@ __device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& 
 // one work item of one RK stage
 // ---------------------------------------------------------------------------------------------
 template <bool CART, int FLUX, bool GEN>
-__device__ void stage_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     // a case that became unrealizable is frozen: the reference aborts that run (Euler2D.py:144-152)
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
@@ -286,7 +290,7 @@ __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
 constexpr int WGW = 4;
 
 template <bool CART, int FLUX, bool GEN>
-__device__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, Q4* lds_F) {
+__device__ __forceinline__ void stage_wg_item(const StageArgs& A, const WorkItem it, Q4* lds_N, Q4* lds_F) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int lane = threadIdx.x & (WAVE - 1);
@@ -486,7 +490,7 @@ __device__ __forceinline__ void emit_half(const StageArgs& A, const DevBlock& B,
 }
 
 template <bool CART, int FLUX, bool GEN, int H>
-__device__ void stage_pair_body(const StageArgs& A, const WorkItem it, PairLds& L) {
+__device__ __forceinline__ void stage_pair_body(const StageArgs& A, const WorkItem it, PairLds& L) {
     const DevBlock& B = const_ref(A.blocks, it.block);
     const int lane = threadIdx.x & (WAVE - 1);
     const int nx = B.nx, ny = B.ny;
@@ -667,7 +671,7 @@ __device__ __forceinline__ void acc_w(Q4& acc, double w, const Q4& F) {
 }
 
 template <bool CART, int FLUX>
-__device__ void recon_x_item_q(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void recon_x_item_q(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx;
@@ -736,7 +740,7 @@ __device__ void recon_x_item_q(const StageArgs& A, const WorkItem it) {
 
 // Gauss sum S_F = sum_i w_i F_i (from 0) of y-face f of column j
 template <bool CART, int FLUX>
-__device__ Q4 yface_sum_q(const StageArgs& A, const DevBlock& B, const double* sn, int f, int j, const Phys& ph) {
+__device__ __forceinline__ Q4 yface_sum_q(const StageArgs& A, const DevBlock& B, const double* sn, int f, int j, const Phys& ph) {
     const int nx = B.nx, ny = B.ny;
     const int nq = ph.nq;
     // N states of row f-1 (f > 0) and S states of row f (f < ny)
@@ -760,7 +764,7 @@ __device__ Q4 yface_sum_q(const StageArgs& A, const DevBlock& B, const double* s
 }
 
 template <bool CART, int FLUX>
-__device__ void yface_item_q(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void yface_item_q(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx;
@@ -773,7 +777,7 @@ __device__ void yface_item_q(const StageArgs& A, const WorkItem it) {
 
 // Gauss faces, y-faces and update in one pass (variant 5; yface_update_item with the sums S_F)
 template <bool CART, int FLUX>
-__device__ void yface_update_item_q(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void yface_update_item_q(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -802,7 +806,7 @@ __device__ void yface_update_item_q(const StageArgs& A, const WorkItem it) {
 }
 
 template <bool CART, int FLUX, bool GEN>
-__device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void recon_x_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = const_ref(A.blocks, it.block);
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx;
@@ -857,7 +861,7 @@ __device__ void recon_x_item(const StageArgs& A, const WorkItem it) {
 // y-faces of face rows [r0, r1) of a strip of 64 columns (face f between rows f-1 and f; f = 0 and
 // f = ny take the outer states of the S and N boundaries): one flux per lane and face, stored.
 template <bool CART, int FLUX, bool GEN>
-__device__ void yface_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void yface_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx, ny = B.ny;
@@ -878,7 +882,7 @@ __device__ void yface_item(const StageArgs& A, const WorkItem it) {
 // residual ((x-part + LS*(2 F_S)) - LN*(2 F_N))/A (Gauss sums S_F in place of 2 F with nq > 1) of rows [r0, r1) of a 64-column strip, then every
 // Butcher output, the CFL quantity and realizability (as the fused kernel).
 template <bool CART>
-__device__ void update_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void update_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -928,7 +932,7 @@ __device__ __forceinline__ Q4 yface_flux_at(const StageArgs& A, const DevBlock& 
 }
 
 template <bool CART, int FLUX, bool GEN>
-__device__ void yface_update_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void yface_update_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -965,7 +969,7 @@ constexpr int YU_MAX_ROWS = 7;  // rows per item (R + 1 <= 8 waves per workgroup
 
 // Q = true: Gauss faces (the sums S_F of yface_sum_q, the residual of the nq-point rule)
 template <bool CART, int FLUX, bool GEN, bool Q>
-__device__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* lds) {
+__device__ __forceinline__ void yface_update_wg_item(const StageArgs& A, const WorkItem it, Q4* lds) {
     const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int lane = threadIdx.x & (WAVE - 1);
@@ -1022,7 +1026,7 @@ __global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(St
 // range).  T = (A.rows_wave + 1) * WAVE; the host sizes the ranges so that n + nx <= A.fq * T, the
 // faces the dynamic LDS holds.
 template <bool CART, int FLUX, bool GEN>
-__device__ void yface_update_flat_item(const StageArgs& A, const WorkItem it, Q4* lds) {
+__device__ __forceinline__ void yface_update_flat_item(const StageArgs& A, const WorkItem it, Q4* lds) {
     const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int T = blockDim.x;
@@ -1096,7 +1100,7 @@ __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
 constexpr int LIN_OWN = WAVE - 2;
 
 template <bool CART, int FLUX, bool GEN>
-__device__ void recon_x_lin_item(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void recon_x_lin_item(const StageArgs& A, const WorkItem it) {
     const DevBlock B = ld_const(A.blocks, it.block);  // descriptor in scalar registers
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int nx = B.nx, npos = B.ny * (nx + 1);
@@ -1219,7 +1223,7 @@ __device__ __forceinline__ double ew_value(const StageArgs& A, const DevBlock& B
 }
 
 template <bool CART, bool GEN>
-__device__ void ew_recon_tile(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void ew_recon_tile(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int c = (int)threadIdx.x >> 2, v = (int)threadIdx.x & 3;
@@ -1243,7 +1247,7 @@ __device__ void ew_recon_tile(const StageArgs& A, const WorkItem it) {
 }
 
 template <bool CART, int FLUX, bool GEN>
-__device__ void ew_face_tile(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void ew_face_tile(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     if ((int)threadIdx.x >= it.r0) return;
@@ -1258,7 +1262,7 @@ __device__ void ew_face_tile(const StageArgs& A, const WorkItem it) {
 }
 
 template <bool CART>
-__device__ void ew_update_tile(const StageArgs& A, const WorkItem it) {
+__device__ __forceinline__ void ew_update_tile(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = A.blocks[it.block];
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;  // uniform over the workgroup
     const int t = (int)threadIdx.x;
