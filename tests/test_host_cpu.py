@@ -43,6 +43,26 @@ def test_library_loads_and_exports_every_declared_function():
     assert set(names) == set(nat.EXPORTS)
 
 
+LLVM_BIN = Path("/opt/rocm/llvm/bin")
+
+
+@pytest.mark.skipif(not (LLVM_BIN / "llvm-objdump").exists(), reason="ROCm LLVM tools not available")
+def test_device_code_makes_no_function_calls(tmp_path):
+    """Every kernel of the built gfx950 code object is call-free (no s_swappc_b64): the per-item
+    functions are force-inlined, because the outlined HLLL Gauss-face kernels returned NaN."""
+    fatbin, code = tmp_path / "fatbin.bin", tmp_path / "gfx950.o"
+    subprocess.run([str(LLVM_BIN / "llvm-objcopy"), f"--dump-section=.hip_fatbin={fatbin}", str(nat.LIB_PATH),
+                    str(tmp_path / "host.so")], check=True, capture_output=True)
+    subprocess.run([str(LLVM_BIN / "clang-offload-bundler"), "--unbundle", "--type=o",
+                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={fatbin}", f"--output={code}"],
+                   check=True, capture_output=True)
+    dis = subprocess.run([str(LLVM_BIN / "llvm-objdump"), "-d", "--mcpu=gfx950", str(code)], check=True,
+                         capture_output=True, text=True).stdout
+    assert dis.count("s_endpgm") > 50
+    calls = [ln.strip() for ln in dis.splitlines() if "s_swappc" in ln]
+    assert not calls, f"{len(calls)} device-function calls, e.g. {calls[:2]}"
+
+
 def test_python_constants_match_the_header():
     d = _defines()
     assert d["PHX_ABI_VERSION"] == nat.PHX_ABI_VERSION
