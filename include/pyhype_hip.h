@@ -189,6 +189,22 @@ int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host,
  * face).  No counterpart in the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
 
+/* Size of the static device work array of phx_residual_vjp_noalloc (per GPU). */
+#define PHX_NOALLOC_SCRATCH_BYTES (32u * 1024u * 1024u)
+
+/* phx_residual_vjp that allocates nothing: the adjoint work arrays live in one static device array
+ * of PHX_NOALLOC_SCRATCH_BYTES per GPU (held by every GPU that loads the library until the process
+ * ends; its address is taken once per device at plan creation) instead of arrays the plan
+ * allocates on its first adjoint call.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.  Same kernels, so the
+ * result equals phx_residual_vjp's bit for bit.  Needs 16*nq + 20 doubles per cell plus 8*nq per
+ * face (nq Gauss points per face): a plan that needs more than the array returns PHX_E_INVALID
+ * and phx_last_error() gives the size.  Every call on a GPU uses that GPU's one array: calls that
+ * overlap in time on one GPU (other streams, other threads, other plans) race and must be ordered
+ * by the caller.  Nothing in the array has to start at zero (each call writes every element it
+ * reads).  The plan's device must be current.  No counterpart in the reference. */
+int phx_residual_vjp_noalloc(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
+
 /* Reverse-mode product of one Runge-Kutta step with a frozen time step: on entry lam holds the
  * adjoint of the state after the step, on return the adjoint of the state before it.  stages:
  * the nstages stage-input states of that step as written by phx_run_record (device,

@@ -15,6 +15,7 @@
 
 #include <string.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -171,17 +172,18 @@ int upload(const std::vector<T>& h, T** d) {
 
 const int OPP[4] = {PHX_W, PHX_E, PHX_S, PHX_N};
 
-// Work arrays and launch tiles of the adjoint; requires neighbour links to be mutual.
-int adj_setup(phx_plan* P) {
-    if (P->adj_ready) return PHX_OK;
+// Per-block offsets and launch tiles of the adjoint; requires neighbour links to be mutual.  Run
+// once, when the plan is created.
+int adj_meta(phx_plan* P) {
     for (int b = 0; b < P->nblocks; ++b) {
         const phx_block_desc& d = P->desc[b];
         for (int s = 0; s < 4; ++s) {
             if (d.side_type[s] != PHX_SIDE_NEIGHBOR) continue;
             const phx_block_desc& o = P->desc[d.neighbor[s]];
-            if (o.side_type[OPP[s]] != PHX_SIDE_NEIGHBOR || o.neighbor[OPP[s]] != b)
-                return fail(PHX_E_INVALID, "adjoint: block " + std::to_string(b) +
-                                               " has a neighbour link that is not mutual");
+            if (o.side_type[OPP[s]] != PHX_SIDE_NEIGHBOR || o.neighbor[OPP[s]] != b) {
+                P->adj_meta_error = "adjoint: block " + std::to_string(b) + " has a neighbour link that is not mutual";
+                return PHX_OK;  // the forward path does not need mutual links; the adjoint calls refuse the plan
+            }
         }
     }
     std::vector<AdjBlock> offs(P->nblocks);
@@ -207,46 +209,98 @@ int adj_setup(phx_plan* P) {
     if ((rc = upload(offs, &P->dadj))) return rc;
     if ((rc = upload(P->adj_cell_tiles, &P->dadj_cell_tiles))) return rc;
     if ((rc = upload(P->adj_face_tiles, &P->dadj_face_tiles))) return rc;
-    const size_t L = (size_t)P->state_len;
+    P->adj_meta_ready = true;
+    return PHX_OK;
+}
+
+// Work arrays of one residual VJP: limited face states per cell, face adjoints, cell slots.
+struct AdjBufs {
+    double *rb, *face, *slot;
+};
+
+// Doubles of each work array of a plan (cells x RB x nq, faces x FA x nq, cells x SLOT).
+void adj_sizes(const phx_plan* P, size_t n[3]) {
     const size_t nq = (size_t)std::max(P->ph.nq, 1);
-    PHX_CK(hipMalloc(&P->adj_rb, (size_t)P->adj_cells * phx::adj::RB * nq * sizeof(double)));
-    PHX_CK(hipMalloc(&P->adj_face, (size_t)faces * phx::adj::FA * nq * sizeof(double)));
-    PHX_CK(hipMalloc(&P->adj_slot, (size_t)P->adj_cells * phx::adj::SLOT * sizeof(double)));
+    n[0] = (size_t)P->adj_cells * phx::adj::RB * nq;
+    n[1] = (size_t)P->adj_faces * phx::adj::FA * nq;
+    n[2] = (size_t)P->adj_cells * phx::adj::SLOT;
+}
+
+// The plan's own work arrays, allocated on the first adjoint call.
+int adj_setup(phx_plan* P) {
+    if (P->adj_ready) return PHX_OK;
+    if (!P->adj_meta_ready) return fail(PHX_E_INVALID, P->adj_meta_error);
+    size_t n[3];
+    adj_sizes(P, n);
+    const size_t L = (size_t)P->state_len;
+    PHX_CK(hipMalloc(&P->adj_rb, n[0] * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_face, n[1] * sizeof(double)));
+    PHX_CK(hipMalloc(&P->adj_slot, n[2] * sizeof(double)));
     for (int s = 1; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->adj_lam[s], L * sizeof(double)));
     P->adj_ready = true;
     return PHX_OK;
 }
 
+// ---- static work array of phx_residual_vjp_noalloc ----
+// One array per GPU: every device that runs the library holds its own copy until the process
+// ends.  In host code `scratch` names a host-side stand-in, never the device memory: the device
+// address comes from hipGetSymbolAddress, taken once per device at plan creation (with the plan's
+// device current) and kept in scratch_ptr.
+constexpr size_t NOALLOC_DOUBLES = PHX_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[MAX_DEVICES] = {};
+
+int scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(PHX_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return PHX_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(PHX_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    scratch_ptr[device] = static_cast<double*>(p);
+    return PHX_OK;
+}
+
+double* scratch_of(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return scratch_ptr[device];
+}
+
 template <int NQ>
 void launch_residual_vjp_nq(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
-                            hipStream_t st) {
+                            const AdjBufs& w, hipStream_t st) {
     const dim3 blk(TILE), vblk(VTILE);
     const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
     if (NQ == 1)
-        hipLaunchKernelGGL(k_adj_recon_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb, P->ph);
+        hipLaunchKernelGGL(k_adj_recon_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, w.rb, P->ph);
     else
-        hipLaunchKernelGGL(k_adj_recon<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_rb,
+        hipLaunchKernelGGL(k_adj_recon<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, w.rb,
                            P->ph);
     if (P->flux == PHX_FLUX_ROE)
-        hipLaunchKernelGGL((k_adj_face<0, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
-                           kappa, P->adj_face, P->ph);
+        hipLaunchKernelGGL((k_adj_face<0, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, w.rb,
+                           kappa, w.face, P->ph);
     else
-        hipLaunchKernelGGL((k_adj_face<1, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, P->adj_rb,
-                           kappa, P->adj_face, P->ph);
+        hipLaunchKernelGGL((k_adj_face<1, NQ>), gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, w.rb,
+                           kappa, w.face, P->ph);
     if (NQ == 1)
-        hipLaunchKernelGGL(k_adj_cell_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
-                           P->adj_slot, P->ph);
+        hipLaunchKernelGGL(k_adj_cell_v, gc, vblk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, w.face,
+                           w.slot, P->ph);
     else
-        hipLaunchKernelGGL(k_adj_cell<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, P->adj_face,
-                           P->adj_slot, P->ph);
-    hipLaunchKernelGGL(k_adj_gather<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_slot,
-                       P->adj_face, U, P->ph, extra, lamU);
+        hipLaunchKernelGGL(k_adj_cell<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, w.face,
+                           w.slot, P->ph);
+    hipLaunchKernelGGL(k_adj_gather<NQ>, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, w.slot,
+                       w.face, U, P->ph, extra, lamU);
 }
 
 void launch_residual_vjp(phx_plan* P, const double* U, const Terms& kappa, const Terms& extra, double* lamU,
-                         hipStream_t st) {
-    if (P->ph.nq > 1) launch_residual_vjp_nq<phx::QMAX>(P, U, kappa, extra, lamU, st);
-    else launch_residual_vjp_nq<1>(P, U, kappa, extra, lamU, st);
+                         const AdjBufs& w, hipStream_t st) {
+    if (P->ph.nq > 1) launch_residual_vjp_nq<phx::QMAX>(P, U, kappa, extra, lamU, w, st);
+    else launch_residual_vjp_nq<1>(P, U, kappa, extra, lamU, w, st);
 }
 
 
@@ -263,6 +317,13 @@ void phx::host::adjoint_release(phx_plan* P) {
     P->adj_rb = P->adj_face = P->adj_slot = nullptr;
     for (int s = 0; s < PHX_MAX_STAGES; ++s) P->adj_lam[s] = nullptr;
     P->adj_ready = false;
+    P->adj_meta_ready = false;
+}
+
+int phx::host::adjoint_prepare(phx_plan* P) {
+    const int rc = adj_meta(P);
+    if (rc) return rc;
+    return scratch_setup(P->device);
 }
 
 extern "C" {
@@ -277,7 +338,44 @@ int phx_residual_vjp(phx_plan* P, const double* U, const double* lamR, double* l
     seed.in[0] = lamR;
     seed.c[0] = 1.0;
     seed.n = 1;
-    launch_residual_vjp(P, U, seed, none, lamU, (hipStream_t)stream);
+    launch_residual_vjp(P, U, seed, none, lamU, AdjBufs{P->adj_rb, P->adj_face, P->adj_slot}, (hipStream_t)stream);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+// phx_residual_vjp with its work arrays in the static array `scratch` (no allocation at all): the
+// same kernels on the same launch tiles, so the result is bit-identical.  Every array element the
+// kernels read is written earlier in the same call, so nothing has to be cleared.
+int phx_residual_vjp_noalloc(phx_plan* P, const double* U, const double* lamR, double* lamU, void* stream) {
+    if (!P || !U || !lamR || !lamU) return fail(PHX_E_INVALID, "null argument");
+    if (!P->adj_meta_ready) return fail(PHX_E_INVALID, P->adj_meta_error);
+    int dev = -1;
+    PHX_CK(hipGetDevice(&dev));
+    if (dev != P->device)
+        return fail(PHX_E_INVALID, "phx_residual_vjp_noalloc: the current device is not the plan's device");
+    double* base = scratch_of(P->device);
+    if (!base) return fail(PHX_E_STATE, "phx_residual_vjp_noalloc: static work array not set up on this device");
+    size_t n[3];
+    adj_sizes(P, n);
+    const size_t align = 256 / sizeof(double);
+    size_t off[3], total = 0;
+    for (int i = 0; i < 3; ++i) {
+        off[i] = total;
+        total += (n[i] + align - 1) / align * align;
+    }
+    if (total > NOALLOC_DOUBLES)
+        return fail(PHX_E_INVALID, "phx_residual_vjp_noalloc: the plan needs " + std::to_string(total * sizeof(double)) +
+                                       " bytes of adjoint work arrays, more than the static array's " +
+                                       std::to_string((size_t)PHX_NOALLOC_SCRATCH_BYTES) +
+                                       " (use phx_residual_vjp)");
+    Terms seed, none;
+    memset(&seed, 0, sizeof(seed));
+    memset(&none, 0, sizeof(none));
+    seed.in[0] = lamR;
+    seed.c[0] = 1.0;
+    seed.n = 1;
+    launch_residual_vjp(P, U, seed, none, lamU, AdjBufs{base + off[0], base + off[1], base + off[2]},
+                        (hipStream_t)stream);
     PHX_CK(hipGetLastError());
     return PHX_OK;
 }
@@ -295,6 +393,7 @@ int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double
     const size_t L = (size_t)P->state_len;
     const double* a = P->butcher.data();
     auto lam_of = [&](int l) -> const double* { return l == S ? lam : P->adj_lam[l]; };
+    const AdjBufs W{P->adj_rb, P->adj_face, P->adj_slot};
     for (int i = S - 1; i >= 0; --i) {
         // seed kappa_i, evaluated inside the face kernel
         Terms kap, extra;
@@ -310,7 +409,7 @@ int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double
         }
         kap.dt = dt_dev;
         if (i > 0) {
-            launch_residual_vjp(P, stages + (size_t)i * L, kap, extra, P->adj_lam[i], st);
+            launch_residual_vjp(P, stages + (size_t)i * L, kap, extra, P->adj_lam[i], W, st);
         } else {
             // lambda_0 = lambda_S + sum_{l=1}^{S-1} lambda_l + (dR/dU(U_0))^T kappa_0, written over lam
             extra.in[extra.n] = lam;
@@ -319,7 +418,7 @@ int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double
                 extra.in[extra.n] = P->adj_lam[l];
                 extra.c[extra.n++] = 1.0;
             }
-            launch_residual_vjp(P, stages, kap, extra, lam, st);
+            launch_residual_vjp(P, stages, kap, extra, lam, W, st);
         }
     }
     PHX_CK(hipGetLastError());

@@ -2120,6 +2120,8 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipGetLastError());
     PHX_CKP(hipDeviceSynchronize());
     PHX_CKP(hipFree(t0));
+    rc = phx::host::adjoint_prepare(P);
+    if (rc) return bail(rc);
     PHX_CKP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
     PHX_CKP(hipEventCreateWithFlags(&P->ev_a, hipEventDisableTiming));
     PHX_CKP(hipEventCreateWithFlags(&P->ev_b, hipEventDisableTiming));

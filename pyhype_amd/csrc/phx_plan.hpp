@@ -115,6 +115,10 @@ struct phx_plan {
     double* graph_U = nullptr;
     bool copy_back = false;  // one-stage tables: result lands in sbuf[0] and is copied to U
 
+    // discrete adjoint: per-block offsets and launch tiles, built when the plan is created (when its
+    // neighbour links are mutual; otherwise adj_meta_error says why not)
+    bool adj_meta_ready = false;
+    std::string adj_meta_error;
     // discrete adjoint work arrays, allocated on the first adjoint call
     bool adj_ready = false;
     long long adj_cells = 0, adj_faces = 0;
@@ -131,5 +135,8 @@ namespace phx {
 namespace host {
 // Frees the adjoint work arrays of a plan (no-op when none were allocated).
 void adjoint_release(phx_plan* P);
+// Set-up of a new plan's adjoint: offsets and launch tiles (when the neighbour links are mutual),
+// and the address of the static work array of phx_residual_vjp_noalloc on the plan's device.
+int adjoint_prepare(phx_plan* P);
 }  // namespace host
 }  // namespace phx
