@@ -60,7 +60,13 @@ def lookup(plan, path: Path = CACHE) -> Optional[dict]:
     return load_cache(path).get(signature(plan))
 
 
+MIN_TRIAL_S = 0.01  # a timed trial lasts at least this long (small meshes: more steps)
+
+
 def _time_steps(plan, steps: int) -> float:
+    """Seconds per step over `steps` graph-replayed steps, or over as many more as make the trial
+    last MIN_TRIAL_S (on small meshes a few steps take well under a millisecond, where launch and
+    synchronisation jitter would decide the ranking)."""
     import torch
 
     plan.run(1, use_graph=True, check=False)
@@ -68,7 +74,14 @@ def _time_steps(plan, steps: int) -> float:
     t0 = time.perf_counter()
     plan.run(steps, use_graph=True, check=False)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps
+    el = time.perf_counter() - t0
+    if el < MIN_TRIAL_S:
+        steps = int(steps * min(64.0, MIN_TRIAL_S / max(el, 1e-6)) + 1)
+        t0 = time.perf_counter()
+        plan.run(steps, use_graph=True, check=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+    return el / steps
 
 
 def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0, 1024, 2048, 3072), waves: Sequence[int] = (2, 3),

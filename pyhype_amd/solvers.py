@@ -88,6 +88,29 @@ def mesh_dict(mesh_config) -> Dict[int, dict]:
     return mesh_config.dict if isinstance(mesh_config, MeshGenerator) else mesh_config
 
 
+class _HostEntry:
+    """The parts of a plan's block entry that HostBlock reads, for host-only set-ups."""
+
+    def __init__(self, num, geom):
+        self.num, self.geom, self.nx, self.ny = num, geom, geom.nx, geom.ny
+
+
+def host_initial_states(config, mesh_config, initial_condition=None) -> Dict[int, np.ndarray]:
+    """Initial conservative state of every block, computed on the host only (no device plan): the
+    same IC protocol as Euler2D.apply_initial_condition (solvers/Euler2D.py:90-92) on blocks whose
+    geometry follows the reference's mesh formulas."""
+    from .mesh import BlockGeometry
+
+    ic = config.initial_condition if initial_condition is None else initial_condition
+    out = {}
+    for n, d in mesh_dict(mesh_config).items():
+        geom = BlockGeometry({k: d[k] for k in ("SW", "SE", "NW", "NE")}, config.nx, config.ny)
+        hb = HostBlock(config, _HostEntry(n, geom))
+        ic.apply_to_block(hb)
+        out[n] = np.array(hb.state.data, dtype=np.float64)
+    return out
+
+
 class Euler2DBatch:
     """Independent cases advanced together.  All cases share flux, limiter, time integrator, CFL and
     fluid; each keeps its own block size, dt (CFL of its own blocks) and time."""
