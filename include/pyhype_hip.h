@@ -189,6 +189,20 @@ int phx_status(phx_plan* plan, int32_t* bad_step, int32_t* step, double* t_host,
  * face).  No counterpart in the reference. */
 int phx_residual_vjp(phx_plan* plan, const double* U, const double* lamR, double* lamU, void* stream);
 
+/* Forward-mode (tangent-linear) product of the residual: dR = (dR/dU) dU at state U, the residual's
+ * own arithmetic evaluated on value-and-derivative pairs, so every limiter, Harten and wave-speed
+ * branch is the forward's (ties as the reverse pass takes them); the transpose of
+ * phx_residual_vjp.  Midpoint faces only (quadrature_points 1; PHX_E_INVALID otherwise).  Uses the
+ * adjoint's work arrays (allocated on the first adjoint or tangent call).  No counterpart in the
+ * reference. */
+int phx_residual_jvp(phx_plan* plan, const double* U, const double* dU, double* dR, void* stream);
+
+/* Forward-mode product of one Runge-Kutta step with a frozen time step: on entry dU holds the
+ * derivative of the state before the step, on return that of the state after it
+ * (dU_i = dU_0 + dt sum_m a[i-1][m] (dR/dU)(U_m) dU_m).  stages, dt_dev: as for phx_step_vjp.  The
+ * first call allocates nstages + 1 state arrays.  No counterpart in the reference. */
+int phx_step_jvp(phx_plan* plan, const double* stages, const double* dt_dev, double* dU, void* stream);
+
 /* Size of the static device work array of phx_residual_vjp_noalloc (per GPU). */
 #define PHX_NOALLOC_SCRATCH_BYTES (32u * 1024u * 1024u)
 

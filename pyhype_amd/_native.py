@@ -27,7 +27,7 @@ EXPORTS = (
     "phx_plan_create", "phx_plan_destroy", "phx_state_len", "phx_reset", "phx_run", "phx_run_frozen",
     "phx_run_record", "phx_stage", "phx_residual", "phx_dt", "phx_ghosts", "phx_status", "phx_residual_vjp",
     "phx_step_vjp", "phx_set_launch", "phx_set_variant", "phx_launch_info", "phx_last_error",
-    "phx_residual_vjp_noalloc",
+    "phx_residual_vjp_noalloc", "phx_residual_jvp", "phx_step_jvp",
 )
 
 
@@ -82,6 +82,8 @@ def load(path: Path = LIB_PATH):
         "phx_status": ([vp, C.POINTER(i32), C.POINTER(i32), dp, dp], C.c_int),
         "phx_residual_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_residual_vjp_noalloc": ([vp, vp, vp, vp, vp], C.c_int),
+        "phx_residual_jvp": ([vp, vp, vp, vp, vp], C.c_int),
+        "phx_step_jvp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_step_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_set_launch": ([vp, i32, i32, i32], C.c_int),
         "phx_set_variant": ([vp, i32, i32], C.c_int),

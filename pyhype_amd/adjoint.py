@@ -13,6 +13,10 @@ result into user objectives.
     traj = Trajectory(plan, dt_sched, checkpoint_every=10)
     U_T = trajectory(U0, traj)                # differentiable in U0
     J = ((rho(U_T) - rho_target) ** 2).sum(); J.backward()
+    dU_T = traj.jvp(U0, dU0)                  # forward mode: (dU_T/dU_0) dU0
+
+The forward-mode product (``Trajectory.jvp``, ``phx_step_jvp``) runs the residual's own arithmetic
+on value-and-derivative pairs; <J v, w> = <v, J^T w> ties it to the reverse sweep.
 """
 from __future__ import annotations
 
@@ -126,6 +130,23 @@ class Trajectory:
             free[b].record(main)
         main.wait_stream(side)          # the plan's state is the caller's again
         return lam
+
+
+    def jvp(self, U0, dU0):
+        """Forward-mode product of the trajectory: the derivative of U_n in direction dU0 at U0.
+        Each segment is run from U0 with its stage inputs recorded (``phx_run_record``), then the
+        step JVPs carry dU through the segment's steps (``phx_step_jvp``).  The plan's state ends
+        at U_n."""
+        P = self.plan
+        torch = P.torch
+        dU = dU0.detach().to(dtype=torch.float64).contiguous().clone()
+        P.U.copy_(U0)
+        for s0 in range(0, self.nsteps, self.every):
+            m = min(self.every, self.nsteps - s0)
+            P.run_record(self.dt[s0:s0 + m], self.stages[:m])
+            for i in range(m):
+                P.step_jvp(self.stages[i], self.dt[s0 + i], dU)
+        return dU
 
 
 _FN = None

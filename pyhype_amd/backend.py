@@ -341,6 +341,19 @@ class DevicePlan:
                   "phx_residual_vjp")
         return lamU
 
+    def residual_jvp(self, U, dU):
+        """(dR/dU) dU at state U (device tensors of the state layout)."""
+        dR = self.torch.zeros_like(U)
+        nat.check(self.lib.phx_residual_jvp(self.handle, self.ptr(U), self.ptr(dU), self.ptr(dR), self.stream()),
+                  "phx_residual_jvp")
+        return dR
+
+    def step_jvp(self, stages, dt, dU):
+        """In place: dU (derivative of the state before one step) -> derivative after it.
+        stages: (nstages, state_len) stage inputs of that step; dt: (ncases,) its time steps."""
+        nat.check(self.lib.phx_step_jvp(self.handle, self.ptr(stages), self.ptr(dt), self.ptr(dU), self.stream()),
+                  "phx_step_jvp")
+
     def run_record(self, dt_sched, stages):
         """Frozen-dt steps on the current state that record every stage input into ``stages``
         (device tensor (nsteps, nstages, state_len))."""

@@ -22,6 +22,7 @@
 #include "../../include/pyhype_hip.h"
 #include "phx_adjoint_ops.hpp"
 #include "phx_plan.hpp"
+#include "phx_tangent.hpp"
 
 using phx::Phys;
 using phx::Q4;
@@ -161,6 +162,62 @@ __global__ void __launch_bounds__(TILE) k_adj_gather(const DevBlock* blocks, con
     phx::adj::st(lamU + B.off + 4LL * ((long long)k * B.nx + j), lam);
 }
 
+// ---- tangent-linear products (forward mode): three element-wise passes on the same tiles ----
+//   k_tan_recon   face states of every cell and their derivatives (16 + 16 doubles)
+//   k_tan_face    derivative of every face flux (4 doubles)
+//   k_tan_cell    derivative of every cell's residual (the residual is linear in the fluxes)
+__global__ void __launch_bounds__(TILE) k_tan_recon(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                    const double* U, const double* dU, double* rb, double* tb,
+                                                    Phys ph) {
+    const AdjTile t = ld_const(tiles, blockIdx.x);
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
+    const int e = t.start + threadIdx.x;
+    const int k = e / B.nx, j = e - k * B.nx;
+    const long long c = phx::adj::RB * phx::adj::cell_index(O, B, k, j);
+    if (B.kind == PHX_GEOM_CARTESIAN) phx::tng::recon_cell_d<true>(blocks, B, U, dU, k, j, rb + c, tb + c, ph);
+    else phx::tng::recon_cell_d<false>(blocks, B, U, dU, k, j, rb + c, tb + c, ph);
+}
+
+template <int FLUX>
+__global__ void __launch_bounds__(TILE) k_tan_face(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                   const double* U, const double* dU, const double* rb,
+                                                   const double* tb, double* dF, Phys ph) {
+    const AdjTile t = ld_const(tiles, blockIdx.x);
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
+    const long long f = (long long)t.start + threadIdx.x;
+    const Q4 d = (B.kind == PHX_GEOM_CARTESIAN)
+                     ? phx::tng::face_flux_d<true, FLUX>(blocks, B, U, dU, rb, tb, O, f, ph)
+                     : phx::tng::face_flux_d<false, FLUX>(blocks, B, U, dU, rb, tb, O, f, ph);
+    phx::adj::st(dF + 4 * (O.face + f), d);
+}
+
+__global__ void __launch_bounds__(TILE) k_tan_cell(const DevBlock* blocks, const AdjBlock* offs, const AdjTile* tiles,
+                                                   const double* dF, double* out) {
+    const AdjTile t = ld_const(tiles, blockIdx.x);
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const AdjBlock O = ld_const(offs, t.block);
+    const int e = t.start + threadIdx.x;
+    const int k = e / B.nx, j = e - k * B.nx;
+    const Q4 r = (B.kind == PHX_GEOM_CARTESIAN) ? phx::adj::cell_residual<true>(B, O, dF, k, j)
+                                                : phx::adj::cell_residual<false>(B, O, dF, k, j);
+    phx::adj::st(out + B.off + 4LL * ((long long)k * B.nx + j), r);
+}
+
+// out = sum of terms at every cell (the stage combinations of the tangent step)
+__global__ void __launch_bounds__(TILE) k_tan_combine(const DevBlock* blocks, const AdjTile* tiles, Terms terms,
+                                                      double* out) {
+    const AdjTile t = ld_const(tiles, blockIdx.x);
+    if ((int)threadIdx.x >= t.count) return;
+    const DevBlock B = ld_const(blocks, t.block);  // descriptor in scalar registers
+    const long long cell = B.off / 4 + t.start + threadIdx.x;
+    phx::adj::st(out + 4 * cell, eval_terms(terms, B, cell));
+}
+
 template <class T>
 int upload(const std::vector<T>& h, T** d) {
     if (*d) { (void)hipFree(*d); *d = nullptr; }
@@ -241,6 +298,39 @@ int adj_setup(phx_plan* P) {
     return PHX_OK;
 }
 
+// The tangent's stage derivatives K_s (one state array per stage) and the derivative of the current
+// stage input, allocated on the first tangent call (besides the adjoint work arrays it shares).
+int tan_setup(phx_plan* P) {
+    int rc = adj_setup(P);
+    if (rc) return rc;
+    if (P->tan_u) return PHX_OK;
+    const size_t bytes = (size_t)P->state_len * sizeof(double);
+    for (int s = 0; s < P->nstages; ++s) PHX_CK(hipMalloc(&P->tan_k[s], bytes));
+    PHX_CK(hipMalloc(&P->tan_u, bytes));
+    return PHX_OK;
+}
+
+void launch_residual_jvp(phx_plan* P, const double* U, const double* dU, double* out, hipStream_t st) {
+    const dim3 blk(TILE);
+    const dim3 gc((unsigned)P->adj_cell_tiles.size()), gf((unsigned)P->adj_face_tiles.size());
+    // face states in adj_rb, their derivatives in adj_slot (RB of its SLOT doubles per cell), face
+    // flux derivatives in adj_face (4 of its FA doubles per face)
+    hipLaunchKernelGGL(k_tan_recon, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, U, dU, P->adj_rb,
+                       P->adj_slot, P->ph);
+    if (P->flux == PHX_FLUX_ROE)
+        hipLaunchKernelGGL(k_tan_face<0>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, dU, P->adj_rb,
+                           P->adj_slot, P->adj_face, P->ph);
+    else
+        hipLaunchKernelGGL(k_tan_face<1>, gf, blk, 0, st, P->dblocks, P->dadj, P->dadj_face_tiles, U, dU, P->adj_rb,
+                           P->adj_slot, P->adj_face, P->ph);
+    hipLaunchKernelGGL(k_tan_cell, gc, blk, 0, st, P->dblocks, P->dadj, P->dadj_cell_tiles, P->adj_face, out);
+}
+
+int tan_check(phx_plan* P) {
+    if (P->ph.nq > 1) return fail(PHX_E_INVALID, "tangent-linear products support midpoint faces only (quadrature_points 1)");
+    return tan_setup(P);
+}
+
 // ---- static work array of phx_residual_vjp_noalloc ----
 // One array per GPU: every device that runs the library holds its own copy until the process
 // ends.  In host code `scratch` names a host-side stand-in, never the device memory: the device
@@ -316,6 +406,12 @@ void phx::host::adjoint_release(phx_plan* P) {
     P->dadj_cell_tiles = P->dadj_face_tiles = nullptr;
     P->adj_rb = P->adj_face = P->adj_slot = nullptr;
     for (int s = 0; s < PHX_MAX_STAGES; ++s) P->adj_lam[s] = nullptr;
+    for (int s = 0; s < PHX_MAX_STAGES; ++s) {
+        if (P->tan_k[s]) (void)hipFree(P->tan_k[s]);
+        P->tan_k[s] = nullptr;
+    }
+    if (P->tan_u) (void)hipFree(P->tan_u);
+    P->tan_u = nullptr;
     P->adj_ready = false;
     P->adj_meta_ready = false;
 }
@@ -420,6 +516,48 @@ int phx_step_vjp(phx_plan* P, const double* stages, const double* dt_dev, double
             }
             launch_residual_vjp(P, stages, kap, extra, lam, W, st);
         }
+    }
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+// Forward-mode product of the residual: dR = (dR/dU) dU at U.
+int phx_residual_jvp(phx_plan* P, const double* U, const double* dU, double* dR, void* stream) {
+    if (!P || !U || !dU || !dR) return fail(PHX_E_INVALID, "null argument");
+    int rc = tan_check(P);
+    if (rc) return rc;
+    launch_residual_jvp(P, U, dU, dR, (hipStream_t)stream);
+    PHX_CK(hipGetLastError());
+    return PHX_OK;
+}
+
+// Forward-mode product of one RK step with a frozen time step: dU_i = dU_0 + dt sum_m a[i-1][m] dK_m,
+// dK_m = (dR/dU)(U_m) dU_m, and dU_S (the step's output) written over dU.
+int phx_step_jvp(phx_plan* P, const double* stages, const double* dt_dev, double* dU, void* stream) {
+    if (!P || !stages || !dt_dev || !dU) return fail(PHX_E_INVALID, "null argument");
+    int rc = tan_check(P);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const int S = P->nstages;
+    const size_t L = (size_t)P->state_len;
+    const double* a = P->butcher.data();
+    const dim3 gc((unsigned)P->adj_cell_tiles.size());
+    for (int s = 0; s < S; ++s) {
+        launch_residual_jvp(P, stages + (size_t)s * L, s == 0 ? dU : P->tan_u, P->tan_k[s], st);
+        Terms T;
+        memset(&T, 0, sizeof(T));
+        T.in[T.n] = dU;
+        T.c[T.n++] = 1.0;
+        for (int m = 0; m <= s; ++m) {
+            const double c = a[(size_t)s * S + m];
+            if (c == 0.0) continue;
+            T.in[T.n] = P->tan_k[m];
+            T.c[T.n] = c;
+            T.scaled[T.n++] = 1;
+        }
+        T.dt = dt_dev;
+        hipLaunchKernelGGL(k_tan_combine, gc, dim3(TILE), 0, st, P->dblocks, P->dadj_cell_tiles, T,
+                           s + 1 < S ? P->tan_u : dU);
     }
     PHX_CK(hipGetLastError());
     return PHX_OK;

@@ -14,6 +14,7 @@ namespace dev {
 
 using phx::Phys;
 using phx::Q4;
+using phx::Q4T;
 
 // Device copy of one phx_block_desc.  `off` is the block's offset (in doubles) into the state;
 // `wall` holds the (cos, sin) pairs of the E, W, N, S boundary normals of a Cartesian block.
@@ -163,9 +164,9 @@ __host__ __device__ inline Q4 fetch_r(const DevBlock* __restrict__ blocks, const
 // Outer state of a boundary face: MUSCL._get_boundary_flux_states (fvm/base.py:305-325) and the
 // BC applied to it (fvm/base.py:352-362,421-431).  `own` is the block's limited face value,
 // `ghost` the ghost cell value (the ghost block's limited value with phi = 0, grad = 0).
-template <bool CART>
-__host__ __device__ inline Q4 outer_state(const DevBlock& B, int side, const Q4& own, const Q4& ghost,
-                                          int k, int j) {
+template <bool CART, class T>
+__host__ __device__ inline Q4T<T> outer_state(const DevBlock& B, int side, const Q4T<T>& own, const Q4T<T>& ghost,
+                                              int k, int j) {
     const int st = B.side[side];
     if (st == PHX_SIDE_NEIGHBOR || st == PHX_SIDE_OPEN) return ghost;
     if (st == PHX_SIDE_OUTLET) return own;
@@ -175,29 +176,31 @@ __host__ __device__ inline Q4 outer_state(const DevBlock& B, int side, const Q4&
 }
 
 // Face flux of face states in the reconstruction variables; GEN = false: conservative states.
-template <int FLUX, bool GEN>
-__host__ __device__ inline Q4 flux_of(const Q4& L, const Q4& R, const Phys& ph) {
+template <int FLUX, bool GEN, class T>
+__host__ __device__ inline Q4T<T> flux_of(const Q4T<T>& L, const Q4T<T>& R, const Phys& ph) {
     return GEN ? phx::face_flux_recon<FLUX>(L, R, ph) : phx::face_flux<FLUX>(L, R, ph);
 }
 
-template <bool CART, int FLUX, bool GEN = false>
-__host__ __device__ inline Q4 xflux(const DevBlock& B, const Q4& L, const Q4& R, int k, int jf, const Phys& ph) {
+template <bool CART, int FLUX, bool GEN = false, class T>
+__host__ __device__ inline Q4T<T> xflux(const DevBlock& B, const Q4T<T>& L, const Q4T<T>& R, int k, int jf,
+                                        const Phys& ph) {
     if (CART) return flux_of<FLUX, GEN>(L, R, ph);
     const GenG G{B.geom, B.nx, B.ny};
     const double c = G.V(F_c, k, jf), s = G.V(F_s, k, jf);
-    const Q4 F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    const Q4T<T> F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
     return phx::unrotate(F, c, s);
 }
 
-template <bool CART, int FLUX, bool GEN = false>
-__host__ __device__ inline Q4 yflux(const DevBlock& B, const Q4& L, const Q4& R, int kf, int j, const Phys& ph) {
+template <bool CART, int FLUX, bool GEN = false, class T>
+__host__ __device__ inline Q4T<T> yflux(const DevBlock& B, const Q4T<T>& L, const Q4T<T>& R, int kf, int j,
+                                        const Phys& ph) {
     if (CART) {
-        const Q4 F = flux_of<FLUX, GEN>(phx::rotate90(L), phx::rotate90(R), ph);
+        const Q4T<T> F = flux_of<FLUX, GEN>(phx::rotate90(L), phx::rotate90(R), ph);
         return phx::unrotate90(F);
     }
     const GenG G{B.geom, B.nx, B.ny};
     const double c = G.H(F_c, kf, j), s = G.H(F_s, kf, j);
-    const Q4 F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
+    const Q4T<T> F = flux_of<FLUX, GEN>(phx::rotate(L, c, s), phx::rotate(R, c, s), ph);
     return phx::unrotate(F, c, s);
 }
 

@@ -11,9 +11,14 @@
 
 namespace phx {
 
-struct Q4 {
-    double v[4];
+// A state or flux of 4 components.  The arithmetic below is written once for a scalar type T:
+// T = double is the path every kernel runs; the tangent-linear kernels instantiate it with a
+// value-and-derivative pair whose value part performs exactly the same double operations.
+template <class T>
+struct Q4T {
+    T v[4];
 };
+using Q4 = Q4T<double>;
 
 // Fluid constants, evaluated on the host exactly as the reference evaluates them:
 // gm1 = gamma - 1 (states/conservative.py:110), one_over_gm1 = 1.0/(gamma - 1.0) and
@@ -25,9 +30,11 @@ struct Phys {
     int nq;       // Gauss points per face (SolverConfig.fvm_num_quadrature_points); 0 or 1: midpoint
 };
 
-struct Prim {
-    double r, u, v, p;
+template <class T>
+struct PrimT {
+    T r, u, v, p;
 };
+using Prim = PrimT<double>;
 
 __host__ __device__ inline Q4 q4zero() {
     Q4 q;
@@ -38,44 +45,50 @@ __host__ __device__ inline Q4 q4zero() {
 // ConservativeConverter.to_primitive (states/converter/concrete_defs.py:85-100), with
 // u = rhou/rho, v = rhov/rho (conservative.py:85-99), ek = rho*Ek, Ek = 0.5*(u*u+v*v)
 // (conservative.py:118-124), p = (gamma-1)*(e - ek) (conservative.py:107-110).
-__host__ __device__ inline Prim cons_to_prim(const Q4& q, const Phys& ph) {
-    Prim w;
+template <class T>
+__host__ __device__ inline PrimT<T> cons_to_prim(const Q4T<T>& q, const Phys& ph) {
+    PrimT<T> w;
     w.r = q.v[0];
     w.u = q.v[1] / w.r;
     w.v = q.v[2] / w.r;
-    const double ek = w.r * (0.5 * (w.u * w.u + w.v * w.v));
+    const T ek = w.r * (0.5 * (w.u * w.u + w.v * w.v));
     w.p = ph.gm1 * (q.v[3] - ek);
     return w;
 }
 
 // rotate_JIT / unrotate_JIT (utils/utils.py:75-85,148-158) with c = cos(theta), s = sin(theta).
-__host__ __device__ inline Q4 rotate(Q4 q, double c, double s) {
-    const double u = q.v[1], v = q.v[2];
+template <class T>
+__host__ __device__ inline Q4T<T> rotate(Q4T<T> q, double c, double s) {
+    const T u = q.v[1], v = q.v[2];
     q.v[1] = u * c + v * s;
     q.v[2] = v * c - u * s;
     return q;
 }
-__host__ __device__ inline Q4 unrotate(Q4 q, double c, double s) {
-    const double u = q.v[1], v = q.v[2];
+template <class T>
+__host__ __device__ inline Q4T<T> unrotate(Q4T<T> q, double c, double s) {
+    const T u = q.v[1], v = q.v[2];
     q.v[1] = u * c - v * s;
     q.v[2] = v * c + u * s;
     return q;
 }
 // rotate90 / unrotate90 (utils/utils.py:88-112,161-185).
-__host__ __device__ inline Q4 rotate90(Q4 q) {
-    const double u = q.v[2], v = q.v[1];
+template <class T>
+__host__ __device__ inline Q4T<T> rotate90(Q4T<T> q) {
+    const T u = q.v[2], v = q.v[1];
     q.v[1] = u;
     q.v[2] = -v;
     return q;
 }
-__host__ __device__ inline Q4 unrotate90(Q4 q) {
-    const double u = q.v[2], v = q.v[1];
+template <class T>
+__host__ __device__ inline Q4T<T> unrotate90(Q4T<T> q) {
+    const T u = q.v[2], v = q.v[1];
     q.v[1] = -u;
     q.v[2] = v;
     return q;
 }
 // BoundaryConditionFunctions.reflection (boundary_conditions/funcs.py:27-43).
-__host__ __device__ inline Q4 reflect(Q4 q, double c, double s) {
+template <class T>
+__host__ __device__ inline Q4T<T> reflect(Q4T<T> q, double c, double s) {
     q = rotate(q, c, s);
     q.v[1] = -q.v[1];
     return unrotate(q, c, s);
@@ -83,12 +96,12 @@ __host__ __device__ inline Q4 reflect(Q4 q, double c, double s) {
 
 // Harten entropy fix on the Roe wave speeds (flux/base.py:119-146).  Inputs are the slow (u-a)
 // and fast (u+a) speeds of the right, left and Roe states; l1/l4 are corrected in place.
-__host__ __device__ inline void harten(double Rslow, double Rfast, double Lslow, double Lfast,
-                                       double& l1, double& l4) {
-    double tp = 2.0 * (Rslow - Lslow);
-    double tm = 2.0 * (Rfast - Lfast);
-    tp = (tp <= 0.0) ? 1e-8 : tp;
-    tm = (tm <= 0.0) ? 1e-8 : tm;
+template <class T>
+__host__ __device__ inline void harten(T Rslow, T Rfast, T Lslow, T Lfast, T& l1, T& l4) {
+    T tp = 2.0 * (Rslow - Lslow);
+    T tm = 2.0 * (Rfast - Lfast);
+    tp = (tp <= 0.0) ? T(1e-8) : tp;
+    tm = (tm <= 0.0) ? T(1e-8) : tm;
     if (fabs(l1) < tp) l1 = 0.5 * ((l1 * l1) / tp + tp);
     if (fabs(l4) < tm) l4 = 0.5 * ((l4 * l4) / tm + tm);
 }
@@ -100,51 +113,52 @@ __host__ __device__ inline void harten(double Rslow, double Rfast, double Lslow,
 //  * Dissipation Rc |Lambda| Lp dW with the SciPy COO accumulation order of the reference's
 //    block layout (flux/eigen_system.py:115-158,205-242; values flux/Roe.py:175-260).  The
 //    leading "0 +" of each COO row sum only changes the sign of a zero and is omitted.
-__host__ __device__ inline Q4 roe_flux(const Prim& L, const Prim& R, const Phys& ph) {
-    const double ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
-    const double ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
-    const double ruL = L.r * L.u, ruR = R.r * R.u;
-    const double FL0 = ruL, FL1 = ruL * L.u + L.p, FL2 = ruL * L.v;
-    const double FL3 = L.u * ((ph.one_over_gm1 * L.p + ekL) + L.p);
-    const double FR0 = ruR, FR1 = ruR * R.u + R.p, FR2 = ruR * R.v;
-    const double FR3 = R.u * ((ph.one_over_gm1 * R.p + ekR) + R.p);
+template <class T>
+__host__ __device__ inline Q4T<T> roe_flux(const PrimT<T>& L, const PrimT<T>& R, const Phys& ph) {
+    const T ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
+    const T ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
+    const T ruL = L.r * L.u, ruR = R.r * R.u;
+    const T FL0 = ruL, FL1 = ruL * L.u + L.p, FL2 = ruL * L.v;
+    const T FL3 = L.u * ((ph.one_over_gm1 * L.p + ekL) + L.p);
+    const T FR0 = ruR, FR1 = ruR * R.u + R.p, FR2 = ruR * R.v;
+    const T FR3 = R.u * ((ph.one_over_gm1 * R.p + ekR) + R.p);
 
-    const double sL = sqrt(L.r), sR = sqrt(R.r);
-    const double inv = 1.0 / (sL + sR);
-    const double rs = sqrt(L.r * R.r);
-    const double us = (L.u * sL + R.u * sR) * inv;
-    const double vs = (L.v * sL + R.v * sR) * inv;
-    const double ps = (L.p * sL + R.p * sR) * inv;
+    const T sL = sqrt(L.r), sR = sqrt(R.r);
+    const T inv = 1.0 / (sL + sR);
+    const T rs = sqrt(L.r * R.r);
+    const T us = (L.u * sL + R.u * sR) * inv;
+    const T vs = (L.v * sL + R.v * sR) * inv;
+    const T ps = (L.p * sL + R.p * sR) * inv;
 
-    const double aL = sqrt(ph.gamma * L.p / L.r);
-    const double aR = sqrt(ph.gamma * R.p / R.r);
-    const double as = sqrt(ph.gamma * ps / rs);
-    double l1 = us - as, l4 = us + as;
+    const T aL = sqrt(ph.gamma * L.p / L.r);
+    const T aR = sqrt(ph.gamma * R.p / R.r);
+    const T as = sqrt(ph.gamma * ps / rs);
+    T l1 = us - as, l4 = us + as;
     harten(R.u - aR, R.u + aR, L.u - aL, L.u + aL, l1, l4);
 
-    const double Ek = 0.5 * (us * us + vs * vs);           // Ek_JIT
-    const double H = ph.g_over_gm1 * ps / rs + Ek;         // H_given_Ek_JIT
-    const double ua = us * as;
-    const double ia = 1.0 / as;                            // compute_Lp
-    const double ia2 = ia * ia;
-    const double h = 0.5 * ia2;
-    const double r2a = 0.5 * rs * ia;
+    const T Ek = 0.5 * (us * us + vs * vs);           // Ek_JIT
+    const T H = ph.g_over_gm1 * ps / rs + Ek;         // H_given_Ek_JIT
+    const T ua = us * as;
+    const T ia = 1.0 / as;                            // compute_Lp
+    const T ia2 = ia * ia;
+    const T h = 0.5 * ia2;
+    const T r2a = 0.5 * rs * ia;
 
-    const double d0 = R.r - L.r, d1 = R.u - L.u, d2 = R.v - L.v, d3 = R.p - L.p;
+    const T d0 = R.r - L.r, d1 = R.u - L.u, d2 = R.v - L.v, d3 = R.p - L.p;
     // Lp * dW: rows in COO order (3,1),(1,0),(2,2),(3,3),(0,1),(1,3),(0,3)
-    const double y0 = (-r2a) * d1 + h * d3;
-    const double y1 = 1.0 * d0 + (-ia2) * d3;
-    const double y2 = 1.0 * d2;
-    const double y3 = r2a * d1 + h * d3;
+    const T y0 = (-r2a) * d1 + h * d3;
+    const T y1 = 1.0 * d0 + (-ia2) * d3;
+    const T y2 = 1.0 * d2;
+    const T y3 = r2a * d1 + h * d3;
     // |Lambda| (diagonal)
-    const double z0 = fabs(l1) * y0, z1 = fabs(us) * y1, z2 = fabs(us) * y2, z3 = fabs(l4) * y3;
+    const T z0 = fabs(l1) * y0, z1 = fabs(us) * y1, z2 = fabs(us) * y2, z3 = fabs(l4) * y3;
     // Rc * z: COO order (3,0),(2,0),(3,1),(1,0),(2,1),(3,2),(0,0),(1,1),(2,2),(3,3),(0,1),(2,3),(1,3),(0,3)
-    const double s0 = (1.0 * z0 + 1.0 * z1) + 1.0 * z3;
-    const double s1 = (l1 * z0 + us * z1) + l4 * z3;
-    const double s2 = ((vs * z0 + vs * z1) + 1.0 * z2) + vs * z3;
-    const double s3 = (((H - ua) * z0 + Ek * z1) + vs * z2) + (H + ua) * z3;
+    const T s0 = (1.0 * z0 + 1.0 * z1) + 1.0 * z3;
+    const T s1 = (l1 * z0 + us * z1) + l4 * z3;
+    const T s2 = ((vs * z0 + vs * z1) + 1.0 * z2) + vs * z3;
+    const T s3 = (((H - ua) * z0 + Ek * z1) + vs * z2) + (H + ua) * z3;
 
-    Q4 F;
+    Q4T<T> F;
     F.v[0] = 0.5 * (FL0 + FR0) - 0.5 * s0;
     F.v[1] = 0.5 * (FL1 + FR1) - 0.5 * s1;
     F.v[2] = 0.5 * (FL2 + FR2) - 0.5 * s2;
@@ -354,47 +368,48 @@ __host__ __device__ inline double nrm4(double a, double b, double c, double d) {
 }
 
 // FluxHLLL.compute_flux + _HLLL_flux_JIT (flux/HLLL.py:25-49,70-103) for one face.
-__host__ __device__ inline Q4 hlll_flux(const Prim& L, const Prim& R, const Phys& ph) {
-    const double sL = sqrt(L.r), sR = sqrt(R.r);
-    const double inv = 1.0 / (sL + sR);
-    const double rs = sqrt(L.r * R.r);
-    const double us = (L.u * sL + R.u * sR) * inv;
-    const double ps = (L.p * sL + R.p * sR) * inv;
-    const double as = sqrt(ph.gamma * ps / rs);
-    const double aL = sqrt(ph.gamma * L.p / L.r);
-    const double aR = sqrt(ph.gamma * R.p / R.r);
-    const double Lslow = L.u - aL, Lfast = L.u + aL;
-    const double Rslow = R.u - aR, Rfast = R.u + aR;
-    double l1 = us - as, l4 = us + as;
+template <class T>
+__host__ __device__ inline Q4T<T> hlll_flux(const PrimT<T>& L, const PrimT<T>& R, const Phys& ph) {
+    const T sL = sqrt(L.r), sR = sqrt(R.r);
+    const T inv = 1.0 / (sL + sR);
+    const T rs = sqrt(L.r * R.r);
+    const T us = (L.u * sL + R.u * sR) * inv;
+    const T ps = (L.p * sL + R.p * sR) * inv;
+    const T as = sqrt(ph.gamma * ps / rs);
+    const T aL = sqrt(ph.gamma * L.p / L.r);
+    const T aR = sqrt(ph.gamma * R.p / R.r);
+    const T Lslow = L.u - aL, Lfast = L.u + aL;
+    const T Rslow = R.u - aR, Rfast = R.u + aR;
+    T l1 = us - as, l4 = us + as;
     harten(Rslow, Rfast, Lslow, Lfast, l1, l4);
-    const double Lplus = (Rfast >= l4) ? Rfast : l4;      // np.maximum(R_m, Lm)
-    const double Lminus = (Lslow <= l1) ? Lslow : l1;     // np.minimum(L_p, Lp)
+    const T Lplus = (Rfast >= l4) ? Rfast : l4;      // np.maximum(R_m, Lm)
+    const T Lminus = (Lslow <= l1) ? Lslow : l1;     // np.minimum(L_p, Lp)
 
     // ConservativeState(state=W): PrimitiveConverter.to_conservative (concrete_defs.py:126-141)
-    const double ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
-    const double ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
-    const double UL0 = L.r, UL1 = L.r * L.u, UL2 = L.r * L.v, UL3 = L.p / ph.gm1 + ekL;
-    const double UR0 = R.r, UR1 = R.r * R.u, UR2 = R.r * R.v, UR3 = R.p / ph.gm1 + ekR;
+    const T ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
+    const T ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
+    const T UL0 = L.r, UL1 = L.r * L.u, UL2 = L.r * L.v, UL3 = L.p / ph.gm1 + ekL;
+    const T UR0 = R.r, UR1 = R.r * R.u, UR2 = R.r * R.v, UR3 = R.p / ph.gm1 + ekR;
     // PrimitiveState.F(U=...) (states/primitive.py:203-209)
-    const double FL0 = UL1, FL1 = UL1 * L.u + L.p, FL2 = UL1 * L.v, FL3 = L.u * (UL3 + L.p);
-    const double FR0 = UR1, FR1 = UR1 * R.u + R.p, FR2 = UR1 * R.v, FR3 = R.u * (UR3 + R.p);
-    Q4 F;
+    const T FL0 = UL1, FL1 = UL1 * L.u + L.p, FL2 = UL1 * L.v, FL3 = L.u * (UL3 + L.p);
+    const T FR0 = UR1, FR1 = UR1 * R.u + R.p, FR2 = UR1 * R.v, FR3 = R.u * (UR3 + R.p);
+    Q4T<T> F;
     if (Lminus >= 0.0) {
         F.v[0] = FL0; F.v[1] = FL1; F.v[2] = FL2; F.v[3] = FL3;
     } else if (Lplus <= 0.0) {
         F.v[0] = FR0; F.v[1] = FR1; F.v[2] = FR2; F.v[3] = FR3;
     } else {
-        const double dU0 = UR0 - UL0, dU1 = UR1 - UL1, dU2 = UR2 - UL2, dU3 = UR3 - UL3;
-        const double dF0 = FR0 - FL0, dF1 = FR1 - FL1, dF2 = FR2 - FL2, dF3 = FR3 - FL3;
-        const double k = as * nrm4(dU0, dU1, dU2, dU3);
-        const double n = nrm4(dF0 - us * dU0, dF1 - us * dU1, dF2 - us * dU2, dF3 - us * dU3);
-        const double den = (k < 1e-16) ? (k + 1e-14) : k;
-        double alpha = 1.0 - n / den;
-        alpha = (0.0 >= alpha) ? 0.0 : alpha;              // np.maximum(0, ...)
-        const double ra = us / Lminus, rb = us / Lplus;
-        const double mx = (ra >= rb) ? ra : rb;            // np.maximum(u/Lm, u/Lp)
-        const double coef = Lminus * Lplus * (1.0 - alpha * (1.0 - mx));
-        const double w = Lplus - Lminus;
+        const T dU0 = UR0 - UL0, dU1 = UR1 - UL1, dU2 = UR2 - UL2, dU3 = UR3 - UL3;
+        const T dF0 = FR0 - FL0, dF1 = FR1 - FL1, dF2 = FR2 - FL2, dF3 = FR3 - FL3;
+        const T k = as * nrm4(dU0, dU1, dU2, dU3);
+        const T n = nrm4(dF0 - us * dU0, dF1 - us * dU1, dF2 - us * dU2, dF3 - us * dU3);
+        const T den = (k < 1e-16) ? T(k + 1e-14) : k;
+        T alpha = 1.0 - n / den;
+        alpha = (0.0 >= alpha) ? T(0.0) : alpha;              // np.maximum(0, ...)
+        const T ra = us / Lminus, rb = us / Lplus;
+        const T mx = (ra >= rb) ? ra : rb;            // np.maximum(u/Lm, u/Lp)
+        const T coef = Lminus * Lplus * (1.0 - alpha * (1.0 - mx));
+        const T w = Lplus - Lminus;
         F.v[0] = (Lplus * FL0 - Lminus * FR0 + coef * dU0) / w;
         F.v[1] = (Lplus * FL1 - Lminus * FR1 + coef * dU1) / w;
         F.v[2] = (Lplus * FL2 - Lminus * FR2 + coef * dU2) / w;
@@ -403,25 +418,27 @@ __host__ __device__ inline Q4 hlll_flux(const Prim& L, const Prim& R, const Phys
     return F;
 }
 
-template <int FLUX>
-__host__ __device__ inline Q4 face_flux(const Q4& qL, const Q4& qR, const Phys& ph) {
-    const Prim L = cons_to_prim(qL, ph);
-    const Prim R = cons_to_prim(qR, ph);
+template <int FLUX, class T>
+__host__ __device__ inline Q4T<T> face_flux(const Q4T<T>& qL, const Q4T<T>& qR, const Phys& ph) {
+    const PrimT<T> L = cons_to_prim(qL, ph);
+    const PrimT<T> R = cons_to_prim(qR, ph);
     if (FLUX == 0) return roe_flux(L, R, ph);
     return hlll_flux(L, R, ph);
 }
 
-__host__ __device__ inline Prim as_prim(const Q4& q) { return Prim{q.v[0], q.v[1], q.v[2], q.v[3]}; }
-__host__ __device__ inline Q4 prim_q4(const Prim& w) {
-    Q4 q;
+template <class T>
+__host__ __device__ inline PrimT<T> as_prim(const Q4T<T>& q) { return PrimT<T>{q.v[0], q.v[1], q.v[2], q.v[3]}; }
+template <class T>
+__host__ __device__ inline Q4T<T> prim_q4(const PrimT<T>& w) {
+    Q4T<T> q;
     q.v[0] = w.r; q.v[1] = w.u; q.v[2] = w.v; q.v[3] = w.p;
     return q;
 }
 
 // face_flux on face states in the reconstruction variables: with PrimitiveState reconstruction the
 // states are primitive already and go to the flux as they are (fvm/base.py:286-289).
-template <int FLUX>
-__host__ __device__ inline Q4 face_flux_recon(const Q4& qL, const Q4& qR, const Phys& ph) {
+template <int FLUX, class T>
+__host__ __device__ inline Q4T<T> face_flux_recon(const Q4T<T>& qL, const Q4T<T>& qR, const Phys& ph) {
     if (!ph.prim) return face_flux<FLUX>(qL, qR, ph);
     if (FLUX == 0) return roe_flux(as_prim(qL), as_prim(qR), ph);
     return hlll_flux(as_prim(qL), as_prim(qR), ph);
@@ -435,31 +452,37 @@ struct CellGeo {
     double dxE, dxW, dxN, dxS, dyE, dyW, dyN, dyS;
 };
 
-struct Recon {
-    Q4 E, W, N, S;  // limited face states (SecondOrderMUSCL.limited_solution_at_quadrature_point)
+template <class T>
+struct ReconT {
+    Q4T<T> E, W, N, S;  // limited face states (SecondOrderMUSCL.limited_solution_at_quadrature_point)
 };
+using Recon = ReconT<double>;
 
 // np.minimum of two scalars: a NaN operand is returned (NaN-propagating, unlike fmin).
-__host__ __device__ inline double np_minimum(double a, double b) {
+template <class T>
+__host__ __device__ inline T np_minimum(T a, T b) {
     return (a != a) ? a : ((b != b) ? b : ((a <= b) ? a : b));
 }
 
 // Slope of one face, SlopeLimiter._compute_slope (limiters/base.py:189-221): dmax/d for d > 0,
 // dmin/d for d < 0, 1 otherwise.
-__host__ __device__ inline double face_slope(double dmax, double dmin, double d) {
-    const double num = (d > 0.0) ? dmax : dmin;
-    return (d > 0.0 || d < 0.0) ? num / d : 1.0;
+template <class T>
+__host__ __device__ inline T face_slope(T dmax, T dmin, T d) {
+    const T num = (d > 0.0) ? dmax : dmin;
+    return (d > 0.0 || d < 0.0) ? num / d : T(1.0);
 }
 
 // Limiter functions of limiters/limiters.py:25-62 (slope**2 is s*s).
-__host__ __device__ inline double venkatakrishnan_fn(double s) {
-    const double s2 = s * s;
+template <class T>
+__host__ __device__ inline T venkatakrishnan_fn(T s) {
+    const T s2 = s * s;
     return (s2 + 2.0 * s) / (s2 + s + 2.0);
 }
-__host__ __device__ inline double limiter_fn(int lim, double s) {
+template <class T>
+__host__ __device__ inline T limiter_fn(int lim, T s) {
     switch (lim) {
-        case 1: return np_minimum(1.0, s);                                    // BarthJespersen
-        case 2: { const double s2 = s * s; return (s2 + s) / (s2 + 1.0); }   // VanAlbada
+        case 1: return np_minimum(T(1.0), s);                                 // BarthJespersen
+        case 2: { const T s2 = s * s; return (s2 + s) / (s2 + 1.0); }        // VanAlbada
         case 3: return (fabs(s) + s) / (s + 1.0);                             // VanLeer
         default: return venkatakrishnan_fn(s);                                 // Venkatakrishnan
     }
@@ -467,9 +490,9 @@ __host__ __device__ inline double limiter_fn(int lim, double s) {
 
 // Limiter value of one face.  GEN = false: Venkatakrishnan (the default, compiled without the
 // limiter switch); GEN = true: the limiter `lim`.
-template <bool GEN>
-__host__ __device__ inline double limit_face(double dmax, double dmin, double d, int lim) {
-    const double s = face_slope(dmax, dmin, d);
+template <bool GEN, class T>
+__host__ __device__ inline T limit_face(T dmax, T dmin, T d, int lim) {
+    const T s = face_slope(dmax, dmin, d);
     return GEN ? limiter_fn(lim, s) : venkatakrishnan_fn(s);
 }
 
@@ -486,41 +509,40 @@ __host__ __device__ inline double venkat_face(double dmax, double dmin, double d
 //    or with GEN the limiter `lim` (BarthJespersen, VanAlbada, VanLeer),
 //  * limited face values U + phi*hot (fvm/SecondOrderMUSCL.py:106-126).
 // U is the cell, Uw/Ue/Us/Un its west/east/south/north neighbours (ghost values at block edges).
-template <bool GEN = false>
-__host__ __device__ inline void reconstruct_var(double u, double uw, double ue, double us, double un,
-                                                const CellGeo& g, int lim, double& qE, double& qW, double& qN,
-                                                double& qS) {
-    const double e = 0.5 * (u + ue);
-    const double w = 0.5 * (uw + u);
-    const double n = 0.5 * (u + un);
-    const double s = 0.5 * (us + u);
-    const double gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
-    const double gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
-    const double hE = gx * g.dxE + gy * g.dyE;
-    const double hW = gx * g.dxW + gy * g.dyW;
-    const double hN = gx * g.dxN + gy * g.dyN;
-    const double hS = gx * g.dxS + gy * g.dyS;
+template <bool GEN = false, class T>
+__host__ __device__ inline void reconstruct_var(T u, T uw, T ue, T us, T un, const CellGeo& g, int lim, T& qE,
+                                                T& qW, T& qN, T& qS) {
+    const T e = 0.5 * (u + ue);
+    const T w = 0.5 * (uw + u);
+    const T n = 0.5 * (u + un);
+    const T s = 0.5 * (us + u);
+    const T gx = (e * g.xlE + w * g.xlW + n * g.xlN + s * g.xlS) * g.ainv;
+    const T gy = (e * g.ylE + w * g.ylW + n * g.ylN + s * g.ylS) * g.ainv;
+    const T hE = gx * g.dxE + gy * g.dyE;
+    const T hW = gx * g.dxW + gy * g.dyW;
+    const T hN = gx * g.dxN + gy * g.dyN;
+    const T hS = gx * g.dxS + gy * g.dyS;
     // np.maximum.reduce((U, UW, UE, US, UN)) - U
-    const double mx = fmax(fmax(fmax(fmax(u, uw), ue), us), un);
-    const double mn = fmin(fmin(fmin(fmin(u, uw), ue), us), un);
-    const double dmax = mx - u;
-    const double dmin = mn - u;
-    const double pE = limit_face<GEN>(dmax, dmin, (u + hE) - u, lim);
-    const double pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
-    const double pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
-    const double pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
-    double phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
-    phi = (phi < 0.0) ? 0.0 : phi;                  // np.where(phi < 0, 0, phi)
+    const T mx = fmax(fmax(fmax(fmax(u, uw), ue), us), un);
+    const T mn = fmin(fmin(fmin(fmin(u, uw), ue), us), un);
+    const T dmax = mx - u;
+    const T dmin = mn - u;
+    const T pE = limit_face<GEN>(dmax, dmin, (u + hE) - u, lim);
+    const T pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
+    const T pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
+    const T pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
+    T phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+    phi = (phi < 0.0) ? T(0.0) : phi;                  // np.where(phi < 0, 0, phi)
     qE = u + phi * hE;
     qW = u + phi * hW;
     qN = u + phi * hN;
     qS = u + phi * hS;
 }
 
-template <bool GEN = false>
-__host__ __device__ inline Recon reconstruct(const Q4& U, const Q4& Uw, const Q4& Ue, const Q4& Us,
-                                             const Q4& Un, const CellGeo& g, int lim = 0) {
-    Recon r;
+template <bool GEN = false, class T>
+__host__ __device__ inline ReconT<T> reconstruct(const Q4T<T>& U, const Q4T<T>& Uw, const Q4T<T>& Ue,
+                                                 const Q4T<T>& Us, const Q4T<T>& Un, const CellGeo& g, int lim = 0) {
+    ReconT<T> r;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
         reconstruct_var<GEN>(U.v[k], Uw.v[k], Ue.v[k], Us.v[k], Un.v[k], g, lim, r.E.v[k], r.W.v[k], r.N.v[k],

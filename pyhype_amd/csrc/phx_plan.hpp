@@ -129,6 +129,9 @@ struct phx_plan {
     double* adj_face = nullptr;   // adjoints of the left/right states of every face (8 doubles)
     double* adj_slot = nullptr;   // reconstruction adjoints: cell, W, E, S, N (20 doubles per cell)
     double* adj_lam[PHX_MAX_STAGES] = {};  // stage-state adjoints of one step (stages 1..S-1)
+    // tangent-linear products, allocated on the first tangent call
+    double* tan_k[PHX_MAX_STAGES] = {};    // stage residual derivatives of one step
+    double* tan_u = nullptr;               // derivative of the current stage input
 };
 
 namespace phx {

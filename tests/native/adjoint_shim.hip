@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../pyhype_amd/csrc/phx_adjoint_ops.hpp"
+#include "../../pyhype_amd/csrc/phx_tangent.hpp"
 
 using namespace phx;
 using phx::dev::DevBlock;
@@ -153,9 +154,63 @@ void residual_vjp_t(const HostPlan& P, const double* U, const double* kappa, dou
     }
 }
 
+// residual through the tangent-linear element operations: values (Rv) and derivatives (dR)
+template <int FLUX>
+void residual_jvp_t(const HostPlan& P, const double* U, const double* dU, double* Rv, double* dR, const Phys& ph) {
+    std::vector<double> rb((size_t)P.cells * adj::RB), tb((size_t)P.cells * adj::RB);
+    std::vector<double> Fv((size_t)P.faces * 4), dF((size_t)P.faces * 4);
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        const DevBlock& B = P.blocks[b];
+        for (int k = 0; k < B.ny; ++k)
+            for (int j = 0; j < B.nx; ++j) {
+                const long long c = adj::RB * adj::cell_index(P.offs[b], B, k, j);
+                if (B.kind == PHX_GEOM_CARTESIAN)
+                    tng::recon_cell_d<true>(P.blocks.data(), B, U, dU, k, j, rb.data() + c, tb.data() + c, ph);
+                else
+                    tng::recon_cell_d<false>(P.blocks.data(), B, U, dU, k, j, rb.data() + c, tb.data() + c, ph);
+            }
+    }
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        const DevBlock& B = P.blocks[b];
+        const long long nf = (long long)B.ny * (B.nx + 1) + (long long)(B.ny + 1) * B.nx;
+        for (long long f = 0; f < nf; ++f) {
+            const tng::QD F = (B.kind == PHX_GEOM_CARTESIAN)
+                                  ? tng::face_flux_dd<true, FLUX>(P.blocks.data(), B, U, dU, rb.data(), tb.data(),
+                                                                  P.offs[b], f, ph)
+                                  : tng::face_flux_dd<false, FLUX>(P.blocks.data(), B, U, dU, rb.data(), tb.data(),
+                                                                   P.offs[b], f, ph);
+            put(Fv.data() + 4 * (P.offs[b].face + f), tng::val(F));
+            put(dF.data() + 4 * (P.offs[b].face + f), tng::der(F));
+        }
+    }
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        const DevBlock& B = P.blocks[b];
+        for (int k = 0; k < B.ny; ++k)
+            for (int j = 0; j < B.nx; ++j) {
+                const bool cart = B.kind == PHX_GEOM_CARTESIAN;
+                const long long o = B.off + 4LL * ((long long)k * B.nx + j);
+                put(Rv + o, cart ? adj::cell_residual<true>(B, P.offs[b], Fv.data(), k, j)
+                                 : adj::cell_residual<false>(B, P.offs[b], Fv.data(), k, j));
+                put(dR + o, cart ? adj::cell_residual<true>(B, P.offs[b], dF.data(), k, j)
+                                 : adj::cell_residual<false>(B, P.offs[b], dF.data(), k, j));
+            }
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+// residual (values) and its forward-mode derivative in direction dU (midpoint faces)
+int shim_residual_jvp(const phx_block_desc* d, int nb, int flux, const double* phys, const double* U,
+                      const double* dU, double* Rv, double* dR) {
+    const Phys ph = make_phys(phys);
+    if (ph.nq > 1) return -1;
+    const HostPlan P = make_plan(d, nb, 1);
+    if (flux == 0) residual_jvp_t<0>(P, U, dU, Rv, dR, ph);
+    else residual_jvp_t<1>(P, U, dU, Rv, dR, ph);
+    return 0;
+}
 
 // element-wise residual of every block (same layout as phx_residual)
 int shim_residual(const phx_block_desc* d, int nb, int flux, const double* phys, const double* U, double* R) {

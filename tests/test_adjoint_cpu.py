@@ -5,7 +5,10 @@
 * the flux and reconstruction VJPs agree with central differences of their forward functions;
 * the residual VJP, applied to every unit vector, reproduces the finite-difference Jacobian of
   the residual entry by entry (smooth states; one-sided differences where a limiter or wave-speed
-  switch lies within the step).
+  switch lies within the step);
+* the tangent-linear residual (the residual's arithmetic on value-and-derivative pairs) has the
+  residual itself as its value part, bit for bit, agrees with central differences, and is the
+  transpose of the VJP: <J v, w> = <v, J^T w> to rounding.
 """
 import ctypes as C
 
@@ -126,6 +129,7 @@ def shim(adjoint_shim):
     s.shim_reconstruct1.argtypes = [dp, dp, dp, C.c_int]
     s.shim_face_states.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, C.c_int]
     s.shim_residual_vjp_var.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp]
+    s.shim_residual_jvp.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp, dp]
     return s
 
 
@@ -281,3 +285,52 @@ def test_residual_vjp_is_linear_and_deterministic(shim):
     c = _vjp(shim, lay, flux, phys, U, 2.0 * w1 - 3.0 * w2)
     np.testing.assert_allclose(c, 2.0 * a - 3.0 * b, rtol=1e-12, atol=1e-12 * np.abs(c).max())
     np.testing.assert_array_equal(a, _vjp(shim, lay, flux, phys, U, w1))
+
+
+def _jvp(shim, lay, flux, phys, U, v):
+    Rv, dR = np.zeros_like(U), np.zeros_like(U)
+    rc = shim.shim_residual_jvp(lay.descs, len(lay.blocks), flux, _p(phys), _p(U), _p(v), _p(Rv), _p(dR))
+    assert rc == 0
+    return Rv, dR
+
+
+MIDPOINT = [n for n in CASES if "gauss" not in n]
+
+
+@pytest.mark.parametrize("name", MIDPOINT)
+def test_tangent_value_is_the_residual_and_its_derivative_transposes_the_vjp(shim, name):
+    """The tangent-linear element operations evaluate the residual on value-and-derivative pairs:
+    the value part is the residual bit for bit (so every limiter, Harten and wave-speed branch is the
+    forward's), and <J v, w> equals <v, J^T w> from the hand-derived VJP to rounding, for seeded v, w."""
+    cfg, mesh, md, lay, flux = _setup(name)
+    phys = _phys(cfg)
+    U = _smooth_state(lay, seed=4, mach=0.9 if flux == 1 else 0.4)
+    rng = np.random.default_rng(11)
+    for trial in range(3):
+        v = rng.standard_normal(U.size) * np.abs(U)
+        w = rng.standard_normal(U.size)
+        Rv, dR = _jvp(shim, lay, flux, phys, U, v)
+        if trial == 0:
+            assert np.array_equal(Rv, _residual(shim, lay, flux, phys, U))
+        lhs = float(dR @ w)
+        rhs = float(v @ _vjp(shim, lay, flux, phys, U, w))
+        scale = float(np.abs(dR) @ np.abs(w))
+        assert abs(lhs - rhs) <= 1e-13 * scale, (lhs, rhs, scale)
+
+
+@pytest.mark.parametrize("name", MIDPOINT)
+def test_tangent_matches_central_differences(shim, name):
+    """J v against (R(U + h v) - R(U - h v)) / 2h with h = 1e-7 along a direction scaled like U.
+    The scheme is piecewise smooth: entries whose cells see a switch within the step are rare."""
+    cfg, mesh, md, lay, flux = _setup(name)
+    phys = _phys(cfg)
+    U = _smooth_state(lay, seed=5, mach=0.9 if flux == 1 else 0.4)
+    v = np.random.default_rng(12).standard_normal(U.size) * np.abs(U)
+    _, dR = _jvp(shim, lay, flux, phys, U, v)
+    h = 1e-7
+    fd = (_residual(shim, lay, flux, phys, U + h * v) - _residual(shim, lay, flux, phys, U - h * v)) / (2 * h)
+    err = np.abs(fd - dR)
+    scale = np.abs(dR) + 1e-3 * np.abs(dR).max()
+    bad = err > 1e-5 * scale
+    assert bad.mean() <= 0.02, (bad.sum(), bad.size)
+    assert np.median(err / scale) < 1e-7
