@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 namespace phx {
 
 // A state or flux of 4 components.  The arithmetic below is written once for a scalar type T:
@@ -106,6 +108,19 @@ __host__ __device__ inline void harten(T Rslow, T Rfast, T Lslow, T Lfast, T& l1
     if (fabs(l4) < tm) l4 = 0.5 * ((l4 * l4) / tm + tm);
 }
 
+// A face of a flat region: every jump dW = W_R - W_L is +0 (all bits zero), with a density and a
+// pressure in [2^-300, 2^300] and speeds of at most 2^300.  There every intermediate of the Roe
+// dissipation below is finite and each term of Rc|Lambda|Lp dW is a finite number times +0, so the
+// four row sums are signed zeros: rows 0, 2 and 3 are +0 (each contains a term 1.0*z or Ek*z1 that
+// is +0), and row 1's sign does not matter because 0.5*(F_L1 + F_R1) >= p > 0.  The flux is then
+// exactly 0.5*(F_L + F_R), and roe_flux skips the square roots and divisions of the dissipation.
+__host__ __device__ inline bool roe_flat(double dr, double du, double dv, double dp, const PrimT<double>& L) {
+    const unsigned long long z = __builtin_bit_cast(unsigned long long, dr) | __builtin_bit_cast(unsigned long long, du) |
+                                 __builtin_bit_cast(unsigned long long, dv) | __builtin_bit_cast(unsigned long long, dp);
+    const double lo = 0x1p-300, hi = 0x1p300;
+    return z == 0ull && L.r >= lo && L.r <= hi && L.p >= lo && L.p <= hi && fabs(L.u) <= hi && fabs(L.v) <= hi;
+}
+
 // FluxRoe.compute_flux (flux/Roe.py:262-304) for one face, primitive left/right states.
 //  * Roe state: RoePrimitiveState._roe_state_from_prim_JIT (states/primitive.py:301-320) --
 //    note the sqrt(rho)-weighted pressure.
@@ -122,6 +137,18 @@ __host__ __device__ inline Q4T<T> roe_flux(const PrimT<T>& L, const PrimT<T>& R,
     const T FL3 = L.u * ((ph.one_over_gm1 * L.p + ekL) + L.p);
     const T FR0 = ruR, FR1 = ruR * R.u + R.p, FR2 = ruR * R.v;
     const T FR3 = R.u * ((ph.one_over_gm1 * R.p + ekR) + R.p);
+    if constexpr (std::is_same<T, double>::value) {
+        // flat region: the dissipation is exactly zero (roe_flat); the derivative of the dissipation
+        // is not, so the tangent-linear instantiation always evaluates it
+        if (roe_flat(R.r - L.r, R.u - L.u, R.v - L.v, R.p - L.p, L)) {
+            Q4T<T> F;
+            F.v[0] = 0.5 * (FL0 + FR0);
+            F.v[1] = 0.5 * (FL1 + FR1);
+            F.v[2] = 0.5 * (FL2 + FR2);
+            F.v[3] = 0.5 * (FL3 + FR3);
+            return F;
+        }
+    }
 
     const T sL = sqrt(L.r), sR = sqrt(R.r);
     const T inv = 1.0 / (sL + sR);
@@ -458,10 +485,16 @@ struct ReconT {
 };
 using Recon = ReconT<double>;
 
-// np.minimum of two scalars: a NaN operand is returned (NaN-propagating, unlike fmin).
+// np.minimum of two scalars: a NaN operand is returned (NaN-propagating, unlike fmin).  Written as
+// three selects of plain values: nested conditionals become branches, and the compiler then moves
+// the evaluation of b (a whole limiter chain) into the branch where a is not NaN, serialising the
+// faces' divisions.
 template <class T>
 __host__ __device__ inline T np_minimum(T a, T b) {
-    return (a != a) ? a : ((b != b) ? b : ((a <= b) ? a : b));
+    const bool a_nan = a != a, b_nan = b != b;
+    const T lo = (a <= b) ? a : b;
+    const T r = b_nan ? b : lo;
+    return a_nan ? a : r;
 }
 
 // Slope of one face, SlopeLimiter._compute_slope (limiters/base.py:189-221): dmax/d for d > 0,
@@ -490,9 +523,19 @@ __host__ __device__ inline T limiter_fn(int lim, T s) {
 
 // Limiter value of one face.  GEN = false: Venkatakrishnan (the default, compiled without the
 // limiter switch); GEN = true: the limiter `lim`.
+// Limiter value at the default slope s = 1 (d = 0): Venkatakrishnan 3/4, the other three 1.
+__host__ __device__ inline double limiter_at_one(bool gen, int lim) {
+    return (gen && lim >= 1 && lim <= 3) ? 1.0 : 0.75;
+}
+
+// Where d = 0 the slope is the default 1 and the limiter value the constant limiter_fn(lim, 1)
+// (3/4 = (1 + 2)/(1 + 1 + 2) exactly, or 1), so neither division is evaluated there: a wave whose
+// faces all have d = 0 (a flat region) skips both.
 template <bool GEN, class T>
 __host__ __device__ inline T limit_face(T dmax, T dmin, T d, int lim) {
-    const T s = face_slope(dmax, dmin, d);
+    if (!(d > 0.0 || d < 0.0)) return T(limiter_at_one(GEN, lim));
+    const T num = (d > 0.0) ? dmax : dmin;
+    const T s = num / d;
     return GEN ? limiter_fn(lim, s) : venkatakrishnan_fn(s);
 }
 

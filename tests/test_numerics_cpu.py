@@ -144,3 +144,31 @@ def test_hlll_norm_is_blas_dnrm2_bit_for_bit(numerics_shim):
     numerics_shim.shim_nrm4(len(X), dp(X), dp(got))
     bad = np.nonzero(got.view(np.int64) != ref.view(np.int64))[0]
     assert bad.size == 0, f"{bad.size} of {len(X)} differ, e.g. {X[bad[:3]]}"
+
+
+def test_roe_flat_faces_bitwise(numerics_shim):
+    """Faces whose two states give jumps of +0 take roe_flux's flat-region shortcut,
+    F = 0.5*(F_L + F_R): bit for bit (signs of zero included) the oracle's full Roe formula, on
+    equal states with zero, negative and supersonic velocities, both signs of zero on either side,
+    and densities and pressures at and beyond the shortcut's range limits (2^-300, 2^300)."""
+    rng = np.random.default_rng(99)
+    rows = []
+    for u in (0.0, -0.0, 0.3, -0.7, 5.0, -5.0, 1e-300, -1e-300):
+        for v in (0.0, -0.0, 0.2, -3.0):
+            for rho, p in ((1.0, 1.0), (0.3, 7.0), (2.0 ** -299, 2.0 ** -299), (2.0 ** 299, 2.0 ** 299),
+                           (2.0 ** -301, 1.0), (1.0, 2.0 ** 301), (1.7, 1e-12)):
+                rows.append(((rho, u, v, p), (rho, u, v, p)))
+    # a +0 on one side and -0 on the other: the jump is +0 (flat) or -0 (not flat)
+    for a, b in ((0.0, -0.0), (-0.0, 0.0)):
+        rows.append(((1.2, a, 0.1, 0.9), (1.2, b, 0.1, 0.9)))
+        rows.append(((1.2, 0.4, a, 0.9), (1.2, 0.4, b, 0.9)))
+    W = rand_prim(rng, 200, 3.0)
+    rows += [(tuple(w), tuple(w)) for w in W]
+    WL = np.array([r[0] for r in rows], dtype=np.float64)
+    WR = np.array([r[1] for r in rows], dtype=np.float64)
+    n = len(rows)
+    F = np.zeros((n, 4))
+    numerics_shim.shim_prim_flux(0, n, dp(WL), dp(WR), dp(F), dp(PHYS))
+    with np.errstate(all="ignore"):
+        ref = O.flux_roe(WL.reshape(1, n, 4), WR.reshape(1, n, 4)).reshape(n, 4)
+    assert np.array_equal(F.view(np.int64), ref.view(np.int64))
