@@ -384,6 +384,9 @@ __host__ __device__ inline double ext_sqrt_to_double(Ext s) {
 // Euclidean norm of a 4-vector as dnrm2 computes it (see above).  Non-finite components take the
 // double formula (inf or NaN, as the extended computation gives).
 __host__ __device__ inline double nrm4(double a, double b, double c, double d) {
+    // all components +-0 (the jumps across a face of a flat region): the extended sum is 0 and so
+    // is the norm; the emulation below is skipped by whole waves there
+    if (((dbits(a) | dbits(b) | dbits(c) | dbits(d)) & 0x7fffffffffffffffull) == 0ull) return 0.0;
     const unsigned long long ex = 0x7ff0000000000000ull;
     if ((dbits(a) & ex) == ex || (dbits(b) & ex) == ex || (dbits(c) & ex) == ex || (dbits(d) & ex) == ex)
         return sqrt(((a * a + b * b) + c * c) + d * d);
