@@ -381,20 +381,135 @@ __host__ __device__ inline double ext_sqrt_to_double(Ext s) {
     return ldexp((double)r53, e64 + shift);
 }
 
+// The extended-precision norm by the integer emulation above (finite components).
+__host__ __device__ inline double nrm4_ext(double a, double b, double c, double d) {
+    Ext s = ext_sq(a);
+    s = ext_add(s, ext_sq(b));
+    s = ext_add(s, ext_sq(c));
+    s = ext_add(s, ext_sq(d));
+    return ext_sqrt_to_double(s);
+}
+
+// ---- fast path: the same extended-precision arithmetic carried out in pairs of doubles ----
+// A value v >= 0 with a 64-bit significand is held as hi + lo (doubles, |lo| <= ulp(hi)/2, lo on
+// v's 64-bit grid).  It covers components of magnitude 0 or in [2^-400, 2^400] (the low parts of the
+// squares do not underflow, nothing overflows).  Each rounding to 64 bits is decided from a pair sum
+// accurate to about 2^-104 of the value; where the value lies within 2^-30 of a grid unit of a
+// midpoint of the 64-bit grid (exact ties included), or the square root within one 64-bit unit of a
+// midpoint between two doubles (only there can the rounding to 64 bits change the rounding to
+// double), the result is reported (ok = false) and taken from the integer emulation.
+struct DD {
+    double hi, lo;
+};
+
+// hi + lo = a + b exactly (Knuth's TwoSum)
+__host__ __device__ inline DD two_sum(double a, double b) {
+    const double s = a + b;
+    const double bv = s - a;
+    const double av = s - bv;
+    return DD{s, (a - av) + (b - bv)};
+}
+
+// RN64(hi + lo), hi > 0, |lo| <= ulp(hi)/2: lo rounded to the 64-bit grid of the value's binade,
+// 2^(E-63) for 2^E <= hi < 2^(E+1) (half that when hi = 2^E and lo < 0: the value is then in the
+// binade below).  Adding and subtracting C = 1.5 * 2^52 * grid rounds lo to that grid, ties to even:
+// C's last place is the grid unit and C / grid is even, so the parity is that of the 64-bit
+// significand.  When lo carries a rounding error (exact = false), ok is cleared where lo lies within
+// 2^-30 of a grid unit of a midpoint of the grid: the side of the midpoint is then not known.
+__host__ __device__ inline DD rn64(double hi, double lo, bool exact, bool& ok) {
+    int ex;
+    const double m = frexp(hi, &ex);  // hi = m 2^ex, m in [1/2, 1): E = ex - 1
+    double C = ldexp(0.75, ex - 11);  // 1.5 * 2^(E - 11) = 1.5 * 2^52 * 2^(E - 63)
+    double half = ldexp(1.0, ex - 65);  // grid / 2
+    if (m == 0.5 && lo < 0.0) {
+        C *= 0.5;
+        half *= 0.5;
+    }
+    const double l = (lo + C) - C;
+    if (!exact && fabs(fabs(lo - l) - half) <= half * 0x1p-29) ok = false;
+    return DD{hi, l};
+}
+
+// RN64(x^2), |x| = 0 or in [2^-400, 2^400]: x^2 = p + e exactly (fma)
+__host__ __device__ inline DD sq64(double x, bool& ok) {
+    const double p = x * x;
+    if (p == 0.0) return DD{0.0, 0.0};
+    return rn64(p, fma(x, x, -p), true, ok);
+}
+
+// RN64(A + B) of two 64-bit values >= 0 (x87 fadd).  The exact sum is s.hi + s.lo + A.lo + B.lo;
+// the low terms are summed with their rounding errors e1, e2 tracked, so a sum carried out exactly
+// (every case where the four terms span fewer than about 50 binades) decides ties exactly.
+__host__ __device__ inline DD add64(DD A, DD B, bool& ok) {
+    if (B.hi == 0.0) return A;
+    if (A.hi == 0.0) return B;
+    const DD s = two_sum(A.hi, B.hi);
+    const DD t = two_sum(s.lo, A.lo);
+    const DD w = two_sum(t.hi, B.lo);
+    const double h = s.hi + w.hi;  // renormalised (Fast2Sum, |s.hi| >= |w.hi|): h + (w.hi - (h - s.hi)) exact
+    return rn64(h, w.hi - (h - s.hi), t.lo == 0.0 && w.lo == 0.0, ok);
+}
+
+// RN53(RN64(sqrt(S))) of a 64-bit value S > 0 (x87 fsqrt, then the store to double).  z = sqrt(S)
+// lies within one double unit of y = RN53(sqrt(S.hi)); z - y is known to about 2^-100 of y.  The
+// rounding to 64 bits matters only where z is within half a 64-bit unit of a midpoint between two
+// doubles: it then lands on the midpoint, and the store rounds that tie to the even neighbour.
+__host__ __device__ inline double sqrt64(DD S, bool& ok) {
+    const double y = sqrt(S.hi);
+    const double p2 = y * y, e2 = fma(y, y, -p2);
+    // z^2 - y^2 for z = sqrt(S) (S.hi - p2 is exact by Sterbenz), then z - y
+    const double r = ((S.hi - p2) - e2) + S.lo;
+    const double dlt = r / (2.0 * y);
+    int ey;
+    (void)frexp(y, &ey);                   // y in [2^(ey-1), 2^ey)
+    const double ulp = ldexp(1.0, ey - 53);
+    const double G = ldexp(1.0, ey - 64);  // 64-bit unit of y's binade
+    const bool pow2 = (y == ldexp(0.5, ey));
+    const double ulp_dn = pow2 ? 0.5 * ulp : ulp;
+    const double g_dn = pow2 ? 0.25 * G : 0.5 * G;  // half the 64-bit unit at the lower midpoint
+    const double up = dlt - 0.5 * ulp, dn = dlt + 0.5 * ulp_dn;  // z minus the midpoints
+    const double mu = fabs(up), md = fabs(dn);
+    if (fabs(mu - 0.5 * G) <= G * 0x1p-20 || fabs(md - g_dn) <= G * 0x1p-20) {  // not decided
+        ok = false;
+        return y;
+    }
+    const bool odd = (dbits(y) & 1ull) != 0ull;
+    if (mu < 0.5 * G) return odd ? y + ulp : y;    // RN64(z) is the upper midpoint
+    if (md < g_dn) return odd ? y - ulp_dn : y;     // RN64(z) is the lower midpoint
+    if (up > 0.0) return y + ulp;
+    if (dn < 0.0) return y - ulp_dn;
+    return y;
+}
+
+__host__ __device__ inline bool nrm4_in_range(double x) {
+    const double t = fabs(x);
+    return t == 0.0 || (t >= 0x1p-400 && t <= 0x1p400);
+}
+
 // Euclidean norm of a 4-vector as dnrm2 computes it (see above).  Non-finite components take the
-// double formula (inf or NaN, as the extended computation gives).
-__host__ __device__ inline double nrm4(double a, double b, double c, double d) {
+// double formula (inf or NaN, as the extended computation gives).  Components within [2^-400, 2^400]
+// take the pair-of-doubles evaluation, which decides all but about one in 10^5 of them; the rest, and
+// components outside that range, take the integer emulation.  Force-inlined: the kernels make no
+// device-function calls.
+__host__ __device__ __forceinline__ double nrm4(double a, double b, double c, double d) {
     // all components +-0 (the jumps across a face of a flat region): the extended sum is 0 and so
     // is the norm; the emulation below is skipped by whole waves there
     if (((dbits(a) | dbits(b) | dbits(c) | dbits(d)) & 0x7fffffffffffffffull) == 0ull) return 0.0;
     const unsigned long long ex = 0x7ff0000000000000ull;
     if ((dbits(a) & ex) == ex || (dbits(b) & ex) == ex || (dbits(c) & ex) == ex || (dbits(d) & ex) == ex)
         return sqrt(((a * a + b * b) + c * c) + d * d);
-    Ext s = ext_sq(a);
-    s = ext_add(s, ext_sq(b));
-    s = ext_add(s, ext_sq(c));
-    s = ext_add(s, ext_sq(d));
-    return ext_sqrt_to_double(s);
+    bool ok = nrm4_in_range(a) && nrm4_in_range(b) && nrm4_in_range(c) && nrm4_in_range(d);
+    if (ok) {
+        DD s = sq64(a, ok);
+        s = add64(s, sq64(b, ok), ok);
+        s = add64(s, sq64(c, ok), ok);
+        s = add64(s, sq64(d, ok), ok);
+        if (ok) {
+            const double r = sqrt64(s, ok);
+            if (ok) return r;
+        }
+    }
+    return nrm4_ext(a, b, c, d);
 }
 
 // FluxHLLL.compute_flux + _HLLL_flux_JIT (flux/HLLL.py:25-49,70-103) for one face.

@@ -52,6 +52,26 @@ void shim_reconstruct_lim(int lim, int n, const double* U, const double* Uw, con
 void shim_nrm4(int n, const double* X, double* out) {
     for (int i = 0; i < n; ++i) out[i] = nrm4(X[4 * i], X[4 * i + 1], X[4 * i + 2], X[4 * i + 3]);
 }
+// the same norm through the 64-bit integer emulation alone (nrm4's fallback path)
+void shim_nrm4_ext(int n, const double* X, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = nrm4_ext(X[4 * i], X[4 * i + 1], X[4 * i + 2], X[4 * i + 3]);
+}
+// nrm4's fast-path sum of squares RN64(RN64(RN64(a^2 + b^2) + c^2) + d^2) as hi + lo, and whether
+// the pair arithmetic decided every rounding (ok = 1) or nrm4 would take the emulation (ok = 0)
+void shim_sum64(int n, const double* X, double* hi, double* lo, int* okv) {
+    for (int i = 0; i < n; ++i) {
+        const double* x = X + 4 * i;
+        bool ok = nrm4_in_range(x[0]) && nrm4_in_range(x[1]) && nrm4_in_range(x[2]) && nrm4_in_range(x[3]);
+        DD s{0.0, 0.0};
+        if (ok) {
+            s = sq64(x[0], ok);
+            for (int k = 1; k < 4; ++k) s = add64(s, sq64(x[k], ok), ok);
+        }
+        hi[i] = s.hi;
+        lo[i] = s.lo;
+        okv[i] = ok ? 1 : 0;
+    }
+}
 void shim_cell_dt(int n, const double* U, const double* dx, const double* dy, double* out, const double* phys) {
     const Phys ph = make_phys(phys);
     for (int i = 0; i < n; ++i) out[i] = cell_dt(q4(U + 4 * i), dx[i], dy[i], ph);

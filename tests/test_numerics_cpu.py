@@ -146,6 +146,81 @@ def test_hlll_norm_is_blas_dnrm2_bit_for_bit(numerics_shim):
     assert bad.size == 0, f"{bad.size} of {len(X)} differ, e.g. {X[bad[:3]]}"
 
 
+def _near_midpoint_vectors(rng, n):
+    """(v, w, t, 0) with v^2 + w^2 close to (v + ulp(v)/2)^2: norms within a few 64-bit units of a
+    midpoint between two doubles, where the x87 sum's rounding to 64 bits decides the result; random
+    signs and orders, a third component 2^-20..2^-60 below w on half of them."""
+    v = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-100, 100, n))
+    u = np.spacing(v)
+    pert = np.where(rng.random(n) < 0.5, rng.uniform(-1e-6, 1e-6, n),
+                    np.ldexp(rng.uniform(-1.0, 1.0, n), -rng.integers(0, 60, n)))
+    w = np.sqrt(v * u + 0.25 * u * u) * (1.0 + pert)
+    t = np.where(rng.random(n) < 0.5, 0.0, w * np.ldexp(rng.uniform(0.5, 1.0, n), -rng.integers(20, 60, n)))
+    X = np.stack([v, w, t, np.zeros(n)], axis=1) * np.where(rng.random((n, 4)) < 0.5, -1.0, 1.0)
+    swap = rng.random(n) < 0.5
+    X[swap, 0], X[swap, 1] = X[swap, 1].copy(), X[swap, 0].copy()
+    return X
+
+
+def test_hlll_norm_paths_at_rounding_ties(numerics_shim):
+    """nrm4 evaluates the extended sum in pairs of doubles and takes the integer emulation only where
+    that cannot decide a rounding.  Both paths equal scipy.linalg.blas.dnrm2 bit for bit on the inputs
+    that stress the decisions: norms at midpoints between doubles (double rounding, ties to even),
+    sums of powers of two (exact 64-bit ties in the additions), short-mantissa integers, spreads of
+    up to 2^160 between components and magnitudes at the fast path's range limits (2^-400, 2^400)."""
+    from scipy.linalg import blas
+    rng = np.random.default_rng(47)
+    n = 60000
+    X = [_near_midpoint_vectors(rng, n)]
+    X.append(np.ldexp(np.where(rng.random((n, 4)) < 0.5, -1.0, 1.0), rng.integers(-30, 30, (n, 4))))
+    X.append(np.ldexp(rng.integers(1, 2 ** 30, (n, 4)).astype(float), rng.integers(-20, 20, (n, 4))))
+    X.append(np.where(rng.random((n, 4)) < 0.3, 0.0,
+                      rng.uniform(-1, 1, (n, 4)) * np.exp2(rng.uniform(-80, 80, (n, 4)))))
+    X.append(rng.uniform(-1, 1, (n, 4)) * np.exp2(rng.uniform(-420, 420, (n, 4))))
+    X.append(np.array([[2.0 ** 400, 2.0 ** -400, 0.0, 0.0], [2.0 ** 401, 1.0, 0.0, 0.0],
+                       [2.0 ** -400, 2.0 ** -401, 0.0, 0.0], [1.0, 2.0 ** -27, 0.0, 0.0],
+                       [1.0, 2.0 ** -32, 2.0 ** -32, 0.0]]))
+    # 1 + 2^-62 + b^2, b = 2^-32 (1 + k 2^-52): the sum is 2^-115 k above a 64-bit tie whose even
+    # neighbour lies below, and the low parts span 54 bits, so the pair sum alone rounds the wrong way
+    X.append(np.array([[1.0, 2.0 ** -31, 2.0 ** -32 * (1.0 + k * 2.0 ** -52), 0.0] for k in (1, 3, 5, 7)]))
+    X = np.ascontiguousarray(np.vstack(X))
+    ref = np.array([blas.dnrm2(x) for x in X])
+    for name in ("shim_nrm4", "shim_nrm4_ext"):
+        got = np.zeros(len(X))
+        getattr(numerics_shim, name)(len(X), dp(X), dp(got))
+        bad = np.nonzero(got.view(np.int64) != ref.view(np.int64))[0]
+        assert bad.size == 0, f"{name}: {bad.size} of {len(X)} differ, e.g. {X[bad[:3]]}"
+
+
+def test_hlll_norm_pair_sum_is_the_x87_sum(numerics_shim):
+    """nrm4's fast path holds the x87 sum of squares (each product and sum rounded to a 64-bit
+    significand) as a pair of doubles.  Where it reports the sum decided, hi + lo equals NumPy's
+    80-bit long double evaluation of ((a^2 + b^2) + c^2) + d^2 exactly: on seeded vectors, on ties
+    from powers of two and short integers, and on sums 2^-115 k above a 64-bit tie whose low parts
+    span 54 bits (there the rounded pair sum alone would round to the wrong side).  The fast path
+    decides at least 99.5% of the seeded vectors."""
+    if np.finfo(np.longdouble).nmant != 63:
+        pytest.skip("long double is not the x87 80-bit format on this host")
+    rng = np.random.default_rng(53)
+    n = 50000
+    X = [rng.standard_normal((n, 4)) * 10.0 ** rng.uniform(-6, 6, (n, 1)),
+         np.ldexp(np.where(rng.random((n, 4)) < 0.5, -1.0, 1.0), rng.integers(-30, 30, (n, 4))),
+         np.ldexp(rng.integers(1, 2 ** 30, (n, 4)).astype(float), rng.integers(-20, 20, (n, 4))),
+         np.array([[1.0, 2.0 ** -31, 2.0 ** -32 * (1.0 + k * 2.0 ** -52), 0.0] for k in (1, 3, 5, 7)])]
+    X = np.ascontiguousarray(np.vstack(X))
+    m = len(X)
+    hi, lo, ok = np.zeros(m), np.zeros(m), np.zeros(m, dtype=np.int32)
+    numerics_shim.shim_sum64(m, dp(X), dp(hi), dp(lo), ok.ctypes.data_as(C.POINTER(C.c_int)))
+    L = X.astype(np.longdouble)
+    ref = ((L[:, 0] * L[:, 0] + L[:, 1] * L[:, 1]) + L[:, 2] * L[:, 2]) + L[:, 3] * L[:, 3]
+    got = hi.astype(np.longdouble) + lo.astype(np.longdouble)
+    dec = ok == 1
+    bad = np.nonzero(dec & (got != ref))[0]
+    assert bad.size == 0, f"{bad.size} decided sums differ, e.g. {X[bad[:3]]}"
+    assert dec[:n].mean() >= 0.995
+    assert not dec[-4:].any()   # the 54-bit low-part cases are left to the emulation
+
+
 def test_roe_flat_faces_bitwise(numerics_shim):
     """Faces whose two states give jumps of +0 take roe_flux's flat-region shortcut,
     F = 0.5*(F_L + F_R): bit for bit (signs of zero included) the oracle's full Roe formula, on
