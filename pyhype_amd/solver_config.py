@@ -81,12 +81,10 @@ class SolverConfig:
         self.write_every_n_timesteps = write_every_n_timesteps
         # Generic2/Generic3 read config.alpha (time_marching/explicit_runge_kutta.py:114,155)
         self.alpha = alpha
-        if show_log_for_procs is None:
-            self.show_log_for_procs = [0]
-        elif show_log_for_procs == "all":
-            self.show_log_for_procs = "all"
-        else:
-            self.show_log_for_procs = show_log_for_procs
+        # unset: the log of rank 0 only; a list of ranks, or "all", is kept as given
+        self.show_log_for_procs = [0] if show_log_for_procs is None else show_log_for_procs
 
     def __str__(self):
-        return "".join(f"\t{atr}: {getattr(self, atr)}\n" for atr in self.__slots__)
+        # one tab-indented "name: value" line per field, in slot order
+        lines = [f"\t{name}: {getattr(self, name)}" for name in self.__slots__]
+        return "\n".join(lines) + "\n"

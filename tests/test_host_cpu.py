@@ -208,6 +208,24 @@ def test_config_validation_errors():
         validate_config(cfg)
 
 
+def test_config_log_ranks_and_printout():
+    """SolverConfig (solver_config.py:130-138): show_log_for_procs defaults to rank 0 and keeps a
+    list or "all" as given; str() prints one tab-indented "name: value" line per slot."""
+    cfg, _ = W.explosion_single(8)
+    assert cfg.show_log_for_procs == [0]
+    text = str(cfg)
+    lines = text.split("\n")
+    assert text.endswith("\n") and lines[-1] == "" and len(lines) == len(cfg.__slots__) + 1
+    for name, line in zip(cfg.__slots__, lines):
+        assert line == f"\t{name}: {getattr(cfg, name)}"
+    from pyhype_amd.solver_config import SolverConfig
+    import inspect
+    args = {k: getattr(cfg, k) for k in inspect.signature(SolverConfig.__init__).parameters
+            if k not in ("self", "show_log_for_procs")}
+    for v in ("all", [1, 3]):
+        assert SolverConfig(**args, show_log_for_procs=v).show_log_for_procs == v
+
+
 def test_backend_butcher_tables_equal_the_oracle_tables():
     """Every table the backend accepts is the oracle's table, entry for entry (same doubles)."""
     from types import SimpleNamespace
