@@ -15,8 +15,8 @@ HIP events on the launch stream; ``roofline.valu`` prices the kernel's FP64 inst
 cell-stage, from the SQ counters kept under profiles/) against the FP64 issue rate.
 ``cpu_baseline`` times the NumPy oracle on the host: one core, and all cores (one process per
 core, each on its own case).  ``extra`` carries the other workloads (8x150^2 multiblock
-explosion, 5x500^2 HLLL oblique shock, 100^2 single block) and ``adjoint`` the discrete-adjoint
-costs on the 256^2 SSP-RK2 problem.
+explosion, 5x500^2 HLLL oblique shock, 100^2 single block, the 256^2 SSP-RK2 forward of the
+adjoint problem) and ``adjoint`` the discrete-adjoint costs on that 256^2 problem.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 """
@@ -38,7 +38,7 @@ HBM_PEAK_GBS = 8000.0
 # FP64 VALU issue rate of MI355X: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz (78.6 TFLOP/s counting FMA twice)
 F64_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9
 PROFILES = ROOT / "profiles"
-EXTRA = ("explosion_multiblock", "oblique_shock", "explosion_single")
+EXTRA = ("explosion_multiblock", "oblique_shock", "explosion_single", "explosion_adjoint")
 MAX_CPU_CORES = 16   # the host-core share of one GPU
 
 

@@ -804,7 +804,7 @@ def _axpy(out, x, c, r):
 
 
 class Simulation:
-    """Step-count driver with the loop semantics of Euler2D._solve (solvers/Euler2D.py:175-216)
+    """Step-count loop with the semantics of Euler2D._solve (solvers/Euler2D.py:175-216)
     and Solver.get_dt (solvers/base.py:114-136)."""
 
     def __init__(self, mesh: Dict[int, dict], nx: int, ny: int, *, flux: str = "Roe",

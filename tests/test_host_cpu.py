@@ -244,9 +244,9 @@ def test_backend_butcher_tables_equal_the_oracle_tables():
 
 
 def test_mesh_dict_recovered_from_block_descriptions():
-    """The reference's solver keeps BlockDescription objects only (solvers/base.py:90-100); the
-    binding in INTEGRATION.md recovers the mesh dictionary from their info / geometry.vertices /
-    neighbors / bc attributes.  Stand-ins with the same attribute structure round-trip the 2x4 mesh."""
+    """The reference's solver keeps BlockDescription objects only (solvers/base.py:90-100); a binding
+    inside the reference's solver recovers the mesh dictionary from their info / geometry.vertices /
+    neighbors / bc attributes (mesh_dict_from_block_descriptions).  Stand-ins with the same attribute structure round-trip the 2x4 mesh."""
     from types import SimpleNamespace as NS
 
     from pyhype_amd.mesh import mesh_dict_from_block_descriptions
