@@ -16,13 +16,26 @@ construction.  Every function cites the reference file:line it restates
 Pinning (what checks this module, see tests/test_oracle_*.py):
   * geometry: closed forms for axis-aligned blocks, angle/normal identities;
   * Green-Gauss: exact on linear fields in interior cells;
-  * limiter: closed-form Venkatakrishnan values, 1-D brute force;
+  * limiter: closed-form Venkatakrishnan values, the s = 1 default, 1-D brute force;
   * Roe/HLLL: consistency F(W,W)=F(W), mirror symmetry, supersonic branches,
-    an independent dense-matrix restatement of the eigen-decomposition;
+    an independent dense-matrix restatement of the eigen-decomposition, and
+    hand-derived closed forms of the reference's own arithmetic: the
+    sqrt(rho)-weighted Roe pressure (states/primitive.py:301-320), the
+    Harten-corrected eigenvalues inside Rc (flux/Roe.py:175-220), the shear
+    strength dv of the unset Lp(2,2) entry (flux/Roe.py:222-251);
+  * HLLL norm: bit for bit equal to scipy.linalg.blas.dnrm2 (flux/HLLL.py:86-93);
+  * inter-block faces: the neighbour's cell average as outer state
+    (fvm/base.py:305-325), through a residual that ignores the neighbour's
+    second column;
   * residual: rest-state invariance, conservation with reflecting walls;
   * RK: stability polynomials on a linear residual;
-  * end-to-end: golden fixtures in tests/golden/ when present.
-Functions without such a check are marked "parity unpinned" below.
+  * a mutation test (tests/test_oracle_mutations.py) applies plausible
+    misreadings of these functions to this module and asserts that a pin fails
+    for each;
+  * end-to-end: golden fixtures in tests/golden/ (regression digests written
+    by this module; they pin nothing about the reference by themselves).
+Every function on the path has such a check; one without would be marked
+"parity unpinned" here.
 """
 from __future__ import annotations
 
