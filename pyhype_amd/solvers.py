@@ -182,12 +182,16 @@ class Euler2DBatch:
         With ``strict`` a RealizabilityError is raised instead."""
         if n <= 0:
             return
-        self.plan.run(n, use_graph=use_graph, check=self.strict)
-        bad, step, t, dt = self.plan.status()
-        self.t = t
-        self.dt_last = dt
-        self.num_time_step = step
-        self.failed = {c: int(b) for c, b in enumerate(bad) if b >= 0}
+        try:
+            self.plan.run(n, use_graph=use_graph, check=self.strict)
+        finally:
+            # also after a RealizabilityError: time, step counter and the failing step of each case
+            # describe the state the plan holds
+            bad, step, t, dt = self.plan.status()
+            self.t = t
+            self.dt_last = dt
+            self.num_time_step = step
+            self.failed = {c: int(b) for c, b in enumerate(bad) if b >= 0}
 
     def autotune(self, steps: int = 10, **kw):
         """Tune the launch shape on the current state (state, time and step are restored)."""

@@ -1,0 +1,66 @@
+"""The reference's example runs on the HIP path, through ``Euler2D.solve()`` (the drop-in entry point:
+the step loop of solvers/Euler2D.py:195-210 with dt clipped to t_final - t), against the oracle's
+digests in tests/golden/example_<name>.npz (written by tests/golden/make_examples.py on the CPU).
+
+* explosion: 40x40 on [0,10]x[0,20], Roe/RK4 CFL 0.7 to t_final 0.07 (examples/explosion);
+* explosion_multi: 2x4 blocks of 150x150, Roe/RK4 CFL 0.7 to t_final 0.07 (examples/explosion_multi);
+* implosion: 40x40, Roe/RK2 CFL 0.4 to t_final 0.1 (examples/implosion);
+* shockbox: one 50x50 block numbered 1, Roe/RK2 CFL 0.4 to t_final 2.0 (examples/shockbox);
+* dmr: 4x1 blocks of 50x50 on the ramp, HLLL/RK2/PrimitiveState CFL 0.4 to t_final 0.25
+  (examples/dmr).
+
+The bar is bit-exact: the same number of steps, the same final time (the last step is the clipped
+one), and every block's final state with the oracle's sha256.  A run that the scheme cannot
+complete (the shock box becomes unrealizable at step 24 in the oracle) must raise
+RealizabilityError at the same step and hold the state that step produced.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.golden import make_examples as ME
+from tests.golden import make_fullsize as MF
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+
+def _fixture(name):
+    path = ME.HERE / f"example_{name}.npz"
+    if not path.exists():
+        pytest.fail(f"missing fixture {path.name}: run tests/golden/make_examples.py {name}")
+    with np.load(path) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name", ME.EXAMPLES)
+def test_example_run_to_final_time_matches_the_oracle_bit_for_bit(name):
+    from pyhype_amd import Euler2D, RealizabilityError
+
+    f = _fixture(name)
+    cfg, mesh, cap = ME.example(name)
+    print(f"example run: {name}, {len(mesh)} block(s) of {cfg.nx}x{cfg.ny}, "
+          f"{int(f['steps'])} steps in the oracle", flush=True)
+    solver = Euler2D(cfg, mesh)
+    solver.apply_initial_condition()
+    states0 = solver.states()[0]
+    assert [MF.sha(states0[int(n)]) for n in f["blocks"]] == list(f["sha0"]), "initial states differ"
+    failed = int(f["failed"])
+    if failed < 0:
+        solver.solve(max_steps=cap)
+        assert solver.num_time_step == int(f["steps"]), f"{solver.num_time_step} steps, oracle {int(f['steps'])}"
+        assert solver.t[0] == f["t"], f"final time {solver.t[0]!r} != {f['t']!r}"
+    else:
+        with pytest.raises(RealizabilityError):
+            solver.solve(max_steps=cap)
+        assert solver.failed == {0: failed}, f"failed at {solver.failed}, oracle at step {failed}"
+    states = solver.states()[0]
+    sha = [MF.sha(states[int(n)]) for n in f["blocks"]]
+    bad = [int(n) for n, a, b in zip(f["blocks"], sha, f["sha"]) if a != b]
+    if bad:
+        worst = 0.0
+        for i, n in enumerate(f["blocks"]):
+            U = states[int(n)]
+            r = f["absmax"][i]
+            worst = max(worst, float(np.max(np.abs(np.max(np.abs(U), axis=(0, 1)) - r) / np.maximum(r, 1e-300))))
+        pytest.fail(f"final states of blocks {bad} differ (per-block max |U| rel. diff up to {worst:.3e})")
