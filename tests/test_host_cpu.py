@@ -267,3 +267,42 @@ def test_mesh_dict_recovered_from_block_descriptions():
             if k.startswith("Neighbor") and len(k) == 10:   # corner neighbours: BlockInfo keeps none
                 continue
             assert got[n][k] == v, (n, k)
+
+
+# Settings of the reference's example configurations (examples/<name>/config.py and mesh.py):
+# (nx, ny, flux, integrator, reconstruction, CFL, t_final, a_inf, blocks, domain corners of block 0)
+EXAMPLE_SETTINGS = {
+    # examples/explosion/config.py:13-31, mesh.py (one block, NW [0, 20], NE [10, 20])
+    "explosion": (40, 40, "Roe", "RK4", "ConservativeState", 0.7, 0.07, 343.0, [0], ([0, 0], [10, 20])),
+    # examples/explosion_multi/config.py:13-31, mesh.py:3-14 (2 x 4 blocks on [0,10] x [0,20])
+    "explosion_multi": (150, 150, "Roe", "RK4", "ConservativeState", 0.7, 0.07, 343.0, list(range(8)),
+                        ([0, 0], [5, 5])),
+    # examples/implosion/config.py:15-32, mesh.py (one block on [0,10]^2)
+    "implosion": (40, 40, "Roe", "RK2", "ConservativeState", 0.4, 0.1, 343.0, [0], ([0, 0], [10, 10])),
+    # examples/shockbox/config.py:15-32, mesh.py (one block on [0,10]^2, numbered 1)
+    "shockbox": (50, 50, "Roe", "RK2", "ConservativeState", 0.4, 2.0, 343.0, [1], ([0, 0], [10, 10])),
+    # examples/dmr/config.py:13-30, mesh.py:1-33 (4 x 1 blocks over the 30 degree ramp)
+    "dmr": (50, 50, "HLLL", "RK2", "PrimitiveState", 0.4, 0.25, 343.0, [0, 1, 2, 3], ([0, 0], [1, 2 / np.sqrt(3)])),
+}
+
+
+def test_example_runs_use_the_reference_example_settings():
+    """tests/golden/make_examples.py builds each example with the settings of the reference's example
+    files: mesh size, flux, integrator, reconstruction, CFL, final time, fluid, block numbering and the
+    extent of the first block; Venkatakrishnan, Green-Gauss, one Gauss point everywhere."""
+    from pyhype_amd.solvers import mesh_dict
+    from tests.golden import make_examples as ME
+
+    assert set(ME.EXAMPLES) <= set(EXAMPLE_SETTINGS)
+    for name, (nx, ny, flux, integ, recon, cfl, t_final, a_inf, blocks, (sw, ne)) in EXAMPLE_SETTINGS.items():
+        cfg, mesh, cap = ME.example(name)
+        assert (cfg.nx, cfg.ny, cfg.fvm_flux_function_type, cfg.time_integrator) == (nx, ny, flux, integ), name
+        assert cfg.reconstruction_type.__name__ == recon, name
+        assert (cfg.CFL, cfg.t_final, cfg.fluid.far_field.a) == (cfl, t_final, a_inf), name
+        assert (cfg.fvm_slope_limiter_type, cfg.fvm_gradient_type, cfg.fvm_num_quadrature_points,
+                cfg.fvm_spatial_order) == ("Venkatakrishnan", "GreenGauss", 1, 2), name
+        assert cap is None, name
+        md = mesh_dict(mesh)
+        assert sorted(md) == blocks, name
+        b0 = md[blocks[0]]
+        assert np.allclose(b0["SW"], sw) and np.allclose(b0["NE"], ne), (name, b0["SW"], b0["NE"])
