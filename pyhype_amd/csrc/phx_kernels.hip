@@ -1736,35 +1736,57 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
 #undef PHX_LAUNCH_M
 }
 
+// One phase of a stage over both geometry kinds: launch(kind, stream) for each kind that has work.
+// With work of both kinds, kind 1 runs on the plan's side stream concurrently with kind 0 on st
+// (fork and join through events, which a graph capture turns into parallel branches): the kinds
+// write disjoint blocks and read only what earlier phases wrote (inter-block outer states are cell
+// averages of the stage input), and their per-case CFL and realizability reductions are atomic
+// minima, so the order of the two launches does not change any result.  On mixed meshes of a few
+// thousand cells the launches, not the arithmetic, set the time of a phase.
+template <class F>
+void for_kinds(phx_plan* P, bool has0, bool has1, hipStream_t st, F&& launch) {
+    if (has0 && has1 && P->side_stream && hipEventRecord(P->ev_fork, st) == hipSuccess &&
+        hipStreamWaitEvent(P->side_stream, P->ev_fork, 0) == hipSuccess) {
+        launch(1, P->side_stream);
+        launch(0, st);
+        if (hipEventRecord(P->ev_join, P->side_stream) != hipSuccess ||
+            hipStreamWaitEvent(st, P->ev_join, 0) != hipSuccess)
+            (void)hipStreamSynchronize(P->side_stream);  // join on the host instead
+        return;
+    }
+    if (has0) launch(0, st);
+    if (has1) launch(1, st);
+}
+
 void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
     if (P->variant == 8) {
         A.sn = P->sn;
         A.p1 = P->p1;
         A.rows_wave = flat_threads(P) / WAVE - 1;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_f[kind].empty())
-                launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
-                                               P->grid_f[kind], P->waves_per_simd, st);
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty()) {
-                A.fq = P->flat_q[kind];
-                launch_kind<k_yface_update_flat_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                     P->grid_b[kind], 2, st);
-            }
+        for_kinds(P, !P->items_f[0].empty(), !P->items_f[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                           P->grid_f[kind], P->waves_per_simd, s);
+        });
+        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+            StageArgs Ak = A;
+            Ak.fq = P->flat_q[kind];
+            launch_kind<k_yface_update_flat_tag>(P, Ak, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                 P->grid_b[kind], 2, s);
+        });
         return;
     }
     if (P->variant == 7) {
         A.sn = P->sn;
         A.p1 = P->p1;
         A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_f[kind].empty())
-                launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
-                                               P->grid_f[kind], P->waves_per_simd, st);
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty())
-                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                   P->grid_b[kind], 2, st);
+        for_kinds(P, !P->items_f[0].empty(), !P->items_f[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                           P->grid_f[kind], P->waves_per_simd, s);
+        });
+        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                               P->grid_b[kind], 2, s);
+        });
         return;
     }
     if (P->variant == 6) {
@@ -1772,116 +1794,110 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
         A.p1 = P->p1;
         A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
         const bool gauss = P->ph.nq > 1;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                                 P->grid[kind], 2, st);
-                else
-                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                               P->grid[kind], P->waves_per_simd, st);
-            }
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_yface_update_wg_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                         P->grid_b[kind], 2, st);
-                else
-                    launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                       P->grid_b[kind], 2, st);
-            }
+        for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                             P->grid[kind], 2, s);
+            else
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                           P->grid[kind], P->waves_per_simd, s);
+        });
+        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_yface_update_wg_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                     P->grid_b[kind], 2, s);
+            else
+                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                   P->grid_b[kind], 2, s);
+        });
         return;
     }
     if (P->variant == 5) {
         A.sn = P->sn;
         A.p1 = P->p1;
         const bool gauss = P->ph.nq > 1;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                                 P->grid[kind], 2, st);
-                else
-                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                               P->grid[kind], P->waves_per_simd, st);
-            }
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_yface_update_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                      P->grid_b[kind], 2, st);
-                else
-                    launch_kind<k_yface_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                                    P->grid_b[kind], 2, st);
-            }
+        for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                             P->grid[kind], 2, s);
+            else
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                           P->grid[kind], P->waves_per_simd, s);
+        });
+        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_yface_update_q_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                  P->grid_b[kind], 2, s);
+            else
+                launch_kind<k_yface_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                P->grid_b[kind], 2, s);
+        });
         return;
     }
     if (P->variant == 1) {
         A.sn = P->sn;
         A.p1 = P->p1;
         const bool gauss = P->ph.nq > 1;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                                 P->grid[kind], 2, st);
-                else
-                    launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
-                                               P->grid[kind], P->waves_per_simd, st);
-            }
+        for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                             P->grid[kind], 2, s);
+            else
+                launch_kind<k_recon_x_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
+                                           P->grid[kind], P->waves_per_simd, s);
+        });
         A.yf = P->yf;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_f[kind].empty()) {
-                if (gauss)
-                    launch_kind<k_yface_q_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
-                                               P->grid_f[kind], 2, st);
-                else
-                    launch_kind<k_yface_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
-                                             P->grid_f[kind], 2, st);
-            }
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_b[kind].empty())
-                launch_kind<k_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
-                                          2, st);
+        for_kinds(P, !P->items_f[0].empty(), !P->items_f[1].empty(), st, [&](int kind, hipStream_t s) {
+            if (gauss)
+                launch_kind<k_yface_q_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                           P->grid_f[kind], 2, s);
+            else
+                launch_kind<k_yface_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                         P->grid_f[kind], 2, s);
+        });
+        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_update_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(), P->grid_b[kind],
+                                      2, s);
+        });
         return;
     }
     if (P->variant == 3) {
         A.rb = P->ew_rb;
         A.ff = P->ew_ff;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_e[0][kind].empty())
-                launch_kind<k_ew_recon_tag>(P, A, kind, P->ditems_e[0][kind], (int)P->items_e[0][kind].size(),
-                                            P->grid_e[0][kind], 2, st);
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_e[1][kind].empty())
-                launch_kind<k_ew_face_tag>(P, A, kind, P->ditems_e[1][kind], (int)P->items_e[1][kind].size(),
-                                           P->grid_e[1][kind], 2, st);
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items_e[2][kind].empty())
-                launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
-                                             P->grid_e[2][kind], 2, st);
+        for_kinds(P, !P->items_e[0][0].empty(), !P->items_e[0][1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_ew_recon_tag>(P, A, kind, P->ditems_e[0][kind], (int)P->items_e[0][kind].size(),
+                                        P->grid_e[0][kind], 2, s);
+        });
+        for_kinds(P, !P->items_e[1][0].empty(), !P->items_e[1][1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_ew_face_tag>(P, A, kind, P->ditems_e[1][kind], (int)P->items_e[1][kind].size(),
+                                       P->grid_e[1][kind], 2, s);
+        });
+        for_kinds(P, !P->items_e[2][0].empty(), !P->items_e[2][1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
+                                         P->grid_e[2][kind], 2, s);
+        });
         return;
     }
     if (P->variant == 4) {
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty())
-                launch_kind<k_stage_pair_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                              std::min(P->waves_per_simd, 3), st);
+        for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_stage_pair_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                          std::min(P->waves_per_simd, 3), s);
+        });
         return;
     }
     if (P->variant == 2) {
         A.rows_wave = P->rows_per_item > 0 ? P->rows_per_item : 1;
-        for (int kind = 0; kind < 2; ++kind)
-            if (!P->items[kind].empty())
-                launch_kind<k_stage_wg_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                            2, st);
+        for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+            launch_kind<k_stage_wg_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                        2, s);
+        });
         return;
     }
     // the fused kernel needs more than 128 registers: at most 3 waves per SIMD
-    for (int kind = 0; kind < 2; ++kind)
-        if (!P->items[kind].empty())
-            launch_kind<k_stage_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
-                                     std::min(P->waves_per_simd, 3), st);
+    for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
+        launch_kind<k_stage_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(), P->grid[kind],
+                                 std::min(P->waves_per_simd, 3), s);
+    });
 }
 
 void launch_dt_reduce(phx_plan* P, const double* U, hipStream_t st) {
@@ -1960,6 +1976,9 @@ int phx_plan_destroy(phx_plan* P) {
     if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
     if (P->ev_a) (void)hipEventDestroy(P->ev_a);
     if (P->ev_b) (void)hipEventDestroy(P->ev_b);
+    if (P->side_stream) (void)hipStreamDestroy(P->side_stream);
+    if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+    if (P->ev_join) (void)hipEventDestroy(P->ev_join);
     void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->ditems_f[0],
                     P->ditems_f[1], P->sn, P->p1, P->yf, P->ew_rb, P->ew_ff, P->ditems_e[0][0], P->ditems_e[0][1],
                     P->ditems_e[1][0], P->ditems_e[1][1], P->ditems_e[2][0], P->ditems_e[2][1],
@@ -2125,6 +2144,9 @@ int phx_plan_create(const phx_plan_desc* D, phx_plan** out) {
     PHX_CKP(hipStreamCreateWithFlags(&P->cap_stream, hipStreamNonBlocking));
     PHX_CKP(hipEventCreateWithFlags(&P->ev_a, hipEventDisableTiming));
     PHX_CKP(hipEventCreateWithFlags(&P->ev_b, hipEventDisableTiming));
+    PHX_CKP(hipStreamCreateWithFlags(&P->side_stream, hipStreamNonBlocking));
+    PHX_CKP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+    PHX_CKP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
 #undef PHX_CKP
     *out = P;
     return PHX_OK;

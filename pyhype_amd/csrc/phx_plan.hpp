@@ -109,6 +109,10 @@ struct phx_plan {
     long long* dghost_off = nullptr;
     hipStream_t cap_stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    // the second geometry kind of a phase runs on side_stream, concurrently with the first
+    // (fork/join through ev_fork/ev_join; captured into the step graphs as parallel branches)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // captured steps: [0] one step, [1] GRAPH_STEPS steps (replayed by phx_run)
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
