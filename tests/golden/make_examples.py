@@ -21,8 +21,8 @@ Each file keeps the number of steps, the final time, every step's dt, the sha256
 initial and final state bytes (NaNs canonical), per-block maxima and sums and 64 seeded sample
 cells per block (the same digest as tests/golden/make_fullsize.py).
 
-Run from the repository root (about 20 minutes on one core per example, most of it
-``explosion_multi``):
+Run from the repository root (one process per example; ``explosion_multi`` takes about an hour, the
+others under a minute each):
     python tests/golden/make_examples.py [name ...]
 """
 from __future__ import annotations
@@ -41,7 +41,7 @@ if str(ROOT) not in sys.path:
 from tests.golden.make_fullsize import sha, summarize  # noqa: E402
 
 HERE = Path(__file__).resolve().parent
-EXAMPLES = ("explosion", "implosion", "shockbox", "dmr")
+EXAMPLES = ("explosion", "explosion_multi", "implosion", "shockbox", "dmr")
 MAX_STEPS = {}   # example -> step cap (none needed: every example ends within 2 000 steps)
 
 

@@ -39,7 +39,7 @@ def test_example_run_to_final_time_matches_the_oracle_bit_for_bit(name):
 
     f = _fixture(name)
     cfg, mesh, cap = ME.example(name)
-    print(f"example run: {name}, {len(mesh)} block(s) of {cfg.nx}x{cfg.ny}, "
+    print(f"example run: {name}, {len(f['blocks'])} block(s) of {cfg.nx}x{cfg.ny}, "
           f"{int(f['steps'])} steps in the oracle", flush=True)
     solver = Euler2D(cfg, mesh)
     solver.apply_initial_condition()
