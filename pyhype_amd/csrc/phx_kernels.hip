@@ -1503,6 +1503,42 @@ void release_graphs(phx_plan* P);
 constexpr int FLAT_MAX_FACES = 2048;
 int flat_threads(const phx_plan* P) { return (std::min(std::max(P->rows_b, 1), FLAT_MAX_WAVES - 1) + 1) * WAVE; }
 
+// The fused kernel's work list of one geometry kind, reordered for the end of the launch.  Workgroup
+// w runs item lo_x + (w >> 3) of XCD x = w & 7, whose items are the contiguous range
+// [n x / 8, n (x + 1) / 8) (item_range), so each XCD works through its range in list order.  Each
+// range is made the same length and its last items are split into two halves of their rows: an XCD
+// then finishes on half-size items, which levels the last round of workgroups (the ranges keep
+// their spatial order, for the L2 reuse of halo rows).  `tail` = splits per XCD; the set of cells
+// and the work per cell are unchanged.
+void split_tails(std::vector<WorkItem>& L, int tail) {
+    const int nb = (int)L.size();
+    if (tail <= 0 || nb < 8 * 4) return;
+    tail = std::min(tail, nb / 8 / 2);
+    const int T = (nb + 7) / 8 + tail;  // items per XCD after the splits
+    std::vector<WorkItem> out;
+    out.reserve((size_t)8 * T);
+    for (int x = 0; x < 8; ++x) {
+        const int lo = (int)((long)nb * x / 8), hi = (int)((long)nb * (x + 1) / 8);
+        int need = T - (hi - lo);
+        std::vector<WorkItem> head(L.begin() + lo, L.begin() + hi), back;
+        while (need > 0 && !head.empty()) {
+            const WorkItem it = head.back();
+            head.pop_back();
+            if (it.r1 - it.r0 < 2) {
+                back.push_back(it);
+                continue;
+            }
+            const int m = it.r0 + (it.r1 - it.r0) / 2;
+            back.push_back(WorkItem{it.block, it.j0, m, it.r1});
+            back.push_back(WorkItem{it.block, it.j0, it.r0, m});
+            --need;
+        }
+        out.insert(out.end(), head.begin(), head.end());
+        out.insert(out.end(), back.rbegin(), back.rend());
+    }
+    L.swap(out);
+}
+
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
@@ -1560,6 +1596,10 @@ int build_items(phx_plan* P) {
                     P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
         }
     }
+    // fused kernel, one workgroup per item: half a round of workgroups per XCD (3 waves on each of
+    // its SIMDs) ends on half-size items
+    if (P->variant == 0 && P->grid_waves == 0)
+        for (int kind = 0; kind < 2; ++kind) split_tails(P->items[kind], P->ncu * 4 * 3 / 8 / 2);
     int rc = upload_items(P->items, P->ditems, P->grid, P->grid_waves);
     if (rc) return rc;
     rc = upload_items(P->items_b, P->ditems_b, P->grid_b, P->grid_waves);
