@@ -1503,24 +1503,37 @@ void release_graphs(phx_plan* P);
 constexpr int FLAT_MAX_FACES = 2048;
 int flat_threads(const phx_plan* P) { return (std::min(std::max(P->rows_b, 1), FLAT_MAX_WAVES - 1) + 1) * WAVE; }
 
-// The fused kernel's work list of one geometry kind, reordered for the end of the launch.  Workgroup
-// w runs item lo_x + (w >> 3) of XCD x = w & 7, whose items are the contiguous range
-// [n x / 8, n (x + 1) / 8) (item_range), so each XCD works through its range in list order.  Each
-// range is made the same length and its last items are split into two halves of their rows: an XCD
-// then finishes on half-size items, which levels the last round of workgroups (the ranges keep
-// their spatial order, for the L2 reuse of halo rows).  `tail` = splits per XCD; the set of cells
-// and the work per cell are unchanged.
-void split_tails(std::vector<WorkItem>& L, int tail) {
+// The fused kernel's work list of one geometry kind, arranged for the XCDs.  Workgroup w runs item
+// lo_x + (w >> 3) of XCD x = w & 7, whose items are the contiguous range [n x / 8, n (x + 1) / 8)
+// (item_range), so each XCD works through its range in list order.
+//  * The row chunks of the blocks (all column strips of one block and one row range, a spatially
+//    coherent group whose halo rows stay in one L2) are dealt to the XCDs in turn, so every XCD
+//    gets an even share of every case: cases differ in cost per cell (uniform regions skip
+//    divisions, cases that failed are skipped), and contiguous ranges of whole cases left some
+//    XCDs with less work than others.
+//  * Every range is made the same length and its last items are split into two halves of their
+//    rows: an XCD then finishes on half-size items, which levels the last round of workgroups.
+// `tail` = splits per XCD; the set of cells and the work per cell are unchanged.
+void order_for_xcds(std::vector<WorkItem>& L, int tail) {
     const int nb = (int)L.size();
     if (tail <= 0 || nb < 8 * 4) return;
     tail = std::min(tail, nb / 8 / 2);
-    const int T = (nb + 7) / 8 + tail;  // items per XCD after the splits
+    std::vector<WorkItem> per[8];
+    for (int i = 0, g = 0; i < nb; ++g) {
+        int k = i;
+        while (k < nb && L[k].block == L[i].block && L[k].r0 == L[i].r0) ++k;
+        per[g % 8].insert(per[g % 8].end(), L.begin() + i, L.begin() + k);
+        i = k;
+    }
+    size_t most = 0;
+    for (int x = 0; x < 8; ++x) most = std::max(most, per[x].size());
+    const int T = (int)most + tail;  // items per XCD after the splits
     std::vector<WorkItem> out;
     out.reserve((size_t)8 * T);
     for (int x = 0; x < 8; ++x) {
-        const int lo = (int)((long)nb * x / 8), hi = (int)((long)nb * (x + 1) / 8);
-        int need = T - (hi - lo);
-        std::vector<WorkItem> head(L.begin() + lo, L.begin() + hi), back;
+        int need = T - (int)per[x].size();
+        std::vector<WorkItem>& head = per[x];
+        std::vector<WorkItem> back;
         while (need > 0 && !head.empty()) {
             const WorkItem it = head.back();
             head.pop_back();
@@ -1596,10 +1609,10 @@ int build_items(phx_plan* P) {
                     P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
         }
     }
-    // fused kernel, one workgroup per item: half a round of workgroups per XCD (3 waves on each of
-    // its SIMDs) ends on half-size items
+    // fused kernel, one workgroup per item: row chunks dealt to the XCDs, half a round of
+    // workgroups per XCD (3 waves on each of its SIMDs) ending on half-size items
     if (P->variant == 0 && P->grid_waves == 0)
-        for (int kind = 0; kind < 2; ++kind) split_tails(P->items[kind], P->ncu * 4 * 3 / 8 / 2);
+        for (int kind = 0; kind < 2; ++kind) order_for_xcds(P->items[kind], P->ncu * 4 * 3 / 8 / 2);
     int rc = upload_items(P->items, P->ditems, P->grid, P->grid_waves);
     if (rc) return rc;
     rc = upload_items(P->items_b, P->ditems_b, P->grid_b, P->grid_waves);
