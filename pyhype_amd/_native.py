@@ -26,7 +26,7 @@ GEOM_GENERAL, GEOM_CARTESIAN = 0, 1
 EXPORTS = (
     "phx_plan_create", "phx_plan_destroy", "phx_state_len", "phx_reset", "phx_run", "phx_run_frozen",
     "phx_run_record", "phx_stage", "phx_residual", "phx_dt", "phx_ghosts", "phx_status", "phx_residual_vjp",
-    "phx_step_vjp", "phx_set_launch", "phx_set_variant", "phx_launch_info", "phx_last_error",
+    "phx_step_vjp", "phx_set_launch", "phx_set_variant", "phx_set_xcd_order", "phx_launch_info", "phx_last_error",
     "phx_residual_vjp_noalloc", "phx_residual_jvp", "phx_step_jvp",
 )
 
@@ -87,6 +87,7 @@ def load(path: Path = LIB_PATH):
         "phx_step_vjp": ([vp, vp, vp, vp, vp], C.c_int),
         "phx_set_launch": ([vp, i32, i32, i32], C.c_int),
         "phx_set_variant": ([vp, i32, i32], C.c_int),
+        "phx_set_xcd_order": ([vp, i32], C.c_int),
         "phx_launch_info": ([vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)], C.c_int),
         "phx_last_error": ([], C.c_char_p),
     }

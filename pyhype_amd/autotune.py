@@ -14,9 +14,11 @@ variant 7: variant 6 with the reconstruction over row-major positions (62 per wa
 parameter of its own); variant 8: that reconstruction with the y-faces and update over flat cell
 ranges (``rows_b`` + 1 waves per workgroup, at least ``rows_per_item`` faces per lane).
 ``autotune`` times whole steps (hipGraph replay) for a set of candidate shapes and keeps the
-fastest; results are cached in a JSON file keyed by a signature of the workload (geometry kind,
-flux, stages, block shapes, device).  The state, time and step counter of the plan are restored
-after tuning.
+fastest; the fastest split or element-wise shape is then also timed with its work lists dealt to the
+XCDs in chunks of ``XCD_CHUNKS`` items (``xcd_chunk``; 0 = one contiguous range per XCD).  Results are
+cached in a JSON file keyed by a signature of the workload (geometry kind, flux, stages, block
+shapes, device).  The state, time and step counter of the plan are restored before every trial
+and after tuning.
 """
 from __future__ import annotations
 
@@ -114,9 +116,13 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0, 1024, 2048
     return out
 
 
+XCD_CHUNKS = (1, 4, 16)   # work-list orders tried for a split or element-wise winner
+SHAPE_KEYS = ("variant", "rows_per_item", "rows_b", "grid_waves", "waves_per_simd", "xcd_chunk")
+
+
 def _apply(plan, c):
     plan.set_launch(c["rows_per_item"], c["grid_waves"], c["waves_per_simd"], variant=c.get("variant", 0),
-                    rows_b=c.get("rows_b", 0))
+                    rows_b=c.get("rows_b", 0), xcd_chunk=c.get("xcd_chunk", 0))
 
 
 def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, path: Optional[Path] = CACHE,
@@ -133,21 +139,30 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
     _, step0, t0, _ = plan.status()
     trials, skipped = [], []
     best = None
+
+    def trial(c):
+        try:
+            _apply(plan, c)
+        except ValueError as e:   # a shape the library refuses for this mesh (e.g. LDS ranges)
+            skipped.append(dict(c, reason=str(e)))
+            return
+        plan.U.copy_(U_saved)
+        plan.reset(t0, step0)
+        ms = _time_steps(plan, steps) * 1e3
+        trials.append(dict(c, ms_per_step=ms))
+        if verbose:
+            print(f"autotune {c}: {ms:.4f} ms/step", flush=True)
+
     try:
         for c in shapes:
-            try:
-                _apply(plan, c)
-            except ValueError as e:   # a shape the library refuses for this mesh (e.g. LDS ranges)
-                skipped.append(dict(c, reason=str(e)))
-                continue
-            plan.U.copy_(U_saved)
-            plan.reset(t0, step0)
-            ms = _time_steps(plan, steps) * 1e3
-            trials.append(dict(c, ms_per_step=ms))
-            if verbose:
-                print(f"autotune {c}: {ms:.4f} ms/step", flush=True)
+            trial(c)
         if not trials:
             raise ValueError("autotune: the library refused every candidate shape")
+        split = [t for t in trials if t["variant"] != 0 and t["grid_waves"] == 0 and not t.get("xcd_chunk", 0)]
+        if split:   # the fastest split or element-wise shape with its work lists dealt to the XCDs
+            b = min(split, key=lambda t: t["ms_per_step"])
+            for k in XCD_CHUNKS:
+                trial(dict({key: b[key] for key in SHAPE_KEYS[:5]}, xcd_chunk=k))
         best = min(trials, key=lambda t: t["ms_per_step"])
         _apply(plan, best)
     finally:
@@ -157,8 +172,9 @@ def autotune(plan, steps: int = 10, shapes: Optional[Sequence[dict]] = None, pat
     result = dict(best, trials=trials, skipped=skipped)
     if path is not None:
         cache = load_cache(path)
-        cache[signature(plan)] = {k: best[k] for k in ("variant", "rows_per_item", "rows_b", "grid_waves",
-                                                          "waves_per_simd", "ms_per_step")}
+        entry = {k: best.get(k, 0) for k in SHAPE_KEYS}
+        entry["ms_per_step"] = best["ms_per_step"]
+        cache[signature(plan)] = entry
         Path(path).write_text(json.dumps(cache, indent=1, sort_keys=True))
     return result
 
@@ -170,5 +186,5 @@ def apply_cached(plan, path: Path = CACHE) -> bool:
         return False
     _apply(plan, {"rows_per_item": int(hit["rows_per_item"]), "grid_waves": int(hit["grid_waves"]),
                   "waves_per_simd": int(hit.get("waves_per_simd", 2)), "variant": int(hit.get("variant", 0)),
-                  "rows_b": int(hit.get("rows_b", 0))})
+                  "rows_b": int(hit.get("rows_b", 0)), "xcd_chunk": int(hit.get("xcd_chunk", 0))})
     return True

@@ -371,12 +371,16 @@ class DevicePlan:
                   "phx_step_vjp")
 
     def set_launch(self, rows_per_item: int, grid_waves: int = 0, waves_per_simd: int = 2,
-                   variant: Optional[int] = None, rows_b: int = 0):
+                   variant: Optional[int] = None, rows_b: int = 0, xcd_chunk: Optional[int] = None):
         """Launch shape of the stage: rows per work item, persistent grid (0 = one wave per item),
         register budget in waves per SIMD; optionally the stage variant (0 fused, 1 split, 2 shared
         halo, 3 element-wise, 4 paired, 5 split with fused y-face/update, 6 the same with one face per
         wave, 7 variant 6 with the row-major reconstruction kernel, 8 that reconstruction with the
-        y-faces and update over flat cell ranges) and the rows per item of the split variants' update kernel."""
+        y-faces and update over flat cell ranges), the rows per item of the split variants' update kernel
+        and the order of the split variants' work lists over the XCDs (0 contiguous ranges, k > 0
+        chunks of k items dealt in turn)."""
+        if xcd_chunk is not None:
+            nat.check(self.lib.phx_set_xcd_order(self.handle, int(xcd_chunk)), "phx_set_xcd_order")
         if variant is not None:
             nat.check(self.lib.phx_set_variant(self.handle, int(variant), int(rows_b)), "phx_set_variant")
         nat.check(self.lib.phx_set_launch(self.handle, int(rows_per_item), int(grid_waves), int(waves_per_simd)),
@@ -388,6 +392,8 @@ class DevicePlan:
             launch["variant"] = int(variant)
             if rows_b:
                 launch["rows_b"] = int(rows_b)
+        if xcd_chunk is not None:
+            launch["xcd_chunk"] = int(xcd_chunk)
         self.launch = launch
 
     def launch_info(self):

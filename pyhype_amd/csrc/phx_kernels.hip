@@ -1552,6 +1552,30 @@ void order_for_xcds(std::vector<WorkItem>& L, int tail) {
     L.swap(out);
 }
 
+// A work list dealt to the XCDs in chunks of `chunk` consecutive items: chunk c goes to XCD c mod 8
+// (the next XCD with room once one is full), each XCD keeping the range length item_range gives it,
+// so every XCD gets a share of every region of the mesh.  Chosen per workload by the autotuner:
+// it pays where the cost per cell varies over the mesh (uniform regions skip divisions), and costs
+// L2 reuse of the items' shared halo rows and columns where it does not.
+void deal_to_xcds(std::vector<WorkItem>& L, int chunk) {
+    const int n = (int)L.size();
+    if (chunk <= 0 || n < 8 * chunk) return;
+    int lo[8], room[8];
+    for (int x = 0; x < 8; ++x) {
+        lo[x] = (int)((long)n * x / 8);
+        room[x] = (int)((long)n * (x + 1) / 8) - lo[x];
+    }
+    std::vector<WorkItem> out(L.size());
+    int used[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0, x = 0; i < n; i += chunk, x = (x + 1) % 8)
+        for (int q = i; q < std::min(n, i + chunk); ++q) {
+            int t = x;
+            while (used[t] >= room[t]) t = (t + 1) % 8;
+            out[lo[t] + used[t]++] = L[q];
+        }
+    L.swap(out);
+}
+
 int build_items(phx_plan* P) {
     for (int kind = 0; kind < 2; ++kind) {
         P->items[kind].clear();
@@ -1613,6 +1637,13 @@ int build_items(phx_plan* P) {
     // workgroups per XCD (3 waves on each of its SIMDs) ending on half-size items
     if (P->variant == 0 && P->grid_waves == 0)
         for (int kind = 0; kind < 2; ++kind) order_for_xcds(P->items[kind], P->ncu * 4 * 3 / 8 / 2);
+    if (P->variant != 0 && P->grid_waves == 0 && P->xcd_chunk > 0)
+        for (int kind = 0; kind < 2; ++kind) {
+            deal_to_xcds(P->items[kind], P->xcd_chunk);
+            deal_to_xcds(P->items_b[kind], P->xcd_chunk);
+            deal_to_xcds(P->items_f[kind], P->xcd_chunk);
+            for (int p = 0; p < 3; ++p) deal_to_xcds(P->items_e[p][kind], P->xcd_chunk);
+        }
     int rc = upload_items(P->items, P->ditems, P->grid, P->grid_waves);
     if (rc) return rc;
     rc = upload_items(P->items_b, P->ditems_b, P->grid_b, P->grid_waves);
@@ -2377,6 +2408,13 @@ int phx_set_variant(phx_plan* P, int32_t variant, int32_t rows_b) {
     }
     P->variant = variant;
     if (rows_b > 0) P->rows_b = rows_b;
+    return build_items(P);
+}
+
+int phx_set_xcd_order(phx_plan* P, int32_t chunk) {
+    if (!P || chunk < 0) return fail(PHX_E_INVALID, "invalid XCD chunk (0 or a positive number of work items)");
+    PHX_CK(hipDeviceSynchronize());
+    P->xcd_chunk = chunk;
     return build_items(P);
 }
 

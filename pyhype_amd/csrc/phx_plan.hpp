@@ -61,6 +61,9 @@ struct AdjTile {
 struct phx_plan {
     int device = 0;
     int ncu = 0;
+    // split and element-wise stages: work items dealt to the XCDs in chunks of xcd_chunk (0: one
+    // contiguous range per XCD)
+    int xcd_chunk = 0;
     int nblocks = 0, ncases = 0, flux = 0;
     phx::Phys ph{};
     std::vector<phx_block_desc> desc;

@@ -56,7 +56,12 @@ def test_apply_cached_sets_the_cached_launch(tmp_path):
     path.write_text(json.dumps({AT.signature(plan): {"variant": 8, "rows_per_item": 2, "rows_b": 7,
                                                      "grid_waves": 0, "waves_per_simd": 3, "ms_per_step": 1.0}}))
     assert AT.apply_cached(plan, path) is True
-    assert calls == [((2, 0, 3), {"variant": 8, "rows_b": 7})]
+    assert calls == [((2, 0, 3), {"variant": 8, "rows_b": 7, "xcd_chunk": 0})]
+    calls.clear()
+    path.write_text(json.dumps({AT.signature(plan): {"variant": 7, "rows_per_item": 1, "rows_b": 3, "grid_waves": 0,
+                                                     "waves_per_simd": 3, "xcd_chunk": 4, "ms_per_step": 1.0}}))
+    assert AT.apply_cached(plan, path) is True
+    assert calls == [((1, 0, 3), {"variant": 7, "rows_b": 3, "xcd_chunk": 4})]
 
 
 def test_autotune_keeps_the_fastest_and_gauss_faces_try_split_stages_only(tmp_path, monkeypatch):
@@ -70,7 +75,7 @@ def test_autotune_keeps_the_fastest_and_gauss_faces_try_split_stages_only(tmp_pa
         state["resets"] += 1
 
     plan.reset = reset
-    plan.set_launch = lambda r, g, w, variant=0, rows_b=0: state.update(launch=(variant, r, w, rows_b))
+    plan.set_launch = lambda r, g, w, variant=0, rows_b=0, xcd_chunk=0: state.update(launch=(variant, r, w, rows_b))
     # a made-up timing: variant 5 with rows_b 2 is fastest
     monkeypatch.setattr(AT, "_time_steps", lambda p, steps: 1.0 - 0.5 * (state["launch"][0] == 5)
                         - 0.1 * (state["launch"][3] == 2))
@@ -81,6 +86,32 @@ def test_autotune_keeps_the_fastest_and_gauss_faces_try_split_stages_only(tmp_pa
     assert state["launch"][0] == 5 and state["launch"][3] == 2   # the winner is applied
     cached = json.loads(path.read_text())[AT.signature(plan)]
     assert cached["variant"] == 5 and cached["rows_b"] == 2
+
+
+def test_autotune_tries_xcd_orders_for_the_best_split_shape(tmp_path, monkeypatch):
+    """After the shape trials, the fastest split shape is timed again with its work lists dealt to
+    the XCDs in chunks of 1, 4 and 16 items, also when a fused shape won the shape trials; the
+    fastest of all trials is applied and cached with its order."""
+    plan = fake_plan([True] * 5, [(500, 500)] * 5, flux="HLLL", stages=2)
+    plan.U = torch.zeros(8, dtype=torch.float64)
+    state = {"launch": None}
+    plan.status = lambda: (None, 0, 0.0, None)
+    plan.reset = lambda t0, step0: None
+    plan.set_launch = lambda r, g, w, variant=0, rows_b=0, xcd_chunk=0: state.update(
+        launch=(variant, r, g, w, rows_b, xcd_chunk))
+    # made-up timings: the fused variant wins the shape trials (0.55), variant 7 with rows_b 3 is the
+    # best split shape (0.6) and wins overall when dealt in chunks of 4 items (0.4)
+    monkeypatch.setattr(AT, "_time_steps", lambda p, steps: 1.0 - 0.45 * (state["launch"][0] == 0)
+                        - 0.3 * (state["launch"][0] == 7) - 0.1 * (state["launch"][4] == 3)
+                        - 0.2 * (state["launch"][5] == 4))
+    path = tmp_path / "tuning.json"
+    best = AT.autotune(plan, steps=1, path=path)
+    orders = [t for t in best["trials"] if t.get("xcd_chunk")]
+    assert sorted(t["xcd_chunk"] for t in orders) == list(AT.XCD_CHUNKS)
+    assert all(t["variant"] == 7 and t["rows_b"] == 3 for t in orders)
+    assert best["variant"] == 7 and best["xcd_chunk"] == 4
+    assert state["launch"][0] == 7 and state["launch"][5] == 4          # the winner is applied
+    assert json.loads(path.read_text())[AT.signature(plan)]["xcd_chunk"] == 4
 
 
 def test_committed_cache_covers_the_tuned_workloads():
@@ -108,7 +139,7 @@ def test_autotune_skips_refused_shapes_and_always_restores_the_plan(tmp_path, mo
     plan.status = lambda: (None, 3, 0.5, None)
     plan.reset = lambda t0, step0: state["resets"].append((float(t0), step0))
 
-    def set_launch(r, g, w, variant=0, rows_b=0):
+    def set_launch(r, g, w, variant=0, rows_b=0, xcd_chunk=0):
         if variant == 8:
             raise ValueError("block too wide for the flat y-face ranges")
         state["launch"] = (variant, r, g, w, rows_b)

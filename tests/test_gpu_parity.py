@@ -285,6 +285,37 @@ def test_split_variant_matches_oracle(case, rows, rows_b, waves, variant):
     assert bitwise_equal_blocks(s.states()[0], sim.states())
 
 
+@pytest.mark.parametrize("chunk", [1, 4])
+@pytest.mark.parametrize("variant,case", [(3, "oblique"), (6, "multiblock"), (7, "oblique"), (7, "skewed"),
+                                          (8, "wide_multiblock"), (1, "wide")])
+def test_split_work_lists_dealt_to_the_xcds_match_oracle(variant, case, chunk):
+    """phx_set_xcd_order: the split and element-wise stages with their work lists dealt to the XCDs
+    in chunks of 1 and 4 items give the oracle's results bit for bit (residual and 5 steps through
+    the step graphs), as the contiguous order does."""
+    if case == "multiblock":
+        cfg, mesh = W.explosion_multiblock(30, 2, 4)
+    elif case == "oblique":
+        cfg, mesh = W.oblique_shock(40, 3)
+    elif case == "wide":
+        cfg, mesh = W.explosion_single(130)
+    elif case == "wide_multiblock":
+        cfg, mesh = W.explosion_multiblock(66, 2, 2)
+    else:
+        cfg, mesh = W.explosion_single(40)[0], skewed_mesh()
+    s = _solver(cfg, mesh, rows_per_item=1)
+    s.plan.set_launch(1, 0, 3, variant=variant, rows_b=3, xcd_chunk=chunk)
+    assert s.plan.launch_info()["xcd_chunk"] == chunk
+    st = _perturbed(s.states()[0], seed=13)
+    s.set_state([st])
+    R = s.plan.get_states(s.plan.residual())[0]
+    sim = oracle_for(cfg, mesh, st)
+    for n, b in sim.blocks.items():
+        np.testing.assert_array_equal(R[n], O.residual(b, sim.flux_fn), err_msg=f"block {n}")
+    s.run_steps(5)
+    sim.run(5)
+    assert bitwise_equal_blocks(s.states()[0], sim.states())
+
+
 @pytest.mark.parametrize("case", ["single", "multiblock", "oblique", "skewed", "wide", "wide_multiblock", "tall"])
 def test_element_wise_variant_matches_oracle(case):
     """The element-wise stage (variant 3: one lane per cell and variable for the reconstruction, one
