@@ -262,8 +262,8 @@ int phx_set_variant(phx_plan* plan, int32_t variant, int32_t rows_b);
 /* Order of the split and element-wise stages' work lists over the 8 XCDs: 0 = one contiguous range
  * of the list per XCD (the default: neighbouring items share halo rows in that XCD's L2); chunk > 0 =
  * chunks of `chunk` consecutive items dealt to the XCDs in turn (every XCD gets a share of every
- * region, which evens out meshes whose cost per cell varies).  The fused stage always deals its row
- * chunks and ends each XCD's range on half-size items.  Results are identical for every order;
+ * region, which evens out meshes whose cost per cell varies).  The fused stage always deals its items
+ * in pairs and ends each XCD's range on half-size items.  Results are identical for every order;
  * rebuilds the work lists.  Used by the autotuner.  No counterpart in the reference. */
 int phx_set_xcd_order(phx_plan* plan, int32_t chunk);
 

@@ -1506,11 +1506,11 @@ int flat_threads(const phx_plan* P) { return (std::min(std::max(P->rows_b, 1), F
 // The fused kernel's work list of one geometry kind, arranged for the XCDs.  Workgroup w runs item
 // lo_x + (w >> 3) of XCD x = w & 7, whose items are the contiguous range [n x / 8, n (x + 1) / 8)
 // (item_range), so each XCD works through its range in list order.
-//  * The row chunks of the blocks (all column strips of one block and one row range, a spatially
-//    coherent group whose halo rows stay in one L2) are dealt to the XCDs in turn, so every XCD
-//    gets an even share of every case: cases differ in cost per cell (uniform regions skip
-//    divisions, cases that failed are skipped), and contiguous ranges of whole cases left some
-//    XCDs with less work than others.
+//  * Pairs of consecutive items (neighbouring column strips, or row ranges, of one block) are dealt
+//    to the XCDs in turn, so every XCD gets an even share of every case: cases differ in cost per
+//    cell (uniform regions skip divisions, cases that failed are skipped), and contiguous ranges of
+//    whole cases left some XCDs with less work than others.  (Dealing whole row chunks, all strips
+//    of one row range, was 1 % slower on the 64-case batch; single items the same as pairs.)
 //  * Every range is made the same length and its last items are split into two halves of their
 //    rows: an XCD then finishes on half-size items, which levels the last round of workgroups.
 // `tail` = splits per XCD; the set of cells and the work per cell are unchanged.
@@ -1520,8 +1520,7 @@ void order_for_xcds(std::vector<WorkItem>& L, int tail) {
     tail = std::min(tail, nb / 8 / 2);
     std::vector<WorkItem> per[8];
     for (int i = 0, g = 0; i < nb; ++g) {
-        int k = i;
-        while (k < nb && L[k].block == L[i].block && L[k].r0 == L[i].r0) ++k;
+        const int k = std::min(nb, i + 2);
         per[g % 8].insert(per[g % 8].end(), L.begin() + i, L.begin() + k);
         i = k;
     }
@@ -1633,7 +1632,7 @@ int build_items(phx_plan* P) {
                     P->items_b[kind].push_back({b, j0, r0, std::min(r0 + P->rows_b, d.ny)});
         }
     }
-    // fused kernel, one workgroup per item: row chunks dealt to the XCDs, half a round of
+    // fused kernel, one workgroup per item: item pairs dealt to the XCDs, half a round of
     // workgroups per XCD (3 waves on each of its SIMDs) ending on half-size items
     if (P->variant == 0 && P->grid_waves == 0)
         for (int kind = 0; kind < 2; ++kind) order_for_xcds(P->items[kind], P->ncu * 4 * 3 / 8 / 2);
