@@ -1513,7 +1513,9 @@ int flat_threads(const phx_plan* P) { return (std::min(std::max(P->rows_b, 1), F
 //    of one row range, was 1 % slower on the 64-case batch; single items the same as pairs.)
 //  * Every range is made the same length and its last items are split into two halves of their
 //    rows: an XCD then finishes on half-size items, which levels the last round of workgroups.
-// `tail` = splits per XCD; the set of cells and the work per cell are unchanged.
+// `tail` = splits per XCD; the set of cells and the work per cell are unchanged.  (Should a range run
+// out of items of two or more rows before reaching the common length, the ranges of item_range
+// shift against these groups: every item still runs exactly once.)
 void order_for_xcds(std::vector<WorkItem>& L, int tail) {
     const int nb = (int)L.size();
     if (tail <= 0 || nb < 8 * 4) return;
