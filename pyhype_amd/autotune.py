@@ -116,7 +116,7 @@ def candidates(rows: Sequence[int] = ROWS, grids: Sequence[int] = (0, 1024, 2048
     return out
 
 
-XCD_CHUNKS = (1, 4, 16)   # work-list orders tried for a split or element-wise winner
+XCD_CHUNKS = (1, 4, 16, 64)   # work-list orders tried for the best split or element-wise shape
 SHAPE_KEYS = ("variant", "rows_per_item", "rows_b", "grid_waves", "waves_per_simd", "xcd_chunk")
 
 
