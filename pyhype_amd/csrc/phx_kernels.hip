@@ -1279,25 +1279,49 @@ __device__ __forceinline__ void ew_update_tile(const StageArgs& A, const WorkIte
     reduce_item(A, B, t & (WAVE - 1), lane_dt, lane_bad);  // every lane of the wave takes part
 }
 
-template <bool CART, bool GEN>
+template <int CK, bool GEN>
 __global__ void __launch_bounds__(EW_WG) k_ew_recon(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) ew_recon_tile<CART, GEN>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {  // items of both geometry kinds: the path of the item's block
+            if (A.blocks[it.block].kind == PHX_GEOM_CARTESIAN) ew_recon_tile<true, GEN>(A, it);
+            else ew_recon_tile<false, GEN>(A, it);
+        } else {
+            ew_recon_tile<CK == 1, GEN>(A, it);
+        }
+    }
 }
 
-template <bool CART, int FLUX, bool GEN>
+template <int CK, int FLUX, bool GEN>
 __global__ void __launch_bounds__(EW_WG) k_ew_face(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) ew_face_tile<CART, FLUX, GEN>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {
+            if (A.blocks[it.block].kind == PHX_GEOM_CARTESIAN) ew_face_tile<true, FLUX, GEN>(A, it);
+            else ew_face_tile<false, FLUX, GEN>(A, it);
+        } else {
+            ew_face_tile<CK == 1, FLUX, GEN>(A, it);
+        }
+    }
 }
 
-template <bool CART>
+template <int CK>
 __global__ void __launch_bounds__(EW_WG) k_ew_update(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) ew_update_tile<CART>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {
+            if (A.blocks[it.block].kind == PHX_GEOM_CARTESIAN) ew_update_tile<true>(A, it);
+            else ew_update_tile<false>(A, it);
+        } else {
+            ew_update_tile<CK == 1>(A, it);
+        }
+    }
 }
 
 // standalone CFL reduction over the owned cells of every work item
@@ -1590,6 +1614,10 @@ int build_items(phx_plan* P) {
             if (P->ditems_e[p][kind]) { (void)hipFree(P->ditems_e[p][kind]); P->ditems_e[p][kind] = nullptr; }
         }
     }
+    for (int p = 0; p < 3; ++p) {
+        if (P->ditems_em[p]) { (void)hipFree(P->ditems_em[p]); P->ditems_em[p] = nullptr; }
+        P->n_em[p] = P->grid_em[p] = 0;
+    }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
     const bool split = P->variant == 1 || P->variant >= 5;
     P->flat_q[0] = P->flat_q[1] = 0;
@@ -1655,6 +1683,19 @@ int build_items(phx_plan* P) {
         rc = upload_items(P->items_e[p], P->ditems_e[p], P->grid_e[p], P->grid_waves);
         if (rc) return rc;
     }
+    // element-wise stage with both geometry kinds: one list per phase (one launch instead of two)
+    if (P->variant == 3)
+        for (int p = 0; p < 3; ++p) {
+            if (P->items_e[p][0].empty() || P->items_e[p][1].empty()) continue;
+            std::vector<WorkItem> both(P->items_e[p][0]);
+            both.insert(both.end(), P->items_e[p][1].begin(), P->items_e[p][1].end());
+            if (P->grid_waves == 0) deal_to_xcds(both, P->xcd_chunk);
+            const size_t n = both.size();
+            PHX_CK(hipMalloc(&P->ditems_em[p], n * sizeof(WorkItem)));
+            PHX_CK(hipMemcpy(P->ditems_em[p], both.data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
+            P->n_em[p] = (int)n;
+            P->grid_em[p] = (int)((P->grid_waves > 0 && (size_t)P->grid_waves < n) ? P->grid_waves : n);
+        }
     release_graphs(P);
     return PHX_OK;
 }
@@ -1740,19 +1781,19 @@ struct k_yface_q_tag {
 struct k_ew_recon_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_ew_recon<C, G>), g, dim3(EW_WG), 0, st, A);
+        hipLaunchKernelGGL((k_ew_recon<C ? 1 : 0, G>), g, dim3(EW_WG), 0, st, A);
     }
 };
 struct k_ew_face_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_ew_face<C, F, G>), g, dim3(EW_WG), 0, st, A);
+        hipLaunchKernelGGL((k_ew_face<C ? 1 : 0, F, G>), g, dim3(EW_WG), 0, st, A);
     }
 };
 struct k_ew_update_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_ew_update<C>), g, dim3(EW_WG), 0, st, A);
+        hipLaunchKernelGGL((k_ew_update<C ? 1 : 0>), g, dim3(EW_WG), 0, st, A);
     }
 };
 struct k_yface_update_tag {
@@ -1793,6 +1834,31 @@ struct k_update_tag {
     }
 };
 
+// the generic instantiation: another limiter, primitive-variable reconstruction or Gauss faces
+bool generic_instance(const phx_plan* P) {
+    return P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN || P->ph.prim || P->ph.nq > 1;
+}
+
+// one launch of phase p (0 reconstruction, 1 faces, 2 update) of the element-wise stage over the
+// merged list of both geometry kinds
+void launch_ew_mixed(const phx_plan* P, StageArgs A, int p, hipStream_t st) {
+    A.items = P->ditems_em[p];
+    A.nitems = P->n_em[p];
+    const dim3 g(P->grid_em[p]), b(EW_WG);
+    const bool gen = generic_instance(P), roe = P->flux == PHX_FLUX_ROE;
+    if (p == 0) {
+        if (gen) hipLaunchKernelGGL((k_ew_recon<2, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_ew_recon<2, false>), g, b, 0, st, A);
+    } else if (p == 1) {
+        if (roe && gen) hipLaunchKernelGGL((k_ew_face<2, 0, true>), g, b, 0, st, A);
+        else if (roe) hipLaunchKernelGGL((k_ew_face<2, 0, false>), g, b, 0, st, A);
+        else if (gen) hipLaunchKernelGGL((k_ew_face<2, 1, true>), g, b, 0, st, A);
+        else hipLaunchKernelGGL((k_ew_face<2, 1, false>), g, b, 0, st, A);
+    } else {
+        hipLaunchKernelGGL((k_ew_update<2>), g, b, 0, st, A);
+    }
+}
+
 // one launch of kernel family T over the items of one geometry kind; minw = register budget in
 // waves per SIMD (2, 3 or 4)
 template <class T>
@@ -1802,8 +1868,7 @@ void launch_kind(const phx_plan* P, StageArgs A, int kind, const WorkItem* items
     A.nitems = nitems;
     const dim3 g(grid);
     const bool roe = P->flux == PHX_FLUX_ROE;
-    // generic instantiation: another limiter, primitive-variable reconstruction or Gauss faces
-    const bool gen = P->ph.limiter != PHX_LIMITER_VENKATAKRISHNAN || P->ph.prim || P->ph.nq > 1;
+    const bool gen = generic_instance(P);
 #define PHX_LAUNCH_M(C, F)                                                                       \
     do {                                                                                         \
         if (gen) T::template launch<C, F, 2, true>(g, st, A);                                    \
@@ -1949,18 +2014,24 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
     if (P->variant == 3) {
         A.rb = P->ew_rb;
         A.ff = P->ew_ff;
-        for_kinds(P, !P->items_e[0][0].empty(), !P->items_e[0][1].empty(), st, [&](int kind, hipStream_t s) {
-            launch_kind<k_ew_recon_tag>(P, A, kind, P->ditems_e[0][kind], (int)P->items_e[0][kind].size(),
-                                        P->grid_e[0][kind], 2, s);
-        });
-        for_kinds(P, !P->items_e[1][0].empty(), !P->items_e[1][1].empty(), st, [&](int kind, hipStream_t s) {
-            launch_kind<k_ew_face_tag>(P, A, kind, P->ditems_e[1][kind], (int)P->items_e[1][kind].size(),
-                                       P->grid_e[1][kind], 2, s);
-        });
-        for_kinds(P, !P->items_e[2][0].empty(), !P->items_e[2][1].empty(), st, [&](int kind, hipStream_t s) {
-            launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
-                                         P->grid_e[2][kind], 2, s);
-        });
+        if (P->ditems_em[0]) launch_ew_mixed(P, A, 0, st);
+        else
+            for_kinds(P, !P->items_e[0][0].empty(), !P->items_e[0][1].empty(), st, [&](int kind, hipStream_t s) {
+                launch_kind<k_ew_recon_tag>(P, A, kind, P->ditems_e[0][kind], (int)P->items_e[0][kind].size(),
+                                            P->grid_e[0][kind], 2, s);
+            });
+        if (P->ditems_em[1]) launch_ew_mixed(P, A, 1, st);
+        else
+            for_kinds(P, !P->items_e[1][0].empty(), !P->items_e[1][1].empty(), st, [&](int kind, hipStream_t s) {
+                launch_kind<k_ew_face_tag>(P, A, kind, P->ditems_e[1][kind], (int)P->items_e[1][kind].size(),
+                                           P->grid_e[1][kind], 2, s);
+            });
+        if (P->ditems_em[2]) launch_ew_mixed(P, A, 2, st);
+        else
+            for_kinds(P, !P->items_e[2][0].empty(), !P->items_e[2][1].empty(), st, [&](int kind, hipStream_t s) {
+                launch_kind<k_ew_update_tag>(P, A, kind, P->ditems_e[2][kind], (int)P->items_e[2][kind].size(),
+                                             P->grid_e[2][kind], 2, s);
+            });
         return;
     }
     if (P->variant == 4) {
@@ -2067,6 +2138,7 @@ int phx_plan_destroy(phx_plan* P) {
     void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->ditems_f[0],
                     P->ditems_f[1], P->sn, P->p1, P->yf, P->ew_rb, P->ew_ff, P->ditems_e[0][0], P->ditems_e[0][1],
                     P->ditems_e[1][0], P->ditems_e[1][1], P->ditems_e[2][0], P->ditems_e[2][1],
+                    P->ditems_em[0], P->ditems_em[1], P->ditems_em[2],
                     P->sbuf[0], P->sbuf[1], P->dt, P->t, P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -91,6 +91,10 @@ struct phx_plan {
     // element-wise variant: tiles of [0] the reconstruction (cells), [1] the faces, [2] the update (cells)
     std::vector<phx::host::WorkItem> items_e[3][2];
     phx::host::WorkItem* ditems_e[3][2] = {};
+    // element-wise stage on a mesh with both geometry kinds: each phase's items of both kinds in one
+    // list (one launch; the kernels pick the geometry path per block)
+    phx::host::WorkItem* ditems_em[3] = {};
+    int n_em[3] = {0, 0, 0}, grid_em[3] = {0, 0, 0};
     int grid_e[3][2] = {};
     double* ew_rb = nullptr;  // element-wise variant: limited face states E, W, N, S per cell (16 doubles)
     double* ew_ff = nullptr;  // element-wise variant: face fluxes (4 doubles per face, DevBlock::foff)
