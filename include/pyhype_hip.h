@@ -10,8 +10,9 @@
  * phx_plan_create/destroy.  A plan is not thread-safe; separate plans on separate streams are.
  * On a mesh with both Cartesian and general blocks, each stage phase also runs on a stream the
  * plan owns, forked from and joined back to the caller's stream with events, so the ordering seen
- * from the caller's stream is unchanged (the element-wise stage instead launches both kinds' items
- * together); two calls on one plan must not overlap in time.
+ * from the caller's stream is unchanged (the element-wise stage, and the split stages with one
+ * y-face per wave and midpoint faces, instead launch both kinds' items together); two calls on one
+ * plan must not overlap in time.
  *
  * State layout: every block is a C-ordered (ny, nx, 4) array of conservative variables
  * (rho, rho*u, rho*v, E), exactly the layout of the reference's ConservativeState.data

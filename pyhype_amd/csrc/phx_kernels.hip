@@ -1006,13 +1006,21 @@ __device__ __forceinline__ void yface_update_wg_item(const StageArgs& A, const W
     reduce_item(A, B, lane, lane_dt, lane_bad);
 }
 
-template <bool CART, int FLUX, bool GEN, bool Q>
+// CK: geometry kind of the launch's blocks, 0 general, 1 Cartesian, 2 both (the path chosen per
+// item's block; one launch over the merged list of a mesh with both kinds)
+template <int CK, int FLUX, bool GEN, bool Q>
 __global__ void __launch_bounds__((YU_MAX_ROWS + 1) * WAVE) k_yface_update_wg(StageArgs A) {
     __shared__ Q4 lds[(YU_MAX_ROWS + 1) * WAVE];
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
     for (int i = first; i < hi; i += stride) {
-        yface_update_wg_item<CART, FLUX, GEN, Q>(A, ld_const(A.items, i), lds);
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {
+            if (ld_const(A.blocks, it.block).kind == PHX_GEOM_CARTESIAN) yface_update_wg_item<true, FLUX, GEN, Q>(A, it, lds);
+            else yface_update_wg_item<false, FLUX, GEN, Q>(A, it, lds);
+        } else {
+            yface_update_wg_item<CK == 1, FLUX, GEN, Q>(A, it, lds);
+        }
         __syncthreads();  // the next item reuses the face buffer
     }
 }
@@ -1081,11 +1089,19 @@ __global__ void __launch_bounds__(WAVE) k_yface_update(StageArgs A) {
     for (int i = first; i < hi; i += stride) yface_update_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
 }
 
-template <bool CART, int FLUX, int MINW, bool GEN>
+template <int CK, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_recon_x(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) recon_x_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {
+            if (ld_const(A.blocks, it.block).kind == PHX_GEOM_CARTESIAN) recon_x_item<true, FLUX, GEN>(A, it);
+            else recon_x_item<false, FLUX, GEN>(A, it);
+        } else {
+            recon_x_item<CK == 1, FLUX, GEN>(A, it);
+        }
+    }
 }
 
 // k_recon_x_lin: the reconstruction kernel of the split stage over the row-major positions of a
@@ -1154,11 +1170,19 @@ __device__ __forceinline__ void recon_x_lin_item(const StageArgs& A, const WorkI
     }
 }
 
-template <bool CART, int FLUX, int MINW, bool GEN>
+template <int CK, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_recon_x_lin(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) recon_x_lin_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) {
+        const WorkItem it = ld_const(A.items, i);
+        if (CK == 2) {
+            if (ld_const(A.blocks, it.block).kind == PHX_GEOM_CARTESIAN) recon_x_lin_item<true, FLUX, GEN>(A, it);
+            else recon_x_lin_item<false, FLUX, GEN>(A, it);
+        } else {
+            recon_x_lin_item<CK == 1, FLUX, GEN>(A, it);
+        }
+    }
 }
 
 template <bool CART, int FLUX, bool GEN>
@@ -1617,6 +1641,8 @@ int build_items(phx_plan* P) {
     for (int p = 0; p < 3; ++p) {
         if (P->ditems_em[p]) { (void)hipFree(P->ditems_em[p]); P->ditems_em[p] = nullptr; }
         P->n_em[p] = P->grid_em[p] = 0;
+        if (P->ditems_m[p]) { (void)hipFree(P->ditems_m[p]); P->ditems_m[p] = nullptr; }
+        P->n_m[p] = P->grid_m[p] = 0;
     }
     const int rows = P->rows_per_item > 0 ? P->rows_per_item : 32;
     const bool split = P->variant == 1 || P->variant >= 5;
@@ -1683,19 +1709,32 @@ int build_items(phx_plan* P) {
         rc = upload_items(P->items_e[p], P->ditems_e[p], P->grid_e[p], P->grid_waves);
         if (rc) return rc;
     }
-    // element-wise stage with both geometry kinds: one list per phase (one launch instead of two)
+    // both geometry kinds: one list per phase of the element-wise stage and of the split stages 6
+    // and 7 with midpoint faces (one launch instead of two; the kernels choose the path per block)
+    auto merge = [&](const std::vector<WorkItem> (&two)[2], WorkItem*& dev, int& n_out, int& grid_out) -> int {
+        if (two[0].empty() || two[1].empty()) return PHX_OK;
+        std::vector<WorkItem> both(two[0]);
+        both.insert(both.end(), two[1].begin(), two[1].end());
+        if (P->grid_waves == 0) deal_to_xcds(both, P->xcd_chunk);
+        const size_t n = both.size();
+        PHX_CK(hipMalloc(&dev, n * sizeof(WorkItem)));
+        PHX_CK(hipMemcpy(dev, both.data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
+        n_out = (int)n;
+        grid_out = (int)((P->grid_waves > 0 && (size_t)P->grid_waves < n) ? P->grid_waves : n);
+        return PHX_OK;
+    };
     if (P->variant == 3)
         for (int p = 0; p < 3; ++p) {
-            if (P->items_e[p][0].empty() || P->items_e[p][1].empty()) continue;
-            std::vector<WorkItem> both(P->items_e[p][0]);
-            both.insert(both.end(), P->items_e[p][1].begin(), P->items_e[p][1].end());
-            if (P->grid_waves == 0) deal_to_xcds(both, P->xcd_chunk);
-            const size_t n = both.size();
-            PHX_CK(hipMalloc(&P->ditems_em[p], n * sizeof(WorkItem)));
-            PHX_CK(hipMemcpy(P->ditems_em[p], both.data(), n * sizeof(WorkItem), hipMemcpyHostToDevice));
-            P->n_em[p] = (int)n;
-            P->grid_em[p] = (int)((P->grid_waves > 0 && (size_t)P->grid_waves < n) ? P->grid_waves : n);
+            rc = merge(P->items_e[p], P->ditems_em[p], P->n_em[p], P->grid_em[p]);
+            if (rc) return rc;
         }
+    if ((P->variant == 6 || P->variant == 7) && P->ph.nq == 1) {
+        rc = merge(P->items_b, P->ditems_m[1], P->n_m[1], P->grid_m[1]);
+        if (rc) return rc;
+        if (P->variant == 6) rc = merge(P->items, P->ditems_m[0], P->n_m[0], P->grid_m[0]);
+        else rc = merge(P->items_f, P->ditems_m[2], P->n_m[2], P->grid_m[2]);
+        if (rc) return rc;
+    }
     release_graphs(P);
     return PHX_OK;
 }
@@ -1750,14 +1789,14 @@ struct k_recon_x_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
         constexpr int MM = G ? 3 : (M < 3 ? 3 : M);
-        hipLaunchKernelGGL((k_recon_x<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_recon_x<C ? 1 : 0, F, MM, G>), g, dim3(WAVE), 0, st, A);
     }
 };
 struct k_recon_x_lin_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
         constexpr int MM = G ? 3 : (M < 3 ? 3 : M);
-        hipLaunchKernelGGL((k_recon_x_lin<C, F, MM, G>), g, dim3(WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_recon_x_lin<C ? 1 : 0, F, MM, G>), g, dim3(WAVE), 0, st, A);
     }
 };
 struct k_yface_tag {
@@ -1811,13 +1850,13 @@ struct k_yface_update_q_tag {
 struct k_yface_update_wg_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_yface_update_wg<C, F, G, false>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_yface_update_wg<C ? 1 : 0, F, G, false>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
     }
 };
 struct k_yface_update_wg_q_tag {
     template <bool C, int F, int M, bool G>
     static void launch(dim3 g, hipStream_t st, const StageArgs& A) {
-        hipLaunchKernelGGL((k_yface_update_wg<C, F, true, true>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
+        hipLaunchKernelGGL((k_yface_update_wg<C ? 1 : 0, F, true, true>), g, dim3((A.rows_wave + 1) * WAVE), 0, st, A);
     }
 };
 struct k_yface_update_flat_tag {
@@ -1857,6 +1896,40 @@ void launch_ew_mixed(const phx_plan* P, StageArgs A, int p, hipStream_t st) {
     } else {
         hipLaunchKernelGGL((k_ew_update<2>), g, b, 0, st, A);
     }
+}
+
+// one launch of the reconstruction kernel of variant 6 (lin = false) or 7 (lin = true) over the
+// merged list of both geometry kinds
+void launch_recon_mixed(const phx_plan* P, StageArgs A, bool lin, hipStream_t st) {
+    const int L = lin ? 2 : 0;
+    A.items = P->ditems_m[L];
+    A.nitems = P->n_m[L];
+    const dim3 g(P->grid_m[L]), b(WAVE);
+    const bool gen = generic_instance(P), roe = P->flux == PHX_FLUX_ROE;
+    const bool w4 = !gen && P->waves_per_simd >= 4;  // the register budgets the per-kind launches use
+#define PHX_RX(K, F, M, G) hipLaunchKernelGGL((K<2, F, M, G>), g, b, 0, st, A)
+    if (lin) {
+        if (gen) { if (roe) PHX_RX(k_recon_x_lin, 0, 3, true); else PHX_RX(k_recon_x_lin, 1, 3, true); }
+        else if (w4) { if (roe) PHX_RX(k_recon_x_lin, 0, 4, false); else PHX_RX(k_recon_x_lin, 1, 4, false); }
+        else { if (roe) PHX_RX(k_recon_x_lin, 0, 3, false); else PHX_RX(k_recon_x_lin, 1, 3, false); }
+    } else {
+        if (gen) { if (roe) PHX_RX(k_recon_x, 0, 3, true); else PHX_RX(k_recon_x, 1, 3, true); }
+        else if (w4) { if (roe) PHX_RX(k_recon_x, 0, 4, false); else PHX_RX(k_recon_x, 1, 4, false); }
+        else { if (roe) PHX_RX(k_recon_x, 0, 3, false); else PHX_RX(k_recon_x, 1, 3, false); }
+    }
+#undef PHX_RX
+}
+
+// one launch of the y-face and update kernel of variants 6 and 7 over the merged list
+void launch_yface_wg_mixed(const phx_plan* P, StageArgs A, hipStream_t st) {
+    A.items = P->ditems_m[1];
+    A.nitems = P->n_m[1];
+    const dim3 g(P->grid_m[1]), b((A.rows_wave + 1) * WAVE);
+    const bool gen = generic_instance(P), roe = P->flux == PHX_FLUX_ROE;
+    if (roe && gen) hipLaunchKernelGGL((k_yface_update_wg<2, 0, true, false>), g, b, 0, st, A);
+    else if (roe) hipLaunchKernelGGL((k_yface_update_wg<2, 0, false, false>), g, b, 0, st, A);
+    else if (gen) hipLaunchKernelGGL((k_yface_update_wg<2, 1, true, false>), g, b, 0, st, A);
+    else hipLaunchKernelGGL((k_yface_update_wg<2, 1, false, false>), g, b, 0, st, A);
 }
 
 // one launch of kernel family T over the items of one geometry kind; minw = register budget in
@@ -1929,14 +2002,18 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
         A.sn = P->sn;
         A.p1 = P->p1;
         A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
-        for_kinds(P, !P->items_f[0].empty(), !P->items_f[1].empty(), st, [&](int kind, hipStream_t s) {
-            launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
-                                           P->grid_f[kind], P->waves_per_simd, s);
-        });
-        for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
-            launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
-                                               P->grid_b[kind], 2, s);
-        });
+        if (P->ditems_m[2]) launch_recon_mixed(P, A, true, st);
+        else
+            for_kinds(P, !P->items_f[0].empty(), !P->items_f[1].empty(), st, [&](int kind, hipStream_t s) {
+                launch_kind<k_recon_x_lin_tag>(P, A, kind, P->ditems_f[kind], (int)P->items_f[kind].size(),
+                                               P->grid_f[kind], P->waves_per_simd, s);
+            });
+        if (P->ditems_m[1]) launch_yface_wg_mixed(P, A, st);
+        else
+            for_kinds(P, !P->items_b[0].empty(), !P->items_b[1].empty(), st, [&](int kind, hipStream_t s) {
+                launch_kind<k_yface_update_wg_tag>(P, A, kind, P->ditems_b[kind], (int)P->items_b[kind].size(),
+                                                   P->grid_b[kind], 2, s);
+            });
         return;
     }
     if (P->variant == 6) {
@@ -1944,6 +2021,11 @@ void launch_stage(phx_plan* P, StageArgs A, hipStream_t st) {
         A.p1 = P->p1;
         A.rows_wave = std::min(std::max(P->rows_b, 1), YU_MAX_ROWS);
         const bool gauss = P->ph.nq > 1;
+        if (!gauss && P->ditems_m[0] && P->ditems_m[1]) {
+            launch_recon_mixed(P, A, false, st);
+            launch_yface_wg_mixed(P, A, st);
+            return;
+        }
         for_kinds(P, !P->items[0].empty(), !P->items[1].empty(), st, [&](int kind, hipStream_t s) {
             if (gauss)
                 launch_kind<k_recon_x_q_tag>(P, A, kind, P->ditems[kind], (int)P->items[kind].size(),
@@ -2138,7 +2220,8 @@ int phx_plan_destroy(phx_plan* P) {
     void* ptrs[] = {P->dblocks, P->dgeom, P->ditems[0], P->ditems[1], P->ditems_b[0], P->ditems_b[1], P->ditems_f[0],
                     P->ditems_f[1], P->sn, P->p1, P->yf, P->ew_rb, P->ew_ff, P->ditems_e[0][0], P->ditems_e[0][1],
                     P->ditems_e[1][0], P->ditems_e[1][1], P->ditems_e[2][0], P->ditems_e[2][1],
-                    P->ditems_em[0], P->ditems_em[1], P->ditems_em[2],
+                    P->ditems_em[0], P->ditems_em[1], P->ditems_em[2], P->ditems_m[0], P->ditems_m[1],
+                    P->ditems_m[2],
                     P->sbuf[0], P->sbuf[1], P->dt, P->t, P->tfinal, P->dtmin, P->step, P->bad, P->dghost_off};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

@@ -95,6 +95,10 @@ struct phx_plan {
     // list (one launch; the kernels pick the geometry path per block)
     phx::host::WorkItem* ditems_em[3] = {};
     int n_em[3] = {0, 0, 0}, grid_em[3] = {0, 0, 0};
+    // split stages 6 and 7 likewise: merged lists of items (variant 6 reconstruction), items_b (y-faces
+    // and update) and items_f (variant 7 reconstruction)
+    phx::host::WorkItem* ditems_m[3] = {};
+    int n_m[3] = {0, 0, 0}, grid_m[3] = {0, 0, 0};
     int grid_e[3][2] = {};
     double* ew_rb = nullptr;  // element-wise variant: limited face states E, W, N, S per cell (16 doubles)
     double* ew_ff = nullptr;  // element-wise variant: face fluxes (4 doubles per face, DevBlock::foff)
