@@ -72,13 +72,28 @@ __host__ __device__ inline T ld_const(const T* p, long i) {
 // ---------------------------------------------------------------------------------------------
 // geometry accessors (layouts documented in include/pyhype_hip.h)
 // ---------------------------------------------------------------------------------------------
-struct CartG {
+// Geometry element i through the constant address space (CG = true; nothing writes the geometry
+// while a kernel runs): wave-uniform reads become scalar loads, and the compiler may move the loads
+// across the kernel's stores (e.g. out of a row loop).  CG = false: a global load.
+template <bool CG>
+__host__ __device__ inline double gldg(const double* p, long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (CG) return ((const __attribute__((address_space(4))) double*)p)[i];
+#endif
+    return gld(p, i);
+}
+
+template <bool CG = false>
+struct CartGT {
     const double* g;
     int nx, ny;
-    __host__ __device__ double col(int q, int j) const { return gld(g, 8 + q * nx + j); }
-    __host__ __device__ double row(int q, int k) const { return gld(g, 8 + 11 * nx + q * ny + k); }
-    __host__ __device__ double A(int k, int j) const { return gld(g, 8 + 11 * nx + 11 * ny + (long)k * nx + j); }
+    __host__ __device__ double col(int q, int j) const { return gldg<CG>(g, 8 + q * nx + j); }
+    __host__ __device__ double row(int q, int k) const { return gldg<CG>(g, 8 + 11 * nx + q * ny + k); }
+    __host__ __device__ double A(int k, int j) const {
+        return gldg<CG>(g, 8 + 11 * nx + 11 * ny + (long)k * nx + j);
+    }
 };
+using CartG = CartGT<false>;
 enum { C_xlN, C_xlS, C_ylN, C_ylS, C_LN, C_LS, C_dxE, C_dxW, C_dxN, C_dxS, C_dx };
 enum { R_xlE, R_xlW, R_ylE, R_ylW, R_LE, R_LW, R_dyE, R_dyW, R_dyN, R_dyS, R_dy };
 
@@ -95,6 +110,41 @@ struct GenG {
 };
 enum { F_L, F_xl, F_yl, F_xq, F_yq, F_c, F_s };
 enum { CL_A, CL_xc, CL_yc, CL_dx, CL_dy };
+
+// Column arrays of a Cartesian block at column j (they do not depend on the row): loaded once per
+// work item by kernels that walk the rows of a strip.
+struct CartCols {
+    double xlN, xlS, ylN, ylS, dxE, dxW, dxN, dxS, LN, LS, dx;
+};
+template <bool CG>
+__host__ __device__ inline CartCols cart_cols(const DevBlock& B, int j) {
+    const CartGT<CG> G{B.geom, B.nx, B.ny};
+    CartCols c;
+    c.xlN = G.col(C_xlN, j); c.xlS = G.col(C_xlS, j);
+    c.ylN = G.col(C_ylN, j); c.ylS = G.col(C_ylS, j);
+    c.dxE = G.col(C_dxE, j); c.dxW = G.col(C_dxW, j);
+    c.dxN = G.col(C_dxN, j); c.dxS = G.col(C_dxS, j);
+    c.LN = G.col(C_LN, j); c.LS = G.col(C_LS, j);
+    c.dx = G.col(C_dx, j);
+    return c;
+}
+
+// cell_geo of a Cartesian cell from its column arrays, the row arrays of row k and its area Ac
+template <bool CG>
+__host__ __device__ inline phx::CellGeo cart_cell_geo(const DevBlock& B, const CartCols& c, int k, double Ac) {
+    const CartGT<CG> G{B.geom, B.nx, B.ny};
+    phx::CellGeo g;
+    g.xlE = G.row(R_xlE, k); g.xlW = G.row(R_xlW, k);
+    g.ylE = G.row(R_ylE, k); g.ylW = G.row(R_ylW, k);
+    g.xlN = c.xlN; g.xlS = c.xlS;
+    g.ylN = c.ylN; g.ylS = c.ylS;
+    g.ainv = 1.0 / Ac;
+    g.dxE = c.dxE; g.dxW = c.dxW;
+    g.dxN = c.dxN; g.dxS = c.dxS;
+    g.dyE = G.row(R_dyE, k); g.dyW = G.row(R_dyW, k);
+    g.dyN = G.row(R_dyN, k); g.dyS = G.row(R_dyS, k);
+    return g;
+}
 
 template <bool CART>
 __host__ __device__ inline void wall_cs(const DevBlock& B, int side, int k, int j, double& c, double& s) {

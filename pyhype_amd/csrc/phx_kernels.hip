@@ -107,24 +107,44 @@ __device__ __forceinline__ Q4 shfl_down1(const Q4& q) {
     return r;
 }
 
+// Outputs of the stage program are written in chunks of EMIT_CHUNK: the chunk's sources are loaded
+// together before its stores (an output's source is never a buffer an earlier output of the stage
+// writes), and the chunk loop is not unrolled, so the outputs' pointers and coefficients are read
+// from the kernel arguments per chunk instead of occupying scalar registers for all MAX_OUT slots.
+constexpr int EMIT_CHUNK = 3;
+
+// Sources of the first output chunk of owned cell (kk, j), loaded ahead of the row's arithmetic so
+// that their latency overlaps it.  Only this lane writes these cells in this stage (after it has
+// read them), so the early read returns what emit_row would read.
+__device__ __forceinline__ void prefetch_sources(const StageArgs& A, const DevBlock& B, int kk, int j,
+                                                 Q4 (&pre)[EMIT_CHUNK]) {
+    const long idx = B.off + 4 * ((long)kk * B.nx + j);
+#pragma unroll
+    for (int q = 0; q < EMIT_CHUNK; ++q)
+        if (q < A.nout) pre[q] = ld4(A.src[q] + idx);
+}
+
 // Residual of owned cell (kk, j) -> every Butcher output of the stage program
 // (explicit_runge_kutta.py:63-80); on the last stage also the cell's CFL quantity and realizability.
-template <bool CART>
+// PF: the first chunk's sources were loaded by prefetch_sources into pre; on a Cartesian block dxc
+// is the cell's column width and the row height a scalar load.
+template <bool CART, bool PF = false>
 __device__ __forceinline__ void emit_row(const StageArgs& A, const DevBlock& B, int kk, int j, double dt, const Q4& Res,
-                                         double& lane_dt, bool& lane_bad) {
+                                         double& lane_dt, bool& lane_bad, const Q4* pre = nullptr,
+                                         double dxc = 0.0) {
     const long idx = B.off + 4 * ((long)kk * B.nx + j);
     if (A.Rout) st4(A.Rout + idx, Res);
-    // Outputs in chunks of EMIT_CHUNK: the chunk's sources are loaded together before its stores (an
-    // output's source is never a buffer an earlier output of the stage writes), and the chunk loop
-    // is not unrolled, so the outputs' pointers and coefficients are read from the kernel arguments
-    // per chunk instead of occupying scalar registers for all MAX_OUT slots.
-    constexpr int EMIT_CHUNK = 3;
 #pragma unroll 1
     for (int m0 = 0; m0 < A.nout; m0 += EMIT_CHUNK) {
         Q4 s[EMIT_CHUNK];
+        if (PF && m0 == 0) {
 #pragma unroll
-        for (int q = 0; q < EMIT_CHUNK; ++q)
-            if (m0 + q < A.nout) s[q] = ld4(A.src[m0 + q] + idx);
+            for (int q = 0; q < EMIT_CHUNK; ++q) s[q] = pre[q];
+        } else {
+#pragma unroll
+            for (int q = 0; q < EMIT_CHUNK; ++q)
+                if (m0 + q < A.nout) s[q] = ld4(A.src[m0 + q] + idx);
+        }
 #pragma unroll
         for (int q = 0; q < EMIT_CHUNK; ++q) {
             const int m = m0 + q;
@@ -141,7 +161,12 @@ __device__ __forceinline__ void emit_row(const StageArgs& A, const DevBlock& B, 
             st4(A.dst[m] + idx, o);
             if (m == A.state_out && A.dtmin != nullptr) {
                 double dx, dy;
-                cell_dxdy<CART>(B, kk, j, dx, dy);
+                if (PF && CART) {
+                    dx = dxc;
+                    dy = CartGT<true>{B.geom, B.nx, B.ny}.row(R_dy, kk);
+                } else {
+                    cell_dxdy<CART>(B, kk, j, dx, dy);
+                }
                 const double t = phx::cell_dt(o, dx, dy, A.ph);
                 if (!(o.v[0] > 0.0) || !(o.v[3] > 0.0) || !(t == t)) lane_bad = true;
                 else lane_dt = fmin(lane_dt, t);
@@ -169,9 +194,16 @@ __device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& 
 // ---------------------------------------------------------------------------------------------
 // one work item of one RK stage
 // ---------------------------------------------------------------------------------------------
-template <bool CART, int FLUX, bool GEN>
+// PF (the instances with a budget of 2 waves per SIMD, which have registers to spare): the block
+// descriptor is read through the constant address space (scalar loads), and each row iteration
+// issues its loads in the order they are needed, before its arithmetic: the row's geometry and face
+// lengths, then the next row's states and the sources of the row it emits.  Vector-memory waits
+// are in issue order, so the wait for the geometry leaves the later loads in flight.  On Cartesian
+// blocks the column arrays are loaded once per item and the cell areas one row ahead, so a row
+// iteration waits for no geometry load.
+template <bool CART, int FLUX, bool GEN, bool PF = false>
 __device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it) {
-    const DevBlock& B = A.blocks[it.block];
+    const DevBlock& B = PF ? const_ref(A.blocks, it.block) : A.blocks[it.block];
     // a case that became unrealizable is frozen: the reference aborts that run (Euler2D.py:144-152)
     if (A.skip_dead && A.bad[B.case_id] < *A.step) return;
     const int lane = threadIdx.x & (WAVE - 1);
@@ -198,23 +230,49 @@ __device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it
     double lane_dt = __longlong_as_double((long long)DT_INF_BITS);
     bool lane_bad = false;
 
+    // Cartesian blocks: the column arrays once per item, the cell areas one row ahead, the row
+    // arrays as scalar loads
+    const CartGT<true> CG{B.geom, nx, ny};
+    CartCols cols{};
+    double Acur = 1.0, Anext = 1.0;
+    if (PF && CART && inner) {
+        cols = cart_cols<true>(B, j);
+        Acur = CG.A(kr0, j);
+    }
+
+    Q4 pre[EMIT_CHUNK];
     for (int k = kr0; k <= kr1; ++k) {
         const bool real = (k < ny);
+        const bool xrow = (k >= it.r0) && (k < it.r1);
+        phx::CellGeo g;
+        double LW = 0.0, LE = 0.0, LS = 0.0, LN = 0.0, Ac = 1.0;
+        if (PF && CART) {
+            if (real && recon) g = cart_cell_geo<true>(B, cols, k, Acur);
+            if (xrow && owner) {
+                LW = CG.row(R_LW, k); LE = CG.row(R_LE, k);
+                LS = cols.LS; LN = cols.LN;
+                Ac = Acur;
+            }
+        } else if (PF) {
+            if (real && recon) g = cell_geo<CART>(B, k, j);
+            if (xrow && owner) cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+        }
         if (real) {
             if (inner) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
             else if (gcol && k + 1 <= kr1 && k + 1 < ny) Up = fetch_r<CART, GEN>(A.blocks, B, A.Uin, k + 1, j, ph);
         }
+        if (PF && owner && k - 1 >= it.r0) prefetch_sources(A, B, k - 1, j, pre);
+        if (PF && CART && inner && k + 1 < ny) Anext = CG.A(k + 1, j);
         const Q4 Uw = shfl_up1(Uc);
         const Q4 Ue = shfl_down1(Uc);
         phx::Recon r;
         r.E = r.W = r.N = r.S = phx::q4zero();
         if (real && recon) {
-            const phx::CellGeo g = cell_geo<CART>(B, k, j);
+            if (!PF) g = cell_geo<CART>(B, k, j);
             r = phx::reconstruct<GEN>(Uc, Uw, Ue, Um, Up, g, ph.limiter);
         }
         const Q4 qsend = (j == -1) ? Uc : r.E;
         const Q4 QEl = shfl_up1(qsend);  // limited east state of cell j-1 (ghost value for j == 0)
-        const bool xrow = (k >= it.r0) && (k < it.r1);
         Q4 FW = phx::q4zero();
         if (xrow && xface) {
             const Q4 L = (j == 0) ? outer_state<CART>(B, PHX_W, r.W, QEl, k, j) : QEl;
@@ -234,11 +292,10 @@ __device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it
                 Q4 Res;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) Res.v[c] = (0.5 * (part.v[c] - prevLN * (2.0 * F.v[c]))) / prevA;
-                emit_row<CART>(A, B, kk, j, dt, Res, lane_dt, lane_bad);
+                emit_row<CART, PF>(A, B, kk, j, dt, Res, lane_dt, lane_bad, pre, cols.dx);
             }
             if (xrow) {
-                double LW, LE, LS, LN, Ac;
-                cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
+                if (!PF) cell_L<CART>(B, k, j, LW, LE, LS, LN, Ac);
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     part.v[c] = (LW * (2.0 * FW.v[c]) - LE * (2.0 * FE.v[c])) + LS * (2.0 * F.v[c]);
@@ -249,6 +306,7 @@ __device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it
         QNprev = r.N;
         Um = Uc;
         Uc = Up;
+        Acur = Anext;
     }
 
     reduce_item(A, B, lane, lane_dt, lane_bad);
@@ -276,7 +334,7 @@ template <bool CART, int FLUX, int MINW, bool GEN>
 __global__ void __launch_bounds__(WAVE, MINW) k_stage(StageArgs A) {
     int lo, hi, first, stride;
     item_range(A.nitems, lo, hi, first, stride);
-    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX, GEN>(A, ld_const(A.items, i));
+    for (int i = first; i < hi; i += stride) stage_item<CART, FLUX, GEN, MINW == 2>(A, ld_const(A.items, i));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -465,7 +523,6 @@ __device__ __forceinline__ void emit_half(const StageArgs& A, const DevBlock& B,
                                           const double res[2], double o[2]) {
     const long idx = B.off + 4 * ((long)kk * B.nx + j) + 2 * H;
     if (A.Rout) *reinterpret_cast<double2*>(A.Rout + idx) = make_double2(res[0], res[1]);
-    constexpr int EMIT_CHUNK = 3;
 #pragma unroll 1
     for (int m0 = 0; m0 < A.nout; m0 += EMIT_CHUNK) {
         double2 s[EMIT_CHUNK];
