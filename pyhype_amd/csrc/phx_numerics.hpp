@@ -615,6 +615,23 @@ __host__ __device__ inline T np_minimum(T a, T b) {
     return a_nan ? a : r;
 }
 
+// np.minimum.reduce of the four faces' limiter values in the order E, W, N, S (the first NaN if
+// any).  Venkatakrishnan values (GEN = false) are never -0: the numerator s*s + 2*s is +0 where it
+// sums to zero (a zero sum of two doubles rounds to +0), the denominator is positive, and the
+// flat-face value is 3/4.  Without a NaN the chain's result is then the minimum, which fmin returns
+// with the same bits; a lane with a NaN takes the chain (a branch that waves without one skip).
+template <bool GEN, class T>
+__host__ __device__ inline T limiter_min4(T pE, T pW, T pN, T pS) {
+    if constexpr (!GEN && std::is_same<T, double>::value) {
+        const bool any_nan = (pE != pE) | (pW != pW) | (pN != pN) | (pS != pS);
+        T m = fmin(fmin(pE, pW), fmin(pN, pS));
+        if (any_nan) m = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+        return m;
+    } else {
+        return np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+    }
+}
+
 // Slope of one face, SlopeLimiter._compute_slope (limiters/base.py:189-221): dmax/d for d > 0,
 // dmin/d for d < 0, 1 otherwise.
 template <class T>
@@ -692,7 +709,7 @@ __host__ __device__ inline void reconstruct_var(T u, T uw, T ue, T us, T un, con
     const T pW = limit_face<GEN>(dmax, dmin, (u + hW) - u, lim);
     const T pN = limit_face<GEN>(dmax, dmin, (u + hN) - u, lim);
     const T pS = limit_face<GEN>(dmax, dmin, (u + hS) - u, lim);
-    T phi = np_minimum(np_minimum(np_minimum(pE, pW), pN), pS);
+    T phi = limiter_min4<GEN>(pE, pW, pN, pS);
     phi = (phi < 0.0) ? T(0.0) : phi;                  // np.where(phi < 0, 0, phi)
     qE = u + phi * hE;
     qW = u + phi * hW;

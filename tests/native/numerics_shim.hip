@@ -25,6 +25,18 @@ void shim_cons_to_prim(int n, const double* U, double* W, const double* phys) {
         W[4 * i] = w.r; W[4 * i + 1] = w.u; W[4 * i + 2] = w.v; W[4 * i + 3] = w.p;
     }
 }
+// limiter value of each face (n x 4: E, W, N, S) -> their minimum: the Venkatakrishnan path
+// (limiter_min4<false>) and the np.minimum chain (limiter_min4<true>)
+void shim_limiter_min4(int n, const double* p, double* fast, double* chain) {
+    for (int i = 0; i < n; ++i) {
+        const double* q = p + 4 * i;
+        fast[i] = limiter_min4<false>(q[0], q[1], q[2], q[3]);
+        chain[i] = limiter_min4<true>(q[0], q[1], q[2], q[3]);
+    }
+}
+void shim_venkat_face(int n, const double* dmax, const double* dmin, const double* d, double* out) {
+    for (int i = 0; i < n; ++i) out[i] = venkat_face(dmax[i], dmin[i], d[i]);
+}
 void shim_reflect(int n, const double* U, const double* c, const double* s, double* out) {
     for (int i = 0; i < n; ++i) put(out + 4 * i, reflect(q4(U + 4 * i), c[i], s[i]));
 }

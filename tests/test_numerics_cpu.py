@@ -247,3 +247,35 @@ def test_roe_flat_faces_bitwise(numerics_shim):
     with np.errstate(all="ignore"):
         ref = O.flux_roe(WL.reshape(1, n, 4), WR.reshape(1, n, 4)).reshape(n, 4)
     assert np.array_equal(F.view(np.int64), ref.view(np.int64))
+
+
+def test_venkatakrishnan_face_minimum_equals_the_np_minimum_chain(numerics_shim):
+    """The four faces' limiter values reduce with np.minimum in the order E, W, N, S
+    (limiters/base.py:47-187).  The Venkatakrishnan path takes fmin where no value is NaN, which is
+    exact because its values are never -0; here the values come from the limiter itself (random
+    slopes, slopes near -2 and 0, the flat-face 3/4, overflowing slopes that give NaN), and the
+    result must equal the chain and numpy's reduction bit for bit."""
+    rng = np.random.default_rng(11)
+    n = 200_000
+    d = rng.standard_normal(n) * np.exp(rng.uniform(-40, 40, n))
+    d[rng.random(n) < 0.1] = 0.0                              # flat faces: 3/4
+    dmax = np.abs(rng.standard_normal(n)) * np.exp(rng.uniform(-40, 40, n))
+    dmin = -np.abs(rng.standard_normal(n)) * np.exp(rng.uniform(-40, 40, n))
+    dmax[rng.random(n) < 0.1] = 0.0                           # local extrema: s = +-0
+    dmin[rng.random(n) < 0.1] = -0.0
+    k = n // 20
+    dmin[:k] = -2.0 * np.abs(d[:k])                           # s = -2 exactly where d > 0 is not taken
+    dmax[k:2 * k] = 1e200
+    d[k:2 * k] = 1e-200                                       # s overflows: s*s = inf -> NaN
+    vals = np.empty(n)
+    numerics_shim.shim_venkat_face(n, dp(dmax), dp(dmin), dp(d), dp(vals))
+    assert np.isnan(vals).any() and (vals == 0.75).any() and (vals == 0.0).any()
+    assert not np.any((vals == 0.0) & np.signbit(vals)), "a Venkatakrishnan value was -0"
+    faces = vals[rng.integers(0, n, (n, 4))]
+    fast, chain = np.empty(n), np.empty(n)
+    numerics_shim.shim_limiter_min4(n, dp(faces), dp(fast), dp(chain))
+    ref = np.minimum(np.minimum(np.minimum(faces[:, 0], faces[:, 1]), faces[:, 2]), faces[:, 3])
+    assert np.isnan(ref).any()
+    assert np.array_equal(fast.view(np.uint64), chain.view(np.uint64))
+    same = (fast.view(np.uint64) == ref.view(np.uint64)) | (np.isnan(fast) & np.isnan(ref))
+    assert same.all()
