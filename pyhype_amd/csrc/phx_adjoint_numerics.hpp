@@ -61,11 +61,9 @@ __host__ __device__ inline void prim_cons_vjp(const Prim& W, const double* bU, c
 // Adjoint of roe_flux (flux/Roe.py:262-304 as restated in roe_flux): lamF -> (bL, bR).
 __host__ __device__ inline void roe_flux_vjp(const Prim& L, const Prim& R, const Q4& lamF, const Phys& ph,
                                              PrimBar& bL, PrimBar& bR) {
-    const double g = ph.gamma, k1 = ph.one_over_gm1, gm = ph.g_over_gm1;
-    // ---------------- primal recompute
-    const double ekL = (0.5 * L.r) * (L.u * L.u + L.v * L.v);
-    const double ekR = (0.5 * R.r) * (R.u * R.u + R.v * R.v);
-    const double ruL = L.r * L.u, ruR = R.r * R.u;
+    const double g = ph.gamma, gm = ph.g_over_gm1;
+    // ---------------- primal recompute (the dissipation's intermediates; the central part 0.5*(F_L + F_R)
+    // has its own adjoint below)
     const double sL = sqrt(L.r), sR = sqrt(R.r);
     const double inv = 1.0 / (sL + sR);
     const double rs = sqrt(L.r * R.r);
