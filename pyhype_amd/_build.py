@@ -52,5 +52,20 @@ def build_library(verbose: bool = False, extra_flags=None) -> Path:
     return LIB
 
 
+def build_test_library(verbose: bool = False) -> Path:
+    """Compile the GPU tests' device numerics library (tests/native/libdevice_numerics.so)."""
+    src = PKG.parent / "tests" / "native" / "device_numerics.hip"
+    out = src.with_name("libdevice_numerics.so")
+    tmp = out.with_suffix(".so.tmp")
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", str(src), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
