@@ -199,8 +199,8 @@ __device__ __forceinline__ void reduce_item(const StageArgs& A, const DevBlock& 
 // issues its loads in the order they are needed, before its arithmetic: the row's geometry and face
 // lengths, then the next row's states and the sources of the row it emits.  Vector-memory waits
 // are in issue order, so the wait for the geometry leaves the later loads in flight.  On Cartesian
-// blocks the column arrays are loaded once per item and the cell areas one row ahead, so a row
-// iteration waits for no geometry load.
+// blocks the column arrays are loaded once per item and the cell areas one row ahead, so the only
+// geometry a row iteration waits for was requested an iteration earlier.
 template <bool CART, int FLUX, bool GEN, bool PF = false>
 __device__ __forceinline__ void stage_item(const StageArgs& A, const WorkItem it) {
     const DevBlock& B = PF ? const_ref(A.blocks, it.block) : A.blocks[it.block];
